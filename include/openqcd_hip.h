@@ -1,0 +1,215 @@
+/*
+ * openqcd_hip.h -- C ABI of libopenqcd_hip.so, an MI355X (gfx950) build of
+ * openQCD-2.4.2's Wilson-clover Dirac operator, SW term and SAP+GCR solver.
+ *
+ * The functions in part B keep the names, argument meaning and error
+ * behaviour of the reference routines they replace (file:line of the
+ * reference prototype given with each group).  Global state (lattice
+ * geometry, gauge field, SW field, parameters) is held by the library, as in
+ * the reference; part A sets it, replacing the compile-time geometry of
+ * include/global.h and the parameter data base of modules/flags.
+ *
+ * Field pointers.  Spinor fields passed to part B are either
+ *   (1) device fields obtained from oqh_alloc_s/oqh_alloc_sd or the workspace
+ *       calls alloc_ws/reserve_ws (alloc_wsd/reserve_wsd); pointer arithmetic
+ *       in units of spinor(_dble) (e.g. psd[0]+VOLUME/2) addresses the sites
+ *       in the reference's order, exactly as in host memory; or
+ *   (2) ordinary host arrays in the reference layout (NSPIN elements, site
+ *       order ipt); these are copied to the device and back around the call.
+ * Device fields are stored as structure-of-arrays in 64-site blocks; their
+ * contents are read and written with oqh_import_* / oqh_export_*.
+ *
+ * Errors (argument, state, allocation) print a message and abort the
+ * process, as error() does in the reference (utils/error.c), unless
+ * oqh_set_error_mode(0) was called; then the call returns and
+ * oqh_last_error() reports the failure.
+ */
+#ifndef OPENQCD_HIP_H
+#define OPENQCD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- data types, layout-identical to include/su3.h:33-121 ---- */
+#ifndef SU3_H
+typedef struct { float re, im; } complex;
+typedef struct { complex c1, c2, c3; } su3_vector;
+typedef struct { complex c11, c12, c13, c21, c22, c23, c31, c32, c33; } su3;
+typedef struct { su3_vector c1, c2; } weyl;
+typedef struct { su3_vector c1, c2, c3, c4; } spinor;
+typedef struct { float u[36]; } pauli;
+typedef struct { double re, im; } complex_dble;
+typedef struct { complex_dble c1, c2, c3; } su3_vector_dble;
+typedef struct { complex_dble c11, c12, c13, c21, c22, c23, c31, c32, c33; } su3_dble;
+typedef struct { su3_vector_dble c1, c2; } weyl_dble;
+typedef struct { su3_vector_dble c1, c2, c3, c4; } spinor_dble;
+typedef struct { double u[36]; } pauli_dble;
+typedef struct { double q[2]; } qflt;
+typedef struct { qflt re, im; } complex_qflt;
+#endif
+#ifndef UTILS_H
+/* include/utils.h:65-68 */
+typedef enum { ALL_PTS, EVEN_PTS, ODD_PTS, NO_PTS, PT_SETS } ptset_t;
+#endif
+#ifndef BLOCK_H
+/* include/block.h:48-52 */
+typedef enum { SAP_BLOCKS, DFL_BLOCKS, TEST_BLOCKS, BLK_GRIDS } blk_grid_t;
+#endif
+
+/* ======================= A. library state (no reference counterpart
+   as functions; they replace include/global.h and modules/flags) ====== */
+
+/* select the HIP device and create the library stream */
+int oqh_init(int device);
+void oqh_finalize(void);
+void oqh_set_error_mode(int abort_on_error);
+int oqh_last_error(char *msg, int len);   /* returns error count since last call */
+
+/* lattice and thread-block sizes of include/global.h:18-36 (NPROC=1);
+   builds the site order ipt of lattice/geometry.c */
+int oqh_geometry(const int *L, const int *Ltrd);
+int oqh_volume(void);
+void oqh_get_ipt(int *ipt);
+
+/* parameter data base, flags/lat_parms.c:516 (set_bc_parms: type, cF, cF'),
+   lat_parms.c:920 (set_sw_parms m0, with csw of set_lat_parms),
+   lat_parms.c:963 (set_tm_parms), flags/sap_parms.c:112 (set_sap_parms) */
+void oqh_set_bc_parms(int type, double cF, double cF_prime);
+void oqh_set_sw_parms(double m0, double csw);
+void oqh_set_tm_parms(int eoflg);
+void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
+
+/* gauge field in the layout of udfld() (uflds/uflds.c:113-156; 8 links per
+   odd point, README.global).  Import invalidates the SW term and the
+   single-precision copy. */
+void oqh_import_ud(const su3_dble *ud);
+void oqh_export_ud(su3_dble *ud);
+/* SW field in the layout of swdfld() (2 pauli_dble per point) */
+void oqh_export_swd(pauli_dble *swd);
+void oqh_import_swd(const pauli_dble *swd);
+
+/* device-resident spinor fields (n sites each, n = volume if n <= 0) */
+spinor *oqh_alloc_s(int n);
+spinor_dble *oqh_alloc_sd(int n);
+void oqh_free(void *field);
+void oqh_import_s(spinor *dev, const spinor *host, int n);
+void oqh_export_s(const spinor *dev, spinor *host, int n);
+void oqh_import_sd(spinor_dble *dev, const spinor_dble *host, int n);
+void oqh_export_sd(const spinor_dble *dev, spinor_dble *host, int n);
+void oqh_sync(void);
+
+/* timing of ncall applications of an operator on device fields, measured
+   with HIP events on the library stream; op: 0 Dw_dble, 1 Dw, 2 Dwhat_dble,
+   3 Dwhat.  Returns the average milliseconds per call. */
+double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall);
+
+/* ======================= B. reference entry points ====================== */
+
+/* include/dirac.h:28-53 (dirac/Dw.c, dirac/Dw_dble.c) */
+void Dw(float mu, spinor *s, spinor *r);
+void Dwee(float mu, spinor *s, spinor *r);
+void Dwoo(float mu, spinor *s, spinor *r);
+void Dweo(spinor *s, spinor *r);
+void Dwoe(spinor *s, spinor *r);
+void Dwhat(float mu, spinor *s, spinor *r);
+void Dw_dble(double mu, spinor_dble *s, spinor_dble *r);
+void Dwee_dble(double mu, spinor_dble *s, spinor_dble *r);
+void Dwoo_dble(double mu, spinor_dble *s, spinor_dble *r);
+void Dweo_dble(spinor_dble *s, spinor_dble *r);
+void Dwoe_dble(spinor_dble *s, spinor_dble *r);
+void Dwhat_dble(double mu, spinor_dble *s, spinor_dble *r);
+
+/* include/sw_term.h:77-82 (sw_term/sw_term.c:399, swflds.c:129) */
+int sw_term(ptset_t set);
+void assign_swd2sw(void);
+/* include/uflds.h:59 (uflds/uflds.c:479) */
+void assign_ud2u(void);
+
+/* include/linalg.h:70-79 (linalg/salg.c:1223-1429) */
+complex spinor_prod(int vol, int icom, spinor *s, spinor *r);
+float spinor_prod_re(int vol, int icom, spinor *s, spinor *r);
+float norm_square(int vol, int icom, spinor *s);
+void mulc_spinor_add(int vol, int icom, spinor *s, spinor *r, complex z);
+void mulr_spinor_add(int vol, int icom, spinor *s, spinor *r, float c);
+void scale(int vol, int icom, float c, spinor *s);
+void project(int vol, int icom, spinor *s, spinor *r);
+float normalize(int vol, int icom, spinor *s);
+void mulg5(int vol, int icom, spinor *s);
+void mulmg5(int vol, int icom, spinor *s);
+
+/* include/linalg.h:82-99 (linalg/salg_dble.c:2611-2891) */
+complex_qflt spinor_prod_dble(int vol, int icom, spinor_dble *s, spinor_dble *r);
+qflt spinor_prod_re_dble(int vol, int icom, spinor_dble *s, spinor_dble *r);
+complex_qflt spinor_prod5_dble(int vol, int icom, spinor_dble *s, spinor_dble *r);
+qflt norm_square_dble(int vol, int icom, spinor_dble *s);
+void mulc_spinor_add_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, complex_dble z);
+void mulr_spinor_add_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, double c);
+void combine_spinor_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, double cs, double cr);
+void scale_dble(int vol, int icom, double c, spinor_dble *s);
+void project_dble(int vol, int icom, spinor_dble *s, spinor_dble *r);
+double normalize_dble(int vol, int icom, spinor_dble *s);
+void mulg5_dble(int vol, int icom, spinor_dble *s);
+void mulmg5_dble(int vol, int icom, spinor_dble *s);
+
+/* include/utils.h:122-126 (utils/qsum.c:262-424); host arithmetic */
+void acc_qflt(double u, double *qr);
+void add_qflt(double *qu, double *qv, double *qr);
+void scl_qflt(double u, double *qr);
+void mul_qflt(double *qu, double *qv, double *qr);
+void global_qsum(int n, double **qu, double **qr);
+
+/* include/sflds.h:37-61 (sflds/sflds.c:1203-1405, sflds/unorm.c:261-314) */
+void set_s2zero(int vol, int icom, spinor *s);
+void set_sd2zero(int vol, int icom, spinor_dble *sd);
+void assign_s2s(int vol, int icom, spinor *s, spinor *r);
+void assign_s2sd(int vol, int icom, spinor *s, spinor_dble *rd);
+void assign_sd2s(int vol, int icom, spinor_dble *sd, spinor *r);
+void assign_sd2sd(int vol, int icom, spinor_dble *sd, spinor_dble *rd);
+void add_s2s(int vol, int icom, spinor *s, spinor *r);
+void add_s2sd(int vol, int icom, spinor *s, spinor_dble *rd);
+void diff_s2s(int vol, int icom, spinor *s, spinor *r);
+void diff_sd2s(int vol, int icom, spinor_dble *sd, spinor_dble *rd, spinor *r);
+float unorm(int vol, int icom, spinor *s);
+double unorm_dble(int vol, int icom, spinor_dble *sd);
+
+/* include/utils.h:143-152 (utils/wspace.c:394-510): workspace fields are
+   device fields */
+void alloc_ws(int n);
+spinor **reserve_ws(int n);
+int release_ws(void);
+void alloc_wsd(int n);
+spinor_dble **reserve_wsd(int n);
+int release_wsd(void);
+
+/* include/block.h:62-75 (block/blk_grid.c:139, map_u2blk.c:308,
+   map_sw2blk.c:102, map_s2blk.c:551-737), SAP_BLOCKS only */
+void alloc_bgr(blk_grid_t grid);
+void assign_ud2ubgr(blk_grid_t grid);
+int assign_swd2swbgr(blk_grid_t grid, ptset_t set);
+void assign_s2sblk(blk_grid_t grid, int n, ptset_t set, spinor *s, int k);
+void assign_sblk2s(blk_grid_t grid, int n, ptset_t set, int k, spinor *s);
+/* include/dirac.h:34-39 (dirac/Dw.c:1475-1907) */
+void Dw_blk(blk_grid_t grid, int n, float mu, int k, int l);
+void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l);
+
+/* include/sap.h:21-35 (sap/blk_solv.c:1117-1186, sap/sap.c:720,
+   sap/sap_gcr.c:140) */
+void blk_mres(int n, float mu, int nmr);
+void blk_eo_mres(int n, float mu, int nmr);
+void sap(float mu, int isolv, int nmr, spinor *psi, spinor *eta);
+double sap_gcr(int nkv, int nmx, int istop, double res, double mu,
+               spinor_dble *eta, spinor_dble *psi, int *ifail, int *status);
+
+/* include/linsolv.h:34-37 (linsolv/fgcr.c:215).  Dop and Mop are host
+   callbacks that receive the library's device fields (the workspace ws/wsd
+   must be device fields). */
+double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
+            void (*Mop)(int k, spinor *rho, spinor *phi, spinor *chi),
+            spinor **ws, spinor_dble **wsd, int nkv, int nmx, int istop,
+            double res, spinor_dble *eta, spinor_dble *psi, int *status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

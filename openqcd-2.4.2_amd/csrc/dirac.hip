@@ -1,0 +1,509 @@
+// dirac.hip -- O(a)-improved Wilson-Dirac operator on gfx950.
+//
+// Replaces dirac/Dw_dble.c:1377-1888 and dirac/Dw.c:973-1474.  The reference
+// visits the odd points, gathers the hopping terms from the even neighbours
+// (doe) and scatters the conjugate terms to them (deo), with a barrier
+// schedule over 16 sub-blocks to keep the scatter race free.  Here every
+// output point gathers its eight hopping terms itself, so no point is written
+// by two work-items and no atomics or barriers are needed:
+//
+//   (D s)(x) = A(x) s(x) - 1/2 sum_mu [ (1-gamma_mu) U(x,mu) s(x+mu)
+//                                      + (1+gamma_mu) U(x-mu,mu)^dag s(x-mu) ]
+//
+// Each (1 -+ gamma_mu) projection is applied before the colour product, so
+// that only two colour vectors are multiplied per direction (the spin
+// structure is that of the reference's doe/deo, Dw_dble.c:1114-1373).  A
+// work-group of 256 handles the 128 even and the 128 odd points with the same
+// offsets in the two halves of the site order; these lie in the same cache
+// block, so the links and spinors shared by them are re-read from L2.
+//
+// Boundary rules (open bc): odd points on the boundary time slices are zeroed
+// in input and output of Dw and contribute no hopping terms to even points
+// (Dw_dble.c:1425-1456); the neighbour table carries that flag in bit 31.
+#include <cstdio>
+
+#include "oqh_internal.h"
+
+namespace oqh {
+
+enum DiracOp { OP_DW = 0, OP_DOE, OP_DEO, OP_HAT1, OP_HAT2, OP_DIAG };
+
+template <typename T> struct DArgs
+{
+   typedef typename cplx_of<T>::t C;
+   const C *s;     // input field
+   C *r;           // output field
+   C *s_w;         // input field, writable (zeroing side effects) or null
+   const C *u;     // links, 72 units per odd point
+   const C *sw;    // Pauli matrices, 36 units per point
+   C *t;           // odd intermediate of the Schur complement
+   const int *nbr; // [8][V]
+   const unsigned char *bnd;
+   int V, h;
+   int p0, np;     // parity ops: first point and number of points
+   T mu, muo, coef;
+   int masked;     // 1 if boundary points exist (bc != 3)
+};
+
+template <typename C> __device__ inline void load_spinor(const C *f, int y, C *s)
+{
+   const C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++) s[c] = p[c * SB];
+}
+
+template <typename C> __device__ inline void store_spinor(C *f, int y, const C *s)
+{
+   C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c * SB] = s[c];
+}
+
+template <typename C> __device__ inline void zero_spinor(C *f, int y)
+{
+   C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+   C z = cmk<C>(0, 0);
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c * SB] = z;
+}
+
+// c = U h  or  c = U^dag h  (U: 9 units starting at up, stride SB)
+template <typename C, bool DAG> __device__ inline void su3_apply(const C *up, const C *h, C *c)
+{
+   C u[9];
+#pragma unroll
+   for (int k = 0; k < 9; k++) u[k] = up[k * SB];
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      if (!DAG)
+         c[i] = cadd(cadd(cmul(u[3 * i], h[0]), cmul(u[3 * i + 1], h[1])), cmul(u[3 * i + 2], h[2]));
+      else
+         c[i] = cadd(cadd(cmulc(u[i], h[0]), cmulc(u[3 + i], h[1])), cmulc(u[6 + i], h[2]));
+   }
+}
+
+// one direction mu, forward (FWD) or backward: project s, multiply, expand
+template <typename C, int MU, bool FWD>
+__device__ inline void hop_one(const C *sp, const C *up, C *acc)
+{
+   C h0[3], h1[3], c0[3], c1[3];
+   // spin components s_a are sp[3a..3a+2]
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      C s0 = sp[i], s1 = sp[3 + i], s2 = sp[6 + i], s3 = sp[9 + i];
+      if (MU == 0) { h0[i] = FWD ? cadd(s0, s2) : csub(s0, s2); h1[i] = FWD ? cadd(s1, s3) : csub(s1, s3); }
+      if (MU == 1) { h0[i] = FWD ? caddi(s0, s3) : csubi(s0, s3); h1[i] = FWD ? caddi(s1, s2) : csubi(s1, s2); }
+      if (MU == 2) { h0[i] = FWD ? cadd(s0, s3) : csub(s0, s3); h1[i] = FWD ? csub(s1, s2) : cadd(s1, s2); }
+      if (MU == 3) { h0[i] = FWD ? caddi(s0, s2) : csubi(s0, s2); h1[i] = FWD ? csubi(s1, s3) : caddi(s1, s3); }
+   }
+   su3_apply<C, !FWD>(up, h0, c0);
+   su3_apply<C, !FWD>(up, h1, c1);
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      acc[i] = cadd(acc[i], c0[i]);
+      acc[3 + i] = cadd(acc[3 + i], c1[i]);
+      if (MU == 0)
+      {
+         acc[6 + i] = FWD ? cadd(acc[6 + i], c0[i]) : csub(acc[6 + i], c0[i]);
+         acc[9 + i] = FWD ? cadd(acc[9 + i], c1[i]) : csub(acc[9 + i], c1[i]);
+      }
+      if (MU == 1)
+      {
+         acc[6 + i] = FWD ? csubi(acc[6 + i], c1[i]) : caddi(acc[6 + i], c1[i]);
+         acc[9 + i] = FWD ? csubi(acc[9 + i], c0[i]) : caddi(acc[9 + i], c0[i]);
+      }
+      if (MU == 2)
+      {
+         acc[6 + i] = FWD ? csub(acc[6 + i], c1[i]) : cadd(acc[6 + i], c1[i]);
+         acc[9 + i] = FWD ? cadd(acc[9 + i], c0[i]) : csub(acc[9 + i], c0[i]);
+      }
+      if (MU == 3)
+      {
+         acc[6 + i] = FWD ? csubi(acc[6 + i], c0[i]) : caddi(acc[6 + i], c0[i]);
+         acc[9 + i] = FWD ? caddi(acc[9 + i], c1[i]) : csubi(acc[9 + i], c1[i]);
+      }
+   }
+}
+
+// address of link l (0..7, udfld order) stored with odd point j = y - V/2
+template <typename C> __device__ inline const C *link_ptr(const C *u, int j, int l)
+{
+   return u + (size_t)(j >> 6) * (72 * SB) + (j & (SB - 1)) + (size_t)(9 * l) * SB;
+}
+
+template <typename T, int MU>
+__device__ inline void hop_mu(const DArgs<T> &a, const typename cplx_of<T>::t *in, int x, int odd,
+                              bool mask, typename cplx_of<T>::t *acc)
+{
+   typedef typename cplx_of<T>::t C;
+   C sp[12];
+   // x + mu
+   int n = a.nbr[(size_t)(2 * MU) * a.V + x];
+   if (!(mask && n < 0))
+   {
+      int y = n & 0x7fffffff;
+      load_spinor(in, y, sp);
+      const C *up = odd ? link_ptr(a.u, x - a.h, 2 * MU) : link_ptr(a.u, y - a.h, 2 * MU + 1);
+      hop_one<C, MU, true>(sp, up, acc);
+   }
+   // x - mu
+   n = a.nbr[(size_t)(2 * MU + 1) * a.V + x];
+   if (!(mask && n < 0))
+   {
+      int y = n & 0x7fffffff;
+      load_spinor(in, y, sp);
+      const C *up = odd ? link_ptr(a.u, x - a.h, 2 * MU + 1) : link_ptr(a.u, y - a.h, 2 * MU);
+      hop_one<C, MU, false>(sp, up, acc);
+   }
+}
+
+template <typename T>
+__device__ inline void hop_all(const DArgs<T> &a, const typename cplx_of<T>::t *in, int x, int odd,
+                               bool mask, typename cplx_of<T>::t *acc)
+{
+   hop_mu<T, 0>(a, in, x, odd, mask, acc);
+   hop_mu<T, 1>(a, in, x, odd, mask, acc);
+   hop_mu<T, 2>(a, in, x, odd, mask, acc);
+   hop_mu<T, 3>(a, in, x, odd, mask, acc);
+}
+
+// out = (A + i mu) v on one Weyl half (6 complex); m points at unit 0 of the
+// half (stride SB); units 0-2 hold the real diagonal, 3-17 the upper
+// off-diagonal entries (0,1),(0,2),...,(4,5)
+template <typename C>
+__device__ inline void pauli_half(const C *m, decltype(C::x) mu, const C *v, C *out)
+{
+   C d0 = m[0], d1 = m[SB], d2 = m[2 * SB];
+   decltype(C::x) d[6] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y};
+#pragma unroll
+   for (int i = 0; i < 6; i++) out[i] = cmk<C>(d[i] * v[i].x - mu * v[i].y, d[i] * v[i].y + mu * v[i].x);
+   int p = 3;
+#pragma unroll
+   for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int j = i + 1; j < 6; j++)
+      {
+         C hij = m[p * SB];
+         out[i] = cadd(out[i], cmul(hij, v[j]));
+         out[j] = cadd(out[j], cmulc(hij, v[i]));
+         p++;
+      }
+}
+
+template <typename C>
+__device__ inline void pauli_site(const C *sw, int x, decltype(C::x) mu, const C *v, C *out)
+{
+   const C *m = sw + (size_t)(x >> 6) * (36 * SB) + (x & (SB - 1));
+   pauli_half(m, mu, v, out);
+   pauli_half(m + 18 * SB, -mu, v + 6, out + 6);
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_dirac(DArgs<T> a)
+{
+   typedef typename cplx_of<T>::t C;
+   int x, odd;
+   if (OP == OP_DW)
+   {
+      int tid = threadIdx.x;
+      int k = blockIdx.x * 128 + (tid & 127);
+      if (k >= a.h) return;
+      odd = tid >> 7;
+      x = k + odd * a.h;
+   }
+   else
+   {
+      int k = blockIdx.x * 256 + threadIdx.x;
+      if (k >= a.np) return;
+      x = a.p0 + k;
+      odd = x >= a.h;
+   }
+
+   C acc[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) acc[c] = cmk<C>(0, 0);
+
+   if (OP == OP_DW)
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.r, x);
+         zero_spinor(a.s_w, x);
+         return;
+      }
+      hop_all(a, a.s, x, odd, a.masked != 0, acc);
+      C v[12], o[12];
+      load_spinor(a.s, x, v);
+      pauli_site(a.sw, x, odd ? a.muo : a.mu, v, o);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], a.coef));
+      store_spinor(a.r, x, o);
+   }
+   else if (OP == OP_DOE)
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.r, x);
+         return;
+      }
+      hop_all(a, a.s, x, 1, a.masked != 0, acc);
+#pragma unroll
+      for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], a.coef);
+      store_spinor(a.r, x, acc);
+   }
+   else if (OP == OP_DEO)
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.r, x);
+         return;
+      }
+      hop_all(a, a.s, x, 0, a.masked != 0, acc);
+      C o[12];
+      load_spinor((const C *)a.r, x, o);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], a.coef));
+      store_spinor(a.r, x, o);
+   }
+   else if (OP == OP_HAT1)
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.t, x);
+         return;
+      }
+      hop_all(a, a.s, x, 1, a.masked != 0, acc);
+      C o[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], a.coef);
+      pauli_site(a.sw, x, (T)0, acc, o);
+      store_spinor(a.t, x, o);
+   }
+   else if (OP == OP_HAT2)
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.r, x);
+         return;
+      }
+      hop_all(a, (const C *)a.t, x, 0, a.masked != 0, acc);
+      C v[12], o[12];
+      load_spinor(a.s, x, v);
+      pauli_site(a.sw, x, a.mu, v, o);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], a.coef));
+      store_spinor(a.r, x, o);
+   }
+   else  // OP_DIAG
+   {
+      if (a.masked && a.bnd[x])
+      {
+         zero_spinor(a.r, x);
+         if (a.s_w) zero_spinor(a.s_w, x);
+         return;
+      }
+      C v[12], o[12];
+      load_spinor(a.s, x, v);
+      pauli_site(a.sw, x, a.mu, v, o);
+      store_spinor(a.r, x, o);
+   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_bnd(typename cplx_of<T>::t *s, const unsigned char *bnd,
+                                                  int p0, int np)
+{
+   int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= np) return;
+   int x = p0 + k;
+   if (bnd[x]) zero_spinor(s, x);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+template <typename T> static DArgs<T> base_args()
+{
+   Lib &L = lib();
+   DArgs<T> a{};
+   a.V = L.g.V;
+   a.h = L.g.V / 2;
+   a.nbr = L.d_nbr;
+   a.bnd = L.d_bnd;
+   a.masked = (L.bc != 3);
+   a.u = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_ud : (const void *)L.d_u);
+   a.sw = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_swd : (const void *)L.d_sw);
+   return a;
+}
+
+template <typename T, int OP> static void launch(const DArgs<T> &a)
+{
+   Lib &L = lib();
+   unsigned nb = (OP == OP_DW) ? (unsigned)((a.h + 127) / 128) : (unsigned)((a.np + 255) / 256);
+   if (nb == 0) return;
+   k_dirac<T, OP><<<nb, 256, 0, L.stream>>>(a);
+}
+
+template <typename T> static void zero_bnd(typename cplx_of<T>::t *s, int p0, int np)
+{
+   Lib &L = lib();
+   if (L.bc == 3 || np <= 0) return;
+   k_zero_bnd<T><<<(np + 255) / 256, 256, 0, L.stream>>>(s, L.d_bnd, p0, np);
+}
+
+// state needed by an operator.  As in the reference, the SW field is used in
+// whatever state sw_term() left it (the Schur complement expects the odd part
+// inverted); it is computed here only if it was never computed for the
+// current gauge field and parameters.
+static bool prepare(const char *where, int prec)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return false;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set (oqh_import_ud)");
+      return false;
+   }
+   if (!L.swd_valid) ensure_swd(0, 0);
+   if (prec == F32)
+   {
+      ensure_u();
+      ensure_sw();
+   }
+   return true;
+}
+
+enum { K_DW, K_DEE, K_DOO, K_DOE, K_DEO, K_HAT };
+
+template <typename T> static void apply(int kind, T mu, void *sp, void *rp, const char *where)
+{
+   typedef typename cplx_of<T>::t C;
+   Lib &L = lib();
+   int prec = sizeof(T) == 8 ? F64 : F32;
+   if (!prepare(where, prec)) return;
+   int V = L.g.V, h = V / 2;
+   if ((kind == K_DW || kind == K_HAT) && sp == rp)
+   {
+      fail(where, "Input and output fields must be different");
+      return;
+   }
+   bool rin = (kind == K_DEO);  // output is read (accumulated into)
+   Res s = resolve(sp, prec, V, 0, true);
+   Res r = resolve(rp, prec, V, 1, rin || kind != K_DW);
+   if (!s.dev || !r.dev) return;
+   if (s.off != 0 || r.off != 0)
+   {
+      fail(where, "the operators act on full fields (NSPIN points)");
+      return;
+   }
+   DArgs<T> a = base_args<T>();
+   a.s = (const C *)s.dev;
+   a.r = (C *)r.dev;
+   a.s_w = (C *)s.dev;
+   T muo = L.eoflg ? (T)0 : mu;
+   switch (kind)
+   {
+   case K_DW:
+      a.mu = mu;
+      a.muo = muo;
+      a.coef = (T)-0.5;
+      launch<T, OP_DW>(a);
+      break;
+   case K_DEE:
+      a.mu = mu;
+      a.p0 = 0;
+      a.np = h;
+      launch<T, OP_DIAG>(a);
+      break;
+   case K_DOO:
+      a.mu = muo;
+      a.p0 = h;
+      a.np = h;
+      launch<T, OP_DIAG>(a);
+      break;
+   case K_DOE:
+      a.coef = (T)-0.5;
+      a.p0 = h;
+      a.np = h;
+      launch<T, OP_DOE>(a);
+      zero_bnd<T>((C *)s.dev, 0, h);   // cps(d)_int_bnd side effect
+      break;
+   case K_DEO:
+      a.coef = (T)0.5;
+      a.p0 = 0;
+      a.np = h;
+      launch<T, OP_DEO>(a);
+      zero_bnd<T>((C *)s.dev, h, h);
+      break;
+   case K_HAT:
+   {
+      // odd intermediate t = A_oo^-1 (-1/2 hop(s_e)) in a scratch field
+      a.t = (C *)tmp_buffer(units_for(V, 12) * sizeof(C));
+      a.coef = (T)-0.5;
+      a.p0 = h;
+      a.np = h;
+      launch<T, OP_HAT1>(a);
+      a.coef = (T)0.5;
+      a.mu = mu;
+      a.p0 = 0;
+      a.np = h;
+      launch<T, OP_HAT2>(a);
+      zero_bnd<T>((C *)s.dev, 0, h);   // cps(d)_int_bnd side effect
+      break;
+   }
+   }
+   // inputs may have been zeroed on boundary points, as in the reference
+   finish(s, prec, V, true);
+   finish(r, prec, V, true);
+}
+
+}  // namespace oqh
+
+using namespace oqh;
+
+extern "C" void Dw_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_DW, mu, s, r, "Dw_dble [Dw_dble.c]"); }
+extern "C" void Dwee_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_DEE, mu, s, r, "Dwee_dble [Dw_dble.c]"); }
+extern "C" void Dwoo_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_DOO, mu, s, r, "Dwoo_dble [Dw_dble.c]"); }
+extern "C" void Dwoe_dble(spinor_dble *s, spinor_dble *r) { apply<double>(K_DOE, 0.0, s, r, "Dwoe_dble [Dw_dble.c]"); }
+extern "C" void Dweo_dble(spinor_dble *s, spinor_dble *r) { apply<double>(K_DEO, 0.0, s, r, "Dweo_dble [Dw_dble.c]"); }
+extern "C" void Dwhat_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_HAT, mu, s, r, "Dwhat_dble [Dw_dble.c]"); }
+extern "C" void Dw(float mu, spinor *s, spinor *r) { apply<float>(K_DW, mu, s, r, "Dw [Dw.c]"); }
+extern "C" void Dwee(float mu, spinor *s, spinor *r) { apply<float>(K_DEE, mu, s, r, "Dwee [Dw.c]"); }
+extern "C" void Dwoo(float mu, spinor *s, spinor *r) { apply<float>(K_DOO, mu, s, r, "Dwoo [Dw.c]"); }
+extern "C" void Dwoe(spinor *s, spinor *r) { apply<float>(K_DOE, 0.0f, s, r, "Dwoe [Dw.c]"); }
+extern "C" void Dweo(spinor *s, spinor *r) { apply<float>(K_DEO, 0.0f, s, r, "Dweo [Dw.c]"); }
+extern "C" void Dwhat(float mu, spinor *s, spinor *r) { apply<float>(K_HAT, mu, s, r, "Dwhat [Dw.c]"); }
+
+extern "C" double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall)
+{
+   Lib &L = lib();
+   if (ncall <= 0) return 0.0;
+   // warm state (SW term, single-precision copies) outside the timed region
+   if (op == 0) Dw_dble(mu, (spinor_dble *)s, (spinor_dble *)r);
+   else if (op == 1) Dw((float)mu, (spinor *)s, (spinor *)r);
+   else if (op == 2) Dwhat_dble(mu, (spinor_dble *)s, (spinor_dble *)r);
+   else Dwhat((float)mu, (spinor *)s, (spinor *)r);
+   HIP_OK(hipStreamSynchronize(L.stream));
+   hipEvent_t e0, e1;
+   HIP_OK(hipEventCreate(&e0));
+   HIP_OK(hipEventCreate(&e1));
+   HIP_OK(hipEventRecord(e0, L.stream));
+   for (int k = 0; k < ncall; k++)
+   {
+      if (op == 0) Dw_dble(mu, (spinor_dble *)s, (spinor_dble *)r);
+      else if (op == 1) Dw((float)mu, (spinor *)s, (spinor *)r);
+      else if (op == 2) Dwhat_dble(mu, (spinor_dble *)s, (spinor_dble *)r);
+      else Dwhat((float)mu, (spinor *)s, (spinor *)r);
+   }
+   HIP_OK(hipEventRecord(e1, L.stream));
+   HIP_OK(hipEventSynchronize(e1));
+   float ms = 0.0f;
+   HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+   HIP_OK(hipEventDestroy(e0));
+   HIP_OK(hipEventDestroy(e1));
+   return (double)ms / ncall;
+}

@@ -1,0 +1,330 @@
+"""ctypes binding of the CPU oracle (liboracle.so).  Test infrastructure only:
+imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+
+Fields are numpy arrays in the reference's formats and site order:
+  spinor_dble -> float64 [V, 24], spinor -> float32 [V, 24]
+  gauge (ud)  -> float64 [4V, 18] (8 links per odd site, README.global)
+  pauli_dble  -> float64 [2V, 36]
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        i, d, f = C.c_int, C.c_double, C.c_float
+        sig = {
+            "oc_create": (P, [P, P]), "oc_destroy": (None, [P]),
+            "oc_volume": (i, [P]), "oc_nthread": (i, [P]),
+            "oc_get_ipt": (None, [P, P]), "oc_get_iupdn": (None, [P, P, P]),
+            "oc_set_bc": (None, [P, i, d, d]), "oc_set_sw": (None, [P, d, d]),
+            "oc_set_tm": (None, [P, i]), "oc_set_sap": (None, [P, P, i, i, i]),
+            "oc_start_ranlux": (None, [P, i, i]),
+            "oc_kat_ranlxs": (None, [i, i, P, i]), "oc_kat_ranlxd": (None, [i, i, P, i]),
+            "oc_random_ud": (None, [P]), "oc_set_ud_phase": (None, [P]),
+            "oc_udfld": (P, [P]), "oc_set_ud": (None, [P, P]),
+            "oc_assign_ud2u": (None, [P]), "oc_ufld": (P, [P]),
+            "oc_random_sd": (None, [P, P, d]), "oc_random_s": (None, [P, P, f]),
+            "oc_bnd_sd2zero": (None, [P, i, P]), "oc_bnd_s2zero": (None, [P, i, P]),
+            "oc_unorm_dble": (d, [i, P]), "oc_unorm": (f, [i, P]),
+            "oc_ftensor": (None, [P, P]), "oc_sw_term": (i, [P, i]),
+            "oc_swdfld": (P, [P]), "oc_assign_swd2sw": (None, [P]), "oc_swfld": (P, [P]),
+            "oc_mul_pauli_dble": (None, [d, P, P, P]), "oc_mul_pauli": (None, [f, P, P, P]),
+            "oc_inv_pauli_dble": (i, [d, P, P]),
+            "oc_apply_sw_dble": (None, [i, d, P, P, P]),
+            "oc_acc_qflt": (None, [d, P]), "oc_add_qflt": (None, [P, P, P]),
+            "oc_spinor_prod_dble": (None, [i, P, P, P, P]),
+            "oc_norm_square_dble": (None, [i, P, P]),
+            "oc_spinor_prod": (None, [i, P, P, P]), "oc_norm_square": (f, [i, P]),
+            "oc_sap_nblocks": (i, [P]), "oc_sap_block_info": (None, [P, i, P, P]),
+            "oc_sap_block_imb": (None, [P, i, P]), "oc_sap_prepare": (i, [P]),
+            "oc_Dw_blk": (None, [P, i, f, P, P]), "oc_Dwhat_blk": (None, [P, i, f, P, P]),
+            "oc_blk_mres": (None, [P, i, f, i, P, P]),
+            "oc_sap": (None, [P, f, i, i, P, P]),
+            "oc_sap_gcr": (d, [P, i, i, i, d, d, P, P, P, P]),
+        }
+        for op in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
+            sig["oc_" + op] = (None, [P, d, P, P])
+        for op in ("Dwoe_dble", "Dweo_dble"):
+            sig["oc_" + op] = (None, [P, P, P])
+        for op in ("Dw", "Dwee", "Dwoo", "Dwhat"):
+            sig["oc_" + op] = (None, [P, f, P, P])
+        for op in ("Dwoe", "Dweo"):
+            sig["oc_" + op] = (None, [P, P, P])
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def iarr(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+class Oracle:
+    """One lattice + parameter set of the oracle."""
+
+    def __init__(self, L, Ltrd=None, bc=3, cF=(1.0, 1.0), m0=0.0, csw=1.0, eoflg=0):
+        self.L = tuple(int(x) for x in L)
+        self.Ltrd = tuple(int(x) for x in (Ltrd or L))
+        self._L = iarr(self.L)
+        self._T = iarr(self.Ltrd)
+        self.h = lib().oc_create(ptr(self._L), ptr(self._T))
+        if not self.h:
+            raise ValueError("illegal lattice geometry")
+        self.V = lib().oc_volume(self.h)
+        self.nthread = lib().oc_nthread(self.h)
+        self.bc = bc
+        lib().oc_set_bc(self.h, bc, cF[0], cF[1])
+        lib().oc_set_sw(self.h, m0, csw)
+        lib().oc_set_tm(self.h, eoflg)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_destroy(self.h)
+            self.h = None
+
+    # geometry
+    def ipt(self):
+        a = np.empty(self.V, np.int32)
+        lib().oc_get_ipt(self.h, ptr(a))
+        return a
+
+    def iupdn(self):
+        a = np.empty((self.V, 4), np.int32)
+        b = np.empty((self.V, 4), np.int32)
+        lib().oc_get_iupdn(self.h, ptr(a), ptr(b))
+        return a, b
+
+    # fields
+    def start_ranlux(self, level, seed):
+        lib().oc_start_ranlux(self.h, level, seed)
+
+    def random_ud(self):
+        lib().oc_random_ud(self.h)
+
+    def set_ud_phase(self):
+        lib().oc_set_ud_phase(self.h)
+
+    def ud(self):
+        p = lib().oc_udfld(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), (4 * self.V, 18)).copy()
+
+    def set_ud(self, ud):
+        ud = np.ascontiguousarray(ud, np.float64)
+        lib().oc_set_ud(self.h, ptr(ud))
+
+    def assign_ud2u(self):
+        lib().oc_assign_ud2u(self.h)
+
+    def random_sd(self, sigma=1.0):
+        a = np.empty((self.V, 24), np.float64)
+        lib().oc_random_sd(self.h, ptr(a), sigma)
+        return a
+
+    def random_s(self, sigma=1.0):
+        a = np.empty((self.V, 24), np.float32)
+        lib().oc_random_s(self.h, ptr(a), sigma)
+        return a
+
+    def bnd_sd2zero(self, sd, pset=0):
+        lib().oc_bnd_sd2zero(self.h, pset, ptr(sd))
+
+    def bnd_s2zero(self, s, pset=0):
+        lib().oc_bnd_s2zero(self.h, pset, ptr(s))
+
+    def ftensor(self):
+        a = np.empty((6, self.V, 9), np.float64)
+        lib().oc_ftensor(self.h, ptr(a))
+        return a
+
+    def sw_term(self, pset):
+        return lib().oc_sw_term(self.h, pset)
+
+    def swd(self):
+        p = lib().oc_swdfld(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), (2 * self.V, 36)).copy()
+
+    def assign_swd2sw(self):
+        lib().oc_assign_swd2sw(self.h)
+
+    # operators (inputs may be modified, as in the reference)
+    def _op_d(self, name, mu, s, r=None):
+        if r is None:
+            r = np.zeros((self.V, 24), np.float64)
+        fn = getattr(lib(), "oc_" + name)
+        if mu is None:
+            fn(self.h, ptr(s), ptr(r))
+        else:
+            fn(self.h, mu, ptr(s), ptr(r))
+        return r
+
+    def _op_f(self, name, mu, s, r=None):
+        if r is None:
+            r = np.zeros((self.V, 24), np.float32)
+        fn = getattr(lib(), "oc_" + name)
+        if mu is None:
+            fn(self.h, ptr(s), ptr(r))
+        else:
+            fn(self.h, mu, ptr(s), ptr(r))
+        return r
+
+    def Dw_dble(self, mu, s, r=None): return self._op_d("Dw_dble", mu, s, r)
+    def Dwee_dble(self, mu, s, r=None): return self._op_d("Dwee_dble", mu, s, r)
+    def Dwoo_dble(self, mu, s, r=None): return self._op_d("Dwoo_dble", mu, s, r)
+    def Dwoe_dble(self, s, r=None): return self._op_d("Dwoe_dble", None, s, r)
+    def Dweo_dble(self, s, r=None): return self._op_d("Dweo_dble", None, s, r)
+    def Dwhat_dble(self, mu, s, r=None): return self._op_d("Dwhat_dble", mu, s, r)
+    def Dw(self, mu, s, r=None): return self._op_f("Dw", mu, s, r)
+    def Dwee(self, mu, s, r=None): return self._op_f("Dwee", mu, s, r)
+    def Dwoo(self, mu, s, r=None): return self._op_f("Dwoo", mu, s, r)
+    def Dwoe(self, s, r=None): return self._op_f("Dwoe", None, s, r)
+    def Dweo(self, s, r=None): return self._op_f("Dweo", None, s, r)
+    def Dwhat(self, mu, s, r=None): return self._op_f("Dwhat", mu, s, r)
+
+    # SAP / solvers
+    def set_sap(self, bs, isolv, nmr, ncy):
+        b = iarr(bs)
+        lib().oc_set_sap(self.h, ptr(b), isolv, nmr, ncy)
+
+    def sap_prepare(self):
+        return lib().oc_sap_prepare(self.h)
+
+    def nblocks(self):
+        return lib().oc_sap_nblocks(self.h)
+
+    def block_imb(self, n):
+        bo = np.zeros(4, np.int32)
+        vol = C.c_int()
+        lib().oc_sap_block_info(self.h, n, ptr(bo), C.byref(vol))
+        a = np.empty(vol.value, np.int32)
+        lib().oc_sap_block_imb(self.h, n, ptr(a))
+        return a
+
+    def Dw_blk(self, n, mu, s):
+        r = np.zeros_like(s)
+        lib().oc_Dw_blk(self.h, n, mu, ptr(s), ptr(r))
+        return r
+
+    def blk_mres(self, n, mu, nmr, s1):
+        s0 = np.zeros_like(s1)
+        lib().oc_blk_mres(self.h, n, mu, nmr, ptr(s0), ptr(s1))
+        return s0, s1
+
+    def sap(self, mu, isolv, nmr, psi, rho):
+        lib().oc_sap(self.h, mu, isolv, nmr, ptr(psi), ptr(rho))
+
+    def sap_gcr(self, nkv, nmx, istop, res, mu, eta, psi=None):
+        if psi is None:
+            psi = np.zeros_like(eta)
+        ifail = C.c_int()
+        status = C.c_int()
+        rho = lib().oc_sap_gcr(self.h, nkv, nmx, istop, res, mu, ptr(eta), ptr(psi),
+                               C.byref(ifail), C.byref(status))
+        return rho, ifail.value, status.value, psi
+
+
+def kat_ranlxs(level, seed, n):
+    a = np.empty(n, np.float32)
+    lib().oc_kat_ranlxs(level, seed, ptr(a), n)
+    return a
+
+
+def kat_ranlxd(level, seed, n):
+    a = np.empty(n, np.float64)
+    lib().oc_kat_ranlxd(level, seed, ptr(a), n)
+    return a
+
+
+def inv_pauli_dble(mu, m):
+    m = np.ascontiguousarray(m, np.float64)
+    im = np.empty(36, np.float64)
+    ifail = lib().oc_inv_pauli_dble(mu, ptr(m), ptr(im))
+    return ifail, im
+
+
+def mul_pauli_dble(mu, m, s):
+    m = np.ascontiguousarray(m, np.float64)
+    s = np.ascontiguousarray(s, np.float64)
+    r = np.empty(12, np.float64)
+    lib().oc_mul_pauli_dble(mu, ptr(m), ptr(s), ptr(r))
+    return r
+
+
+def spinor_prod_dble(s, r):
+    qre = np.zeros(2)
+    qim = np.zeros(2)
+    lib().oc_spinor_prod_dble(s.shape[0], ptr(s), ptr(r), ptr(qre), ptr(qim))
+    return qre, qim
+
+
+def norm_square_dble(s):
+    q = np.zeros(2)
+    lib().oc_norm_square_dble(s.shape[0], ptr(s), ptr(q))
+    return q
+
+
+def acc_qflt(u, q):
+    q = np.ascontiguousarray(q, np.float64)
+    lib().oc_acc_qflt(u, ptr(q))
+    return q
+
+
+def add_qflt(qu, qv):
+    qu = np.ascontiguousarray(qu, np.float64)
+    qv = np.ascontiguousarray(qv, np.float64)
+    qr = np.zeros(2)
+    lib().oc_add_qflt(ptr(qu), ptr(qv), ptr(qr))
+    return qr
+
+
+def ref_exe(tag):
+    """Path of a reference driver executable, or None when not built."""
+    p = os.path.join(HERE, "_ref", "ref_" + tag)
+    return p if os.path.exists(p) else None
+
+
+def ref_dump(tag, outdir, **kw):
+    """Run the reference driver in dump mode and return a loader."""
+    exe = ref_exe(tag)
+    if exe is None:
+        return None
+    args = [exe, "dump", outdir]
+    for k, v in kw.items():
+        args.append("-" + k)
+        if isinstance(v, (list, tuple)):
+            args += [str(x) for x in v]
+        else:
+            args.append(str(v))
+    os.makedirs(outdir, exist_ok=True)
+    env = dict(os.environ, OMP_NUM_THREADS="16")
+    subprocess.run(args, check=True, capture_output=True, env=env, timeout=600)
+
+    def load(name, dtype, shape=None):
+        a = np.fromfile(os.path.join(outdir, name), dtype=dtype)
+        return a if shape is None else a.reshape(shape)
+
+    return load

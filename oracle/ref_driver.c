@@ -1,0 +1,281 @@
+/*
+ * ref_driver.c -- driver program linked against the reference's own module
+ * sources (compiled from /root/reference by oracle/build_ref.py).  It is test
+ * infrastructure: it produces reference outputs for the parity tests and the
+ * reference CPU timing reported by bench.py.  Lattice sizes are compile-time
+ * (the reference's global.h), so one executable is built per geometry.
+ *
+ * Usage:
+ *   ref_<tag> dump  OUTDIR [options]   write fields and operator outputs
+ *   ref_<tag> bench [options]          time Dw_dble/Dw/Dwhat(_dble), print JSON
+ * Options: -bc T -cF a b -csw c -m0 m -seed s -mu mu -eoflg e
+ *          -bs b0 b1 b2 b3 -isolv i -nmr n -ncy n -nkv n -nmx n -res r
+ *          -secs t (bench: target seconds per operator)
+ *
+ * The random-number sequence of "dump" is fixed:
+ *   start_ranlux(0,seed); geometry(); random_ud(); set_ud_phase();
+ *   random_sd(VOLUME_TRD,2,eta,1.0) [bnd_sd2zero(ALL_PTS) for bc!=3];
+ *   random_s (VOLUME_TRD,2,s32,1.0) [bnd_s2zero].
+ */
+#include <stdlib.h>
+#include <stdio.h>
+#include <string.h>
+#include <math.h>
+#include <omp.h>
+#include "mpi.h"
+#include "utils.h"
+#include "flags.h"
+#include "random.h"
+#include "lattice.h"
+#include "uflds.h"
+#include "sflds.h"
+#include "linalg.h"
+#include "sw_term.h"
+#include "tcharge.h"
+#include "dirac.h"
+#include "block.h"
+#include "sap.h"
+#include "global.h"
+
+static char outdir[512];
+
+static void wr(const char *name, const void *p, size_t nbytes)
+{
+   char path[700];
+   snprintf(path, sizeof(path), "%s/%s", outdir, name);
+   FILE *f = fopen(path, "wb");
+   if (f == NULL) { fprintf(stderr, "cannot write %s\n", path); exit(2); }
+   fwrite(p, 1, nbytes, f);
+   fclose(f);
+}
+
+static double wtime(void) { return MPI_Wtime(); }
+
+int main(int argc, char *argv[])
+{
+   int bc = 3, seed = 1, eoflg = 0, isolv = 0, nmr = 4, ncy = 5, nkv = 16, nmx = 256;
+   int bs[4] = {4, 4, 4, 4}, dump = 1, i;
+   double cF[2] = {1.0, 1.0}, csw = 1.0, m0 = -0.5, mu = 0.0, res = 1.0e-10, secs = 2.0;
+   double phi[2] = {0.0, 0.0}, phip[2] = {0.0, 0.0}, theta[3] = {0.0, 0.0, 0.0};
+
+   if (argc < 2) { fprintf(stderr, "usage: %s dump OUTDIR|bench [options]\n", argv[0]); return 1; }
+   i = 2;
+   if (strcmp(argv[1], "dump") == 0)
+   {
+      if (argc < 3) return 1;
+      snprintf(outdir, sizeof(outdir), "%s", argv[2]);
+      i = 3;
+   }
+   else
+      dump = 0;
+   for (; i < argc; i++)
+   {
+      if (!strcmp(argv[i], "-bc")) bc = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-cF")) { cF[0] = atof(argv[++i]); cF[1] = atof(argv[++i]); }
+      else if (!strcmp(argv[i], "-csw")) csw = atof(argv[++i]);
+      else if (!strcmp(argv[i], "-m0")) m0 = atof(argv[++i]);
+      else if (!strcmp(argv[i], "-seed")) seed = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-mu")) mu = atof(argv[++i]);
+      else if (!strcmp(argv[i], "-eoflg")) eoflg = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-bs")) { for (int k = 0; k < 4; k++) bs[k] = atoi(argv[++i]); }
+      else if (!strcmp(argv[i], "-isolv")) isolv = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-nmr")) nmr = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-ncy")) ncy = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-nkv")) nkv = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-nmx")) nmx = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-res")) res = atof(argv[++i]);
+      else if (!strcmp(argv[i], "-secs")) secs = atof(argv[++i]);
+   }
+
+   mpi_init(argc, argv);
+   set_lat_parms(6.0, 1.0, 0, NULL, 0, csw);
+   set_bc_parms(bc, 1.0, 1.0, cF[0], cF[1], phi, phip, theta);
+   start_ranlux(0, seed);
+   geometry();
+   set_sw_parms(m0);
+   set_tm_parms(eoflg);
+   if (dump && (((L0 / bs[0]) * (L1 / bs[1]) * (L2 / bs[2]) * (L3 / bs[3])) % (2 * NTHREAD)) == 0)
+      set_sap_parms(bs, isolv, nmr, ncy);
+
+   alloc_wsd(10);
+   alloc_ws(2 * nkv + 9);
+   spinor_dble **wsd = reserve_wsd(8);
+   spinor **ws = reserve_ws(8);
+
+   random_ud();
+   set_ud_phase();
+
+   if (!dump)
+   {
+      /* ---------------- timing of the operators (OpenMP, NTHREAD) ------ */
+      double wt, t1, t2;
+      int nc, k;
+      sw_term(NO_PTS);
+      assign_ud2u();
+      assign_swd2sw();
+      random_sd(VOLUME_TRD, 2, wsd[0], 1.0);
+      bnd_sd2zero(ALL_PTS, wsd[0]);
+      assign_sd2s(VOLUME_TRD, 2, wsd[0], ws[0]);
+      printf("{\"nthread\": %d, \"volume\": %d", NTHREAD, VOLUME);
+      for (k = 0; k < 4; k++)
+      {
+         const char *nm[4] = {"Dw_dble", "Dw", "Dwhat_dble", "Dwhat"};
+         if (k == 2) { sw_term(ODD_PTS); assign_swd2sw(); }
+         nc = 0;
+         wt = 0.0;
+         t1 = wtime();
+         while (wt < secs)
+         {
+            if (k == 0) Dw_dble(mu, wsd[0], wsd[1]);
+            else if (k == 1) Dw((float)mu, ws[0], ws[1]);
+            else if (k == 2) Dwhat_dble(mu, wsd[0], wsd[1]);
+            else Dwhat((float)mu, ws[0], ws[1]);
+            nc++;
+            t2 = wtime();
+            wt = t2 - t1;
+         }
+         printf(", \"%s_calls\": %d, \"%s_sec\": %.6e", nm[k], nc, nm[k], wt);
+      }
+      printf("}\n");
+      MPI_Finalize();
+      return 0;
+   }
+
+   /* ------------------------------ dump --------------------------------- */
+   int V = VOLUME;
+   wr("ipt.i32", ipt, sizeof(int) * V);
+   wr("ud.f64", udfld(), sizeof(su3_dble) * 4 * V);
+
+   spinor_dble *eta = wsd[0], *sd1 = wsd[1], *sd2 = wsd[2], *sd3 = wsd[3];
+   random_sd(VOLUME_TRD, 2, eta, 1.0);
+   bnd_sd2zero(ALL_PTS, eta);
+   wr("eta.f64", eta, sizeof(spinor_dble) * V);
+   random_s(VOLUME_TRD, 2, ws[0], 1.0f);
+   bnd_s2zero(ALL_PTS, ws[0]);
+   wr("s32.f32", ws[0], sizeof(spinor) * V);
+
+   u3_alg_dble **ft = ftensor();
+   for (int n = 0; n < 6; n++)
+   {
+      char nm[32];
+      snprintf(nm, sizeof(nm), "ft%d.f64", n);
+      wr(nm, ft[n], sizeof(u3_alg_dble) * V);
+   }
+
+   sw_term(NO_PTS);
+   wr("swd.f64", swdfld(), sizeof(pauli_dble) * 2 * V);
+   assign_ud2u();
+   assign_swd2sw();
+
+   /* FP64 operator pieces; inputs may be modified (written back as *_in) */
+   struct { const char *name; int kind; } ops[5] = {
+      {"Dw_dble", 0}, {"Dwee_dble", 1}, {"Dwoo_dble", 2}, {"Dwoe_dble", 3}, {"Dweo_dble", 4}};
+   for (int k = 0; k < 5; k++)
+   {
+      char nm[64];
+      assign_sd2sd(VOLUME_TRD, 2, eta, sd1);
+      random_sd(VOLUME_TRD, 2, sd2, 1.0);    /* pre-fill of the output */
+      snprintf(nm, sizeof(nm), "%s_r0.f64", ops[k].name);
+      wr(nm, sd2, sizeof(spinor_dble) * V);
+      if (ops[k].kind == 0) Dw_dble(mu, sd1, sd2);
+      else if (ops[k].kind == 1) Dwee_dble(mu, sd1, sd2);
+      else if (ops[k].kind == 2) Dwoo_dble(mu, sd1, sd2);
+      else if (ops[k].kind == 3) Dwoe_dble(sd1, sd2);
+      else Dweo_dble(sd1, sd2);
+      snprintf(nm, sizeof(nm), "%s_in.f64", ops[k].name);
+      wr(nm, sd1, sizeof(spinor_dble) * V);
+      snprintf(nm, sizeof(nm), "%s.f64", ops[k].name);
+      wr(nm, sd2, sizeof(spinor_dble) * V);
+   }
+
+   /* FP32 operator */
+   assign_s2s(VOLUME_TRD, 2, ws[0], ws[1]);
+   Dw((float)mu, ws[1], ws[2]);
+   wr("Dw_in.f32", ws[1], sizeof(spinor) * V);
+   wr("Dw.f32", ws[2], sizeof(spinor) * V);
+
+   /* Schur complement (odd part inverted) */
+   int ifail_inv = sw_term(ODD_PTS);
+   wr("swd_oinv.f64", swdfld(), sizeof(pauli_dble) * 2 * V);
+   assign_sd2sd(VOLUME_TRD, 2, eta, sd1);
+   set_sd2zero(VOLUME_TRD, 2, sd2);
+   Dwhat_dble(mu, sd1, sd2);
+   wr("Dwhat_dble.f64", sd2, sizeof(spinor_dble) * V);
+   assign_swd2sw();
+   assign_s2s(VOLUME_TRD, 2, ws[0], ws[1]);
+   set_s2zero(VOLUME_TRD, 2, ws[2]);
+   Dwhat((float)mu, ws[1], ws[2]);
+   wr("Dwhat.f32", ws[2], sizeof(spinor) * V);
+
+   /* linear algebra: double-double dot products */
+   complex_qflt cq = spinor_prod_dble(VOLUME_TRD, 3, eta, wsd[1]);
+   qflt nq = norm_square_dble(VOLUME_TRD, 3, eta);
+   double la[6] = {cq.re.q[0], cq.re.q[1], cq.im.q[0], cq.im.q[1], nq.q[0], nq.q[1]};
+   wr("qsum.f64", la, sizeof(la));
+   complex z32 = spinor_prod(VOLUME_TRD, 3, ws[0], ws[2]);
+   float n32 = norm_square(VOLUME_TRD, 3, ws[0]);
+   float lf[3] = {z32.re, z32.im, n32};
+   wr("salg.f32", lf, sizeof(lf));
+
+   /* the SAP block grid needs an even multiple of NTHREAD blocks
+      (flags/sap_parms.c:105); otherwise the SAP parts are left out */
+   int nblk = (L0 / bs[0]) * (L1 / bs[1]) * (L2 / bs[2]) * (L3 / bs[3]);
+   int ifail_bgr = 0, ifail = 0, status = 0, dosap = (nblk % (2 * NTHREAD)) == 0;
+   double rho = 0.0, t1 = 0.0, t2 = 0.0;
+   qflt q1 = {{0.0, 0.0}}, q0 = {{1.0, 0.0}};
+   if (!dosap) goto info;
+
+   /* SAP sweep: ncy cycles on (psi=0, rho=s32) */
+   sw_term(NO_PTS);
+   assign_ud2u();
+   assign_swd2sw();
+   alloc_bgr(SAP_BLOCKS);
+   assign_ud2ubgr(SAP_BLOCKS);
+   ifail_bgr = assign_swd2swbgr(SAP_BLOCKS, isolv ? ODD_PTS : NO_PTS);
+   set_s2zero(VOLUME_TRD, 2, ws[3]);
+   assign_s2s(VOLUME_TRD, 2, ws[0], ws[4]);
+   for (int n = 0; n < ncy; n++) sap((float)mu, isolv, nmr, ws[3], ws[4]);
+   wr("sap_psi.f32", ws[3], sizeof(spinor) * V);
+   wr("sap_rho.f32", ws[4], sizeof(spinor) * V);
+
+   /* block MinRes on block 0: s[1] = rho restricted to the block */
+   {
+      int nb, isw;
+      block_t *b = blk_list(SAP_BLOCKS, &nb, &isw);
+      if (isolv == 0)
+      {
+         assign_s2sblk(SAP_BLOCKS, 0, ALL_PTS, ws[0], 1);
+         wr("blk0_s1_in.f32", b[0].s[1], sizeof(spinor) * b[0].vol);
+         Dw_blk(SAP_BLOCKS, 0, (float)mu, 1, 2);
+         wr("blk0_Dw.f32", b[0].s[2], sizeof(spinor) * b[0].vol);
+         blk_mres(0, (float)mu, nmr);
+         wr("blk0_s0.f32", b[0].s[0], sizeof(spinor) * b[0].vol);
+         wr("blk0_s1.f32", b[0].s[1], sizeof(spinor) * b[0].vol);
+      }
+      wr("blk0_imb.i32", b[0].imb, sizeof(int) * b[0].vol);
+   }
+
+   /* SAP+GCR */
+   t1 = wtime();
+   rho = sap_gcr(nkv, nmx, 0, res, mu, eta, sd3, &ifail, &status);
+   t2 = wtime();
+   wr("gcr_psi.f64", sd3, sizeof(spinor_dble) * V);
+   Dw_dble(mu, sd3, sd1);
+   mulr_spinor_add_dble(VOLUME_TRD, 2, sd1, eta, -1.0);
+   q1 = norm_square_dble(VOLUME_TRD, 3, sd1);
+   q0 = norm_square_dble(VOLUME_TRD, 3, eta);
+info:
+   {
+      char path[700];
+      snprintf(path, sizeof(path), "%s/info.json", outdir);
+      FILE *f = fopen(path, "w");
+      fprintf(f, "{\"V\": %d, \"NTHREAD\": %d, \"sap\": %d, \"ifail_inv\": %d, \"ifail_bgr\": %d, "
+                 "\"gcr_ifail\": %d, \"gcr_status\": %d, \"gcr_rho\": %.17e, "
+                 "\"gcr_check\": %.17e, \"gcr_sec\": %.6e}\n",
+              V, NTHREAD, dosap, ifail_inv, ifail_bgr, ifail, status, rho, sqrt(q1.q[0] / q0.q[0]),
+              t2 - t1);
+      fclose(f);
+   }
+   MPI_Finalize();
+   return 0;
+}
