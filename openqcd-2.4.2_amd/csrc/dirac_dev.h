@@ -1,0 +1,132 @@
+// dirac_dev.h -- device building blocks of the Wilson-Dirac operator shared
+// by the full-lattice kernels (dirac.hip) and the SAP block solver (sap.hip):
+// site-block loads/stores, the spin projections (1 -+ gamma_mu) with the
+// colour products of dirac/Dw_dble.c:1114-1373, and the Pauli matrix-vector
+// product of sw_term/pauli_dble.c:1336-1440.
+#pragma once
+
+#include "oqh_internal.h"
+
+namespace oqh {
+
+template <typename C> __device__ inline void load_spinor(const C *f, int y, C *s)
+{
+   const C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++) s[c] = p[c * SB];
+}
+
+template <typename C> __device__ inline void store_spinor(C *f, int y, const C *s)
+{
+   C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c * SB] = s[c];
+}
+
+template <typename C> __device__ inline void zero_spinor(C *f, int y)
+{
+   C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+   C z = cmk<C>(0, 0);
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c * SB] = z;
+}
+
+// c = U h  or  c = U^dag h  (U: 9 units starting at up, stride SB)
+template <typename C, bool DAG> __device__ inline void su3_apply(const C *up, const C *h, C *c)
+{
+   C u[9];
+#pragma unroll
+   for (int k = 0; k < 9; k++) u[k] = up[k * SB];
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      if (!DAG)
+         c[i] = cadd(cadd(cmul(u[3 * i], h[0]), cmul(u[3 * i + 1], h[1])), cmul(u[3 * i + 2], h[2]));
+      else
+         c[i] = cadd(cadd(cmulc(u[i], h[0]), cmulc(u[3 + i], h[1])), cmulc(u[6 + i], h[2]));
+   }
+}
+
+// one direction mu, forward (FWD) or backward: project s, multiply, expand
+template <typename C, int MU, bool FWD>
+__device__ inline void hop_one(const C *sp, const C *up, C *acc)
+{
+   C h0[3], h1[3], c0[3], c1[3];
+   // spin components s_a are sp[3a..3a+2]
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      C s0 = sp[i], s1 = sp[3 + i], s2 = sp[6 + i], s3 = sp[9 + i];
+      if (MU == 0) { h0[i] = FWD ? cadd(s0, s2) : csub(s0, s2); h1[i] = FWD ? cadd(s1, s3) : csub(s1, s3); }
+      if (MU == 1) { h0[i] = FWD ? caddi(s0, s3) : csubi(s0, s3); h1[i] = FWD ? caddi(s1, s2) : csubi(s1, s2); }
+      if (MU == 2) { h0[i] = FWD ? cadd(s0, s3) : csub(s0, s3); h1[i] = FWD ? csub(s1, s2) : cadd(s1, s2); }
+      if (MU == 3) { h0[i] = FWD ? caddi(s0, s2) : csubi(s0, s2); h1[i] = FWD ? csubi(s1, s3) : caddi(s1, s3); }
+   }
+   su3_apply<C, !FWD>(up, h0, c0);
+   su3_apply<C, !FWD>(up, h1, c1);
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      acc[i] = cadd(acc[i], c0[i]);
+      acc[3 + i] = cadd(acc[3 + i], c1[i]);
+      if (MU == 0)
+      {
+         acc[6 + i] = FWD ? cadd(acc[6 + i], c0[i]) : csub(acc[6 + i], c0[i]);
+         acc[9 + i] = FWD ? cadd(acc[9 + i], c1[i]) : csub(acc[9 + i], c1[i]);
+      }
+      if (MU == 1)
+      {
+         acc[6 + i] = FWD ? csubi(acc[6 + i], c1[i]) : caddi(acc[6 + i], c1[i]);
+         acc[9 + i] = FWD ? csubi(acc[9 + i], c0[i]) : caddi(acc[9 + i], c0[i]);
+      }
+      if (MU == 2)
+      {
+         acc[6 + i] = FWD ? csub(acc[6 + i], c1[i]) : cadd(acc[6 + i], c1[i]);
+         acc[9 + i] = FWD ? cadd(acc[9 + i], c0[i]) : csub(acc[9 + i], c0[i]);
+      }
+      if (MU == 3)
+      {
+         acc[6 + i] = FWD ? csubi(acc[6 + i], c0[i]) : caddi(acc[6 + i], c0[i]);
+         acc[9 + i] = FWD ? caddi(acc[9 + i], c1[i]) : csubi(acc[9 + i], c1[i]);
+      }
+   }
+}
+
+// address of link l (0..7, udfld order) stored with odd point j = y - V/2
+template <typename C> __device__ inline const C *link_ptr(const C *u, int j, int l)
+{
+   return u + (size_t)(j >> 6) * (72 * SB) + (j & (SB - 1)) + (size_t)(9 * l) * SB;
+}
+
+// out = (A + i mu) v on one Weyl half (6 complex); m points at unit 0 of the
+// half (stride SB); units 0-2 hold the real diagonal, 3-17 the upper
+// off-diagonal entries (0,1),(0,2),...,(4,5)
+template <typename C>
+__device__ inline void pauli_half(const C *m, decltype(C::x) mu, const C *v, C *out)
+{
+   C d0 = m[0], d1 = m[SB], d2 = m[2 * SB];
+   decltype(C::x) d[6] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y};
+#pragma unroll
+   for (int i = 0; i < 6; i++) out[i] = cmk<C>(d[i] * v[i].x - mu * v[i].y, d[i] * v[i].y + mu * v[i].x);
+   int p = 3;
+#pragma unroll
+   for (int i = 0; i < 6; i++)
+#pragma unroll
+      for (int j = i + 1; j < 6; j++)
+      {
+         C hij = m[p * SB];
+         out[i] = cadd(out[i], cmul(hij, v[j]));
+         out[j] = cadd(out[j], cmulc(hij, v[i]));
+         p++;
+      }
+}
+
+template <typename C>
+__device__ inline void pauli_site(const C *sw, int x, decltype(C::x) mu, const C *v, C *out)
+{
+   const C *m = sw + (size_t)(x >> 6) * (36 * SB) + (x & (SB - 1));
+   pauli_half(m, mu, v, out);
+   pauli_half(m + 18 * SB, -mu, v + 6, out + 6);
+}
+
+}  // namespace oqh

@@ -1,0 +1,730 @@
+// linalg.hip -- spinor BLAS-1, field assignments, uniform norms and the
+// double-double sums, on gfx950.
+//
+// Replaces linalg/salg.c:1223-1429, linalg/salg_dble.c:2611-2891,
+// sflds/sflds.c:1203-1405, sflds/unorm.c:261-314 and utils/qsum.c:262-424.
+//
+// Reductions are deterministic: every work-item sums a fixed set of points in
+// double precision (grid-stride order), the partial sums are combined as
+// double-double numbers (Knuth's two-sum, as acc_qflt/add_qflt do) first
+// across the wavefront with lane shuffles, then across the work-group through
+// LDS, and finally over the work-group partials in a fixed order by a single
+// work-group.  The FP64 products therefore end in a (hi,lo) pair whose sum is
+// at least as accurate as the reference's chunked accumulation.  The FP32
+// products follow salg.c: per point the colour-vector products are formed in
+// float, converted to double and summed in double; the result is rounded to
+// float.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "oqh_internal.h"
+
+namespace oqh {
+
+constexpr int RB = 256;      // reduction work-group size
+constexpr int RMAXB = 1024;  // maximum number of reduction work-groups
+
+struct dd
+{
+   double hi, lo;
+};
+
+__device__ __host__ inline dd dd_add(dd a, dd b)
+{
+   // sum of two double-double numbers (same steps as add_qflt, qsum.c:284)
+   double s = a.hi + b.hi;
+   double c = a.lo + b.lo;
+   double up = s - b.hi;
+   double rp = c - b.lo;
+   double vp = s - up;
+   double sp = c - rp;
+   double e1 = (a.hi - up) + (b.hi - vp);
+   double d = (a.lo - rp) + (b.lo - sp);
+   e1 += c;
+   double e = s + e1;
+   double f = e1 - (e - s);
+   f += d;
+   double w = e + f;
+   dd r;
+   r.hi = w;
+   r.lo = f - (w - e);
+   return r;
+}
+
+__device__ inline dd shfl_xor_dd(dd v, int m)
+{
+   dd r;
+   r.hi = __shfl_xor(v.hi, m);
+   r.lo = __shfl_xor(v.lo, m);
+   return r;
+}
+
+// reduce NV double-double values over the work-group; result in thread 0
+template <int NV> __device__ inline void block_reduce_dd(dd *v)
+{
+   __shared__ dd sh[RB / 64][NV];
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+      for (int k = 0; k < NV; k++) v[k] = dd_add(v[k], shfl_xor_dd(v[k], m));
+   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+   if (l == 0)
+#pragma unroll
+      for (int k = 0; k < NV; k++) sh[w][k] = v[k];
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      for (int ww = 1; ww < RB / 64; ww++)
+#pragma unroll
+         for (int k = 0; k < NV; k++) v[k] = dd_add(v[k], sh[ww][k]);
+   }
+}
+
+// per-point contribution of the supported products
+enum DotKind { DOT_C = 0, DOT_RE, DOT_NORM, DOT_C5 };
+
+template <typename C, int KIND>
+__device__ inline void point_dot(const C *a, const C *b, int x, int y, double *re, double *im)
+{
+   const C *pa = a + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+   const C *pb = b + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+   typedef decltype(C::x) R;
+   if (sizeof(R) == 8)
+   {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int c = 0; c < 12; c++)
+      {
+         C u = pa[c * SB], v = (KIND == DOT_NORM) ? u : pb[c * SB];
+         double g = (KIND == DOT_C5 && c >= 6) ? -1.0 : 1.0;
+         sr += g * ((double)u.x * v.x + (double)u.y * v.y);
+         if (KIND == DOT_C || KIND == DOT_C5) si += g * ((double)u.x * v.y - (double)u.y * v.x);
+      }
+      *re += sr;
+      *im += si;
+   }
+   else
+   {
+      // salg.c: per colour vector in float, then in double
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+      {
+         R pr = 0.0f, pi = 0.0f;
+#pragma unroll
+         for (int i = 0; i < 3; i++)
+         {
+            C u = pa[(3 * k + i) * SB], v = (KIND == DOT_NORM) ? u : pb[(3 * k + i) * SB];
+            pr += u.x * v.x + u.y * v.y;
+            if (KIND == DOT_C) pi += u.x * v.y - u.y * v.x;
+         }
+         *re += (double)pr;
+         *im += (double)pi;
+      }
+   }
+}
+
+template <typename C, int KIND>
+__global__ __launch_bounds__(RB) void k_dot_partial(const C *a, const C *b, int aoff, int boff, int n, dd *part)
+{
+   double re = 0.0, im = 0.0;
+   int stride = gridDim.x * RB;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+      point_dot<C, KIND>(a, b, aoff + k, boff + k, &re, &im);
+   dd v[2] = {{re, 0.0}, {im, 0.0}};
+   block_reduce_dd<2>(v);
+   if (threadIdx.x == 0)
+   {
+      part[2 * blockIdx.x] = v[0];
+      part[2 * blockIdx.x + 1] = v[1];
+   }
+}
+
+__global__ __launch_bounds__(RB) void k_dd_final(const dd *part, int nb, dd *out)
+{
+   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+   for (int k = threadIdx.x; k < nb; k += RB)
+   {
+      v[0] = dd_add(v[0], part[2 * k]);
+      v[1] = dd_add(v[1], part[2 * k + 1]);
+   }
+   block_reduce_dd<2>(v);
+   if (threadIdx.x == 0)
+   {
+      out[0] = v[0];
+      out[1] = v[1];
+   }
+}
+
+static dd *d_part = nullptr, *h_res = nullptr;
+
+static int red_blocks(int n)
+{
+   int nb = (n + RB - 1) / RB;
+   return nb < 1 ? 1 : (nb > RMAXB ? RMAXB : nb);
+}
+
+// launches the reduction; result (re, im) as double-double in res[0..1]
+template <typename C, int KIND>
+void dot_async(const C *a, const C *b, int aoff, int boff, int n, dd *d_out, hipStream_t st)
+{
+   if (!d_part) d_part = (dd *)dev_alloc(sizeof(dd) * 2 * (RMAXB + 8));
+   int nb = red_blocks(n);
+   k_dot_partial<C, KIND><<<nb, RB, 0, st>>>(a, b, aoff, boff, n, d_part);
+   k_dd_final<<<1, RB, 0, st>>>(d_part, nb, d_out);
+}
+
+static dd *d_out_scalar = nullptr;
+
+template <typename C, int KIND> static void dot_sync(const C *a, const C *b, int aoff, int boff, int n, dd *res)
+{
+   Lib &L = lib();
+   if (!d_out_scalar) d_out_scalar = (dd *)dev_alloc(sizeof(dd) * 2);
+   if (!h_res) HIP_OK(hipHostMalloc((void **)&h_res, sizeof(dd) * 2, 0));
+   dot_async<C, KIND>(a, b, aoff, boff, n, d_out_scalar, L.stream);
+   HIP_OK(hipMemcpyAsync(h_res, d_out_scalar, sizeof(dd) * 2, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   res[0] = h_res[0];
+   res[1] = h_res[1];
+}
+
+// ---------------------------------------------------------------------------
+// element-wise kernels (one work-item per point)
+// ---------------------------------------------------------------------------
+
+enum AxKind { AX_MULC = 0, AX_MULR, AX_SCALE, AX_G5, AX_MG5, AX_COMB, AX_ZERO, AX_COPY, AX_ADD, AX_DIFF };
+
+template <typename C, int K>
+__global__ __launch_bounds__(256) void k_axpy(C *s, const C *r, int soff, int roff, int n,
+                                              decltype(C::x) zr, decltype(C::x) zi, decltype(C::x) cr)
+{
+   int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= n) return;
+   int xs = soff + k, xr = roff + k;
+   C *ps = s + (size_t)(xs >> 6) * (12 * SB) + (xs & (SB - 1));
+   const C *pr = r ? r + (size_t)(xr >> 6) * (12 * SB) + (xr & (SB - 1)) : nullptr;
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      C a = ps[c * SB];
+      if (K == AX_MULC)
+      {
+         C b = pr[c * SB];
+         a = cmk<C>(a.x + (zr * b.x - zi * b.y), a.y + (zr * b.y + zi * b.x));
+      }
+      else if (K == AX_MULR)
+      {
+         C b = pr[c * SB];
+         a = cmk<C>(a.x + zr * b.x, a.y + zr * b.y);
+      }
+      else if (K == AX_SCALE)
+         a = cmk<C>(zr * a.x, zr * a.y);
+      else if (K == AX_G5)
+      {
+         if (c >= 6) a = cmk<C>(-a.x, -a.y);
+      }
+      else if (K == AX_MG5)
+      {
+         if (c < 6) a = cmk<C>(-a.x, -a.y);
+      }
+      else if (K == AX_COMB)
+      {
+         C b = pr[c * SB];
+         a = cmk<C>(zr * a.x + cr * b.x, zr * a.y + cr * b.y);
+      }
+      else if (K == AX_ZERO)
+         a = cmk<C>(0, 0);
+      else if (K == AX_COPY)
+         a = pr[c * SB];
+      else if (K == AX_ADD)
+      {
+         C b = pr[c * SB];
+         a = cmk<C>(a.x + b.x, a.y + b.y);
+      }
+      else if (K == AX_DIFF)  // s <- r - s   (diff_s2s: r = s - r with roles swapped by caller)
+      {
+         C b = pr[c * SB];
+         a = cmk<C>(b.x - a.x, b.y - a.y);
+      }
+      ps[c * SB] = a;
+   }
+}
+
+// mixed-precision assignments
+template <int K>
+__global__ __launch_bounds__(256) void k_mixed(const void *a, const void *b, void *r, int aoff, int boff,
+                                               int roff, int n)
+{
+   int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= n) return;
+   int xa = aoff + k, xb = boff + k, xr = roff + k;
+   size_t ia = (size_t)(xa >> 6) * (12 * SB) + (xa & (SB - 1));
+   size_t ib = (size_t)(xb >> 6) * (12 * SB) + (xb & (SB - 1));
+   size_t ir = (size_t)(xr >> 6) * (12 * SB) + (xr & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      if (K == 0)  // assign_s2sd: float a -> double r
+      {
+         float2 v = ((const float2 *)a)[ia + c * SB];
+         ((double2 *)r)[ir + c * SB] = make_double2(v.x, v.y);
+      }
+      else if (K == 1)  // assign_sd2s: double a -> float r
+      {
+         double2 v = ((const double2 *)a)[ia + c * SB];
+         ((float2 *)r)[ir + c * SB] = make_float2((float)v.x, (float)v.y);
+      }
+      else if (K == 2)  // add_s2sd: double r += float a
+      {
+         float2 v = ((const float2 *)a)[ia + c * SB];
+         double2 w = ((double2 *)r)[ir + c * SB];
+         ((double2 *)r)[ir + c * SB] = make_double2(w.x + (double)v.x, w.y + (double)v.y);
+      }
+      else  // diff_sd2s: float r = double a - double b
+      {
+         double2 v = ((const double2 *)a)[ia + c * SB];
+         double2 w = ((const double2 *)b)[ib + c * SB];
+         ((float2 *)r)[ir + c * SB] = make_float2((float)(v.x - w.x), (float)(v.y - w.y));
+      }
+   }
+}
+
+// per-point squared norm maximum
+template <typename C>
+__global__ __launch_bounds__(RB) void k_unorm_partial(const C *a, int off, int n, double *part)
+{
+   __shared__ double sh[RB / 64];
+   double m = 0.0;
+   int stride = gridDim.x * RB;
+   typedef decltype(C::x) R;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+   {
+      int x = off + k;
+      const C *p = a + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+      R s = 0;
+#pragma unroll
+      for (int c = 0; c < 12; c++)
+      {
+         C u = p[c * SB];
+         s += u.x * u.x + u.y * u.y;
+      }
+      m = fmax(m, (double)s);
+   }
+#pragma unroll
+   for (int sft = 32; sft >= 1; sft >>= 1) m = fmax(m, __shfl_xor(m, sft));
+   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0)
+   {
+      for (int w = 1; w < RB / 64; w++) m = fmax(m, sh[w]);
+      part[blockIdx.x] = m;
+   }
+}
+
+__global__ void k_max_final(const double *part, int nb, double *out)
+{
+   if (threadIdx.x != 0) return;
+   double m = 0.0;
+   for (int k = 0; k < nb; k++) m = fmax(m, part[k]);
+   out[0] = m;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+static int nsites(int vol, int icom)
+{
+   Lib &L = lib();
+   return (icom & 0x2) ? vol * L.g.nthread : vol;
+}
+
+static unsigned nb256(int n) { return (unsigned)((n + 255) / 256); }
+
+struct Arg2
+{
+   Res a, b;
+   int n;
+   bool ok;
+};
+
+static Arg2 args2(const void *a, const void *b, int prec, int vol, int icom, const char *where,
+                  bool upload_b = true)
+{
+   Arg2 g;
+   g.ok = false;
+   if (!require_geometry(where)) return g;
+   g.n = nsites(vol, icom);
+   if (g.n <= 0) return g;
+   g.a = resolve(a, prec, g.n, 2, true);
+   if (b) g.b = resolve(b, prec, g.n, 3, upload_b);
+   g.ok = g.a.dev != nullptr && (!b || g.b.dev != nullptr);
+   return g;
+}
+
+template <typename C, int K>
+static void axpy(void *s, void *r, int prec, int vol, int icom, double zr, double zi, double cr, const char *where)
+{
+   Lib &L = lib();
+   Arg2 g = args2(s, r, prec, vol, icom, where);
+   if (!g.ok) return;
+   typedef decltype(C::x) R;
+   k_axpy<C, K><<<nb256(g.n), 256, 0, L.stream>>>((C *)g.a.dev, r ? (const C *)g.b.dev : nullptr, g.a.off,
+                                                  r ? g.b.off : 0, g.n, (R)zr, (R)zi, (R)cr);
+   finish(g.a, prec, g.n, true);
+   if (r) finish(g.b, prec, g.n, false);
+}
+
+template <typename C, int KIND>
+static void dot(void *s, void *r, int prec, int vol, int icom, dd *res, const char *where)
+{
+   Arg2 g = args2(s, r, prec, vol, icom, where);
+   res[0] = res[1] = dd{0.0, 0.0};
+   if (!g.ok) return;
+   dot_sync<C, KIND>((const C *)g.a.dev, r ? (const C *)g.b.dev : (const C *)g.a.dev, g.a.off,
+                     r ? g.b.off : g.a.off, g.n, res);
+}
+
+}  // namespace oqh
+
+using namespace oqh;
+
+// ---------------------------------------------------------------------------
+// C ABI: linalg/salg.c (single precision)
+// ---------------------------------------------------------------------------
+
+extern "C" complex spinor_prod(int vol, int icom, spinor *s, spinor *r)
+{
+   dd res[2];
+   dot<float2, DOT_C>(s, r, F32, vol, icom, res, "spinor_prod [salg.c]");
+   complex z;
+   z.re = (float)(res[0].hi + res[0].lo);
+   z.im = (float)(res[1].hi + res[1].lo);
+   return z;
+}
+
+extern "C" float spinor_prod_re(int vol, int icom, spinor *s, spinor *r)
+{
+   dd res[2];
+   dot<float2, DOT_RE>(s, r, F32, vol, icom, res, "spinor_prod_re [salg.c]");
+   return (float)(res[0].hi + res[0].lo);
+}
+
+extern "C" float norm_square(int vol, int icom, spinor *s)
+{
+   dd res[2];
+   dot<float2, DOT_NORM>(s, nullptr, F32, vol, icom, res, "norm_square [salg.c]");
+   return (float)(res[0].hi + res[0].lo);
+}
+
+extern "C" void mulc_spinor_add(int vol, int icom, spinor *s, spinor *r, complex z)
+{
+   axpy<float2, AX_MULC>(s, r, F32, vol, icom, z.re, z.im, 0.0, "mulc_spinor_add [salg.c]");
+}
+
+extern "C" void mulr_spinor_add(int vol, int icom, spinor *s, spinor *r, float c)
+{
+   axpy<float2, AX_MULR>(s, r, F32, vol, icom, c, 0.0, 0.0, "mulr_spinor_add [salg.c]");
+}
+
+extern "C" void scale(int vol, int icom, float c, spinor *s)
+{
+   axpy<float2, AX_SCALE>(s, nullptr, F32, vol, icom, c, 0.0, 0.0, "scale [salg.c]");
+}
+
+extern "C" void project(int vol, int icom, spinor *s, spinor *r)
+{
+   complex z = spinor_prod(vol, icom, r, s);
+   z.re = -z.re;
+   z.im = -z.im;
+   mulc_spinor_add(vol, icom, s, r, z);
+}
+
+extern "C" float normalize(int vol, int icom, spinor *s)
+{
+   float r = norm_square(vol, icom, s);
+   r = (float)sqrt((double)r);
+   if (r != 0.0f)
+      scale(vol, icom, 1.0f / r, s);
+   else if (icom != 0)
+      fail("normalize [salg.c]", "Vector has vanishing norm");
+   return r;
+}
+
+extern "C" void mulg5(int vol, int icom, spinor *s)
+{
+   axpy<float2, AX_G5>(s, nullptr, F32, vol, icom, 0.0, 0.0, 0.0, "mulg5 [salg.c]");
+}
+
+extern "C" void mulmg5(int vol, int icom, spinor *s)
+{
+   axpy<float2, AX_MG5>(s, nullptr, F32, vol, icom, 0.0, 0.0, 0.0, "mulmg5 [salg.c]");
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: linalg/salg_dble.c (double precision, double-double sums)
+// ---------------------------------------------------------------------------
+
+static qflt to_qflt(dd v)
+{
+   qflt q;
+   q.q[0] = v.hi;
+   q.q[1] = v.lo;
+   return q;
+}
+
+extern "C" complex_qflt spinor_prod_dble(int vol, int icom, spinor_dble *s, spinor_dble *r)
+{
+   dd res[2];
+   dot<double2, DOT_C>(s, r, F64, vol, icom, res, "spinor_prod_dble [salg_dble.c]");
+   complex_qflt z;
+   z.re = to_qflt(res[0]);
+   z.im = to_qflt(res[1]);
+   return z;
+}
+
+extern "C" qflt spinor_prod_re_dble(int vol, int icom, spinor_dble *s, spinor_dble *r)
+{
+   dd res[2];
+   dot<double2, DOT_RE>(s, r, F64, vol, icom, res, "spinor_prod_re_dble [salg_dble.c]");
+   return to_qflt(res[0]);
+}
+
+extern "C" complex_qflt spinor_prod5_dble(int vol, int icom, spinor_dble *s, spinor_dble *r)
+{
+   dd res[2];
+   dot<double2, DOT_C5>(s, r, F64, vol, icom, res, "spinor_prod5_dble [salg_dble.c]");
+   complex_qflt z;
+   z.re = to_qflt(res[0]);
+   z.im = to_qflt(res[1]);
+   return z;
+}
+
+extern "C" qflt norm_square_dble(int vol, int icom, spinor_dble *s)
+{
+   dd res[2];
+   dot<double2, DOT_NORM>(s, nullptr, F64, vol, icom, res, "norm_square_dble [salg_dble.c]");
+   return to_qflt(res[0]);
+}
+
+extern "C" void mulc_spinor_add_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, complex_dble z)
+{
+   axpy<double2, AX_MULC>(s, r, F64, vol, icom, z.re, z.im, 0.0, "mulc_spinor_add_dble [salg_dble.c]");
+}
+
+extern "C" void mulr_spinor_add_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, double c)
+{
+   axpy<double2, AX_MULR>(s, r, F64, vol, icom, c, 0.0, 0.0, "mulr_spinor_add_dble [salg_dble.c]");
+}
+
+extern "C" void combine_spinor_dble(int vol, int icom, spinor_dble *s, spinor_dble *r, double cs, double cr)
+{
+   axpy<double2, AX_COMB>(s, r, F64, vol, icom, cs, 0.0, cr, "combine_spinor_dble [salg_dble.c]");
+}
+
+extern "C" void scale_dble(int vol, int icom, double c, spinor_dble *s)
+{
+   axpy<double2, AX_SCALE>(s, nullptr, F64, vol, icom, c, 0.0, 0.0, "scale_dble [salg_dble.c]");
+}
+
+extern "C" void project_dble(int vol, int icom, spinor_dble *s, spinor_dble *r)
+{
+   complex_qflt q = spinor_prod_dble(vol, icom, r, s);
+   complex_dble z;
+   z.re = -q.re.q[0];
+   z.im = -q.im.q[0];
+   mulc_spinor_add_dble(vol, icom, s, r, z);
+}
+
+extern "C" double normalize_dble(int vol, int icom, spinor_dble *s)
+{
+   qflt q = norm_square_dble(vol, icom, s);
+   double r = sqrt(q.q[0]);
+   if (r != 0.0)
+      scale_dble(vol, icom, 1.0 / r, s);
+   else
+      fail("normalize_dble [salg_dble.c]", "Vector field has vanishing norm");
+   return r;
+}
+
+extern "C" void mulg5_dble(int vol, int icom, spinor_dble *s)
+{
+   axpy<double2, AX_G5>(s, nullptr, F64, vol, icom, 0.0, 0.0, 0.0, "mulg5_dble [salg_dble.c]");
+}
+
+extern "C" void mulmg5_dble(int vol, int icom, spinor_dble *s)
+{
+   axpy<double2, AX_MG5>(s, nullptr, F64, vol, icom, 0.0, 0.0, 0.0, "mulmg5_dble [salg_dble.c]");
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: utils/qsum.c -- host arithmetic on (hi,lo) pairs
+// ---------------------------------------------------------------------------
+
+extern "C" void acc_qflt(double u, double *qr)
+{
+   double a = qr[0] + u;
+   double qp = a - u;
+   double up = a - qp;
+   double b = (qr[0] - qp) + (u - up);
+   double c = qr[1] + b;
+   double d = a + c;
+   qr[0] = d;
+   qr[1] = c - (d - a);
+}
+
+extern "C" void add_qflt(double *qu, double *qv, double *qr)
+{
+   dd r = dd_add(dd{qu[0], qu[1]}, dd{qv[0], qv[1]});
+   qr[0] = r.hi;
+   qr[1] = r.lo;
+}
+
+// Dekker's exact product with the splitting constant 2^27+1 (qsum.c:68,313)
+static void exact_prod(double u, double v, double *qr)
+{
+   const double mm = 134217729.0;
+   double ux = mm * u, vx = mm * v;
+   double uy = ux - u, vy = vx - v;
+   double uh = ux - uy, vh = vx - vy;
+   double ul = u - uh, vl = v - vh;
+   double x = u * v;
+   double c = uh * vh - x;
+   double d = uh * vl + c;
+   double e = ul * vh + d;
+   double y = ul * vl + e;
+   qr[0] = x;
+   qr[1] = y;
+}
+
+extern "C" void scl_qflt(double u, double *qr)
+{
+   double p[2], s;
+   exact_prod(u, qr[0], p);
+   s = u * qr[1];
+   acc_qflt(s, p);
+   qr[0] = p[0];
+   qr[1] = p[1];
+}
+
+extern "C" void mul_qflt(double *qu, double *qv, double *qr)
+{
+   double p[2], s;
+   s = qu[0] * qv[1] + qu[1] * qv[0];
+   exact_prod(qu[0], qv[0], p);
+   acc_qflt(s, p);
+   qr[0] = p[0];
+   qr[1] = p[1];
+}
+
+extern "C" void global_qsum(int n, double **qu, double **qr)
+{
+   // one process: the global sum is the local value (qsum.c:416-423)
+   for (int k = 0; k < n; k++)
+   {
+      qr[k][0] = qu[k][0];
+      qr[k][1] = qu[k][1];
+   }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: sflds/sflds.c, sflds/unorm.c
+// ---------------------------------------------------------------------------
+
+extern "C" void set_s2zero(int vol, int icom, spinor *s)
+{
+   axpy<float2, AX_ZERO>(s, nullptr, F32, vol, icom, 0, 0, 0, "set_s2zero [sflds.c]");
+}
+
+extern "C" void set_sd2zero(int vol, int icom, spinor_dble *sd)
+{
+   axpy<double2, AX_ZERO>(sd, nullptr, F64, vol, icom, 0, 0, 0, "set_sd2zero [sflds.c]");
+}
+
+extern "C" void assign_s2s(int vol, int icom, spinor *s, spinor *r)
+{
+   axpy<float2, AX_COPY>(r, s, F32, vol, icom, 0, 0, 0, "assign_s2s [sflds.c]");
+}
+
+extern "C" void assign_sd2sd(int vol, int icom, spinor_dble *sd, spinor_dble *rd)
+{
+   axpy<double2, AX_COPY>(rd, sd, F64, vol, icom, 0, 0, 0, "assign_sd2sd [sflds.c]");
+}
+
+extern "C" void add_s2s(int vol, int icom, spinor *s, spinor *r)
+{
+   axpy<float2, AX_ADD>(r, s, F32, vol, icom, 0, 0, 0, "add_s2s [sflds.c]");
+}
+
+extern "C" void diff_s2s(int vol, int icom, spinor *s, spinor *r)
+{
+   // r = s - r
+   axpy<float2, AX_DIFF>(r, s, F32, vol, icom, 0, 0, 0, "diff_s2s [sflds.c]");
+}
+
+template <int K>
+static void mixed(const void *a, int pa, const void *b, void *r, int pr, int vol, int icom, const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return;
+   int n = nsites(vol, icom);
+   if (n <= 0) return;
+   Res ra = resolve(a, pa, n, 0, true);
+   Res rb;
+   if (b) rb = resolve(b, pa, n, 1, true);
+   Res rr = resolve(r, pr, n, 2, K == 2);
+   if (!ra.dev || !rr.dev || (b && !rb.dev)) return;
+   k_mixed<K><<<nb256(n), 256, 0, L.stream>>>(ra.dev, b ? rb.dev : nullptr, rr.dev, ra.off, b ? rb.off : 0,
+                                              rr.off, n);
+   finish(rr, pr, n, true);
+}
+
+extern "C" void assign_s2sd(int vol, int icom, spinor *s, spinor_dble *rd)
+{
+   mixed<0>(s, F32, nullptr, rd, F64, vol, icom, "assign_s2sd [sflds.c]");
+}
+
+extern "C" void assign_sd2s(int vol, int icom, spinor_dble *sd, spinor *r)
+{
+   mixed<1>(sd, F64, nullptr, r, F32, vol, icom, "assign_sd2s [sflds.c]");
+}
+
+extern "C" void add_s2sd(int vol, int icom, spinor *s, spinor_dble *rd)
+{
+   mixed<2>(s, F32, nullptr, rd, F64, vol, icom, "add_s2sd [sflds.c]");
+}
+
+extern "C" void diff_sd2s(int vol, int icom, spinor_dble *sd, spinor_dble *rd, spinor *r)
+{
+   mixed<3>(sd, F64, rd, r, F32, vol, icom, "diff_sd2s [sflds.c]");
+}
+
+template <typename C> static double unorm_any(const void *s, int prec, int vol, int icom, const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return 0.0;
+   int n = nsites(vol, icom);
+   if (n <= 0) return 0.0;
+   Res r = resolve(s, prec, n, 0, true);
+   if (!r.dev) return 0.0;
+   static double *d_p = nullptr, *h_p = nullptr;
+   if (!d_p) d_p = (double *)dev_alloc(sizeof(double) * (RMAXB + 8));
+   if (!h_p) HIP_OK(hipHostMalloc((void **)&h_p, sizeof(double), 0));
+   int nb = red_blocks(n);
+   k_unorm_partial<C><<<nb, RB, 0, L.stream>>>((const C *)r.dev, r.off, n, d_p);
+   k_max_final<<<1, 64, 0, L.stream>>>(d_p, nb, d_p + RMAXB);
+   HIP_OK(hipMemcpyAsync(h_p, d_p + RMAXB, sizeof(double), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   return *h_p;
+}
+
+extern "C" float unorm(int vol, int icom, spinor *s)
+{
+   double m = unorm_any<float2>(s, F32, vol, icom, "unorm [unorm.c]");
+   return (float)sqrt((double)(float)m);
+}
+
+extern "C" double unorm_dble(int vol, int icom, spinor_dble *sd)
+{
+   return sqrt(unorm_any<double2>(sd, F64, vol, icom, "unorm_dble [unorm.c]"));
+}

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(128) void k_sw_term(SwArgs a)
 // inversion of the Pauli blocks
 // ---------------------------------------------------------------------------
 
-constexpr int INV_WG = 64;
+constexpr int INV_WG = 32;  // 32 x 72 x 16 B = 36.9 KB of LDS per work-group
 
 struct InvArgs
 {
