@@ -104,6 +104,12 @@ void oqh_sync(void);
    3 Dwhat.  Returns the average milliseconds per call. */
 double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall);
 
+/* SAP block grid access for tests: block-local field k (0..2) of block n in
+   the block's own point order (vol spinors), and the embedding of the block
+   points into the lattice (b.imb of block/block.c) */
+void oqh_export_sblk(int n, int k, spinor *host);
+void oqh_get_imb(int n, int *imb);
+
 /* ======================= B. reference entry points ====================== */
 
 /* include/dirac.h:28-53 (dirac/Dw.c, dirac/Dw_dble.c) */

@@ -80,6 +80,7 @@ struct Lib
    float2 *d_sw = nullptr;
    bool ud_set = false, u_valid = false, swd_valid = false, sw_valid = false;
    bool swd_inv[2] = {false, false};
+   int swd_epoch = 0;  // incremented whenever the contents of d_swd change
 
    std::map<uintptr_t, Field> reg;
 

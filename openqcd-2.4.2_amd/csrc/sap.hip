@@ -1,0 +1,903 @@
+// sap.hip -- Schwarz alternating procedure (SAP) on gfx950.
+//
+// Replaces sap/sap.c:592-790 (sap, update_flds0/1/2), sap/blk_solv.c:1117-1186
+// (blk_mres, blk_eo_mres), dirac/Dw.c:1475-1907 (the block operators),
+// dirac/Dw_bnd.c:801-1011 with sap/sap_com.c:465-501 (block boundary update)
+// and the SAP_BLOCKS parts of block/blk_grid.c, map_u2blk.c, map_sw2blk.c and
+// map_s2blk.c.
+//
+// One work-group handles one block of the current colour.  The block's three
+// fields (solution s0, residual s1, work s2) are kept block-compact in a
+// scratch area (site-block layout over the block-local point index, even
+// points first, as the reference's block geometry, lattice/blk_geometry.c);
+// links and SW matrices are read from the global single-precision fields
+// through the block's embedding table imb.  All nmr minimal-residual steps of
+// a block run inside the work-group: the block Dirac operator with Dirichlet
+// exterior, the two block reductions (LDS tree in double precision, rounded
+// to float as salg.c does) and the two vector updates, separated by
+// work-group barriers.
+//
+// The reference computes the hopping terms from each block across its faces
+// (Dw_bnd) and subtracts them from the residual of the neighbouring blocks
+// after all blocks of the colour are done (sap_com).  Here the block
+// corrections s0 are written to a global field and a second kernel gathers,
+// for every face point of the blocks of the other colour, the hopping terms
+// from its neighbours in other blocks, so that no point is updated by two
+// work-items.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "dirac_dev.h"
+
+namespace oqh {
+
+constexpr int SWG = 256;  // work-group size of the block kernels
+
+struct SapGrid
+{
+   bool ready = false;
+   int bs[4] = {0, 0, 0, 0};
+   int nb = 0, nbh = 0, bvol = 0, bh = 0;
+   size_t Ub = 0;                        // units per block field
+   int *d_imb = nullptr;                 // [nb][bvol]
+   int *d_bnb = nullptr;                 // [8][bvol], bvol = outside
+   int *d_face[2] = {nullptr, nullptr};  // face points of colour-c blocks
+   unsigned char *d_fmask[2] = {nullptr, nullptr};
+   int nface[2] = {0, 0};
+   float2 *d_swb = nullptr;              // FP32 SW field of the block solver
+   int swb_set = -1;                     // ptset_t used for the inversion, -1 none
+   int swb_epoch = -1;                   // SW field version it was made from
+   float2 *d_scr = nullptr;              // [nbh][3][Ub]
+   float2 *d_s0g = nullptr;              // global field of block corrections
+   float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
+   std::vector<int> imb;                 // host copy
+   std::vector<int> bo;                  // [nb][4]
+};
+
+static SapGrid G;
+
+struct SArgs
+{
+   const float2 *u, *sw;
+   const int *imb, *bnb, *nbr;
+   const unsigned char *bnd;
+   int V, h, bvol, bh, masked, eoflg, nmr;
+   float mu, muo;
+   size_t Ub;
+   float2 *psi, *rho, *s0g, *scr;
+   int n0;             // first block of the colour
+};
+
+// link of direction mu, forward (x->x+mu) or backward (x-mu->x), seen from
+// global point gx with global neighbour gy
+__device__ inline const float2 *blk_link(const SArgs &a, int gx, int gy, int mu, bool fwd)
+{
+   if (gx >= a.h) return link_ptr(a.u, gx - a.h, fwd ? 2 * mu : 2 * mu + 1);
+   return link_ptr(a.u, gy - a.h, fwd ? 2 * mu + 1 : 2 * mu);
+}
+
+// acc += hopping terms from the in-block neighbours of block point i;
+// neighbours on the global boundary (open bc) contribute nothing
+template <int MU>
+__device__ inline void blk_hop_mu(const SArgs &a, const int *imb, const float2 *S, int i, int gx, float2 *acc)
+{
+   float2 sp[12];
+   int j = a.bnb[(2 * MU) * a.bvol + i];
+   if (j < a.bvol)
+   {
+      int gy = imb[j];
+      if (!(a.masked && a.bnd[gy]))
+      {
+         load_spinor(S, j, sp);
+         hop_one<float2, MU, true>(sp, blk_link(a, gx, gy, MU, true), acc);
+      }
+   }
+   j = a.bnb[(2 * MU + 1) * a.bvol + i];
+   if (j < a.bvol)
+   {
+      int gy = imb[j];
+      if (!(a.masked && a.bnd[gy]))
+      {
+         load_spinor(S, j, sp);
+         hop_one<float2, MU, false>(sp, blk_link(a, gx, gy, MU, false), acc);
+      }
+   }
+}
+
+__device__ inline void blk_hop(const SArgs &a, const int *imb, const float2 *S, int i, int gx, float2 *acc)
+{
+   blk_hop_mu<0>(a, imb, S, i, gx, acc);
+   blk_hop_mu<1>(a, imb, S, i, gx, acc);
+   blk_hop_mu<2>(a, imb, S, i, gx, acc);
+   blk_hop_mu<3>(a, imb, S, i, gx, acc);
+}
+
+__device__ inline void zero12(float2 *v)
+{
+#pragma unroll
+   for (int c = 0; c < 12; c++) v[c] = make_float2(0.0f, 0.0f);
+}
+
+// work-group sum of two doubles, broadcast to all threads
+__device__ inline void wg_sum2(double &x, double &y)
+{
+   __shared__ double sh[2][SWG / 64];
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      x += __shfl_xor(x, m);
+      y += __shfl_xor(y, m);
+   }
+   int w = threadIdx.x >> 6;
+   if ((threadIdx.x & 63) == 0)
+   {
+      sh[0][w] = x;
+      sh[1][w] = y;
+   }
+   __syncthreads();
+   x = 0.0;
+   y = 0.0;
+   for (int k = 0; k < SWG / 64; k++)
+   {
+      x += sh[0][k];
+      y += sh[1][k];
+   }
+   __syncthreads();
+}
+
+__device__ inline void wg_sum3(double &x, double &y, double &z)
+{
+   __shared__ double sh[3][SWG / 64];
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      x += __shfl_xor(x, m);
+      y += __shfl_xor(y, m);
+      z += __shfl_xor(z, m);
+   }
+   int w = threadIdx.x >> 6;
+   if ((threadIdx.x & 63) == 0)
+   {
+      sh[0][w] = x;
+      sh[1][w] = y;
+      sh[2][w] = z;
+   }
+   __syncthreads();
+   x = y = z = 0.0;
+   for (int k = 0; k < SWG / 64; k++)
+   {
+      x += sh[0][k];
+      y += sh[1][k];
+      z += sh[2][k];
+   }
+   __syncthreads();
+}
+
+// S2 = Dw_blk S1 on the points [i0, i1) of the block (Dw.c:1475)
+__device__ void blk_dw(const SArgs &a, const int *imb, float2 *S1, float2 *S2, int i0, int i1, float mu_e)
+{
+   for (int i = i0 + threadIdx.x; i < i1; i += SWG)
+   {
+      int gx = imb[i];
+      float2 o[12];
+      if (a.masked && a.bnd[gx])
+      {
+         zero12(o);
+         store_spinor(S2, i, o);
+         continue;
+      }
+      float2 acc[12], v[12];
+      zero12(acc);
+      blk_hop(a, imb, S1, i, gx, acc);
+      load_spinor((const float2 *)S1, i, v);
+      pauli_site(a.sw, gx, i < a.bh ? mu_e : a.muo, v, o);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], -0.5f));
+      store_spinor(S2, i, o);
+   }
+}
+
+// even points: S2_e = Dwhat_blk S1 (Dw.c:1828), odd half of S2 used for the
+// intermediate A_oo^-1 D_oe S1
+__device__ void blk_dwhat(const SArgs &a, const int *imb, float2 *S1, float2 *S2, float mu)
+{
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   {
+      int gx = imb[i];
+      float2 o[12];
+      if (a.masked && a.bnd[gx])
+      {
+         zero12(o);
+         store_spinor(S2, i, o);
+         continue;
+      }
+      float2 acc[12];
+      zero12(acc);
+      blk_hop(a, imb, S1, i, gx, acc);
+#pragma unroll
+      for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
+      pauli_site(a.sw, gx, 0.0f, acc, o);
+      store_spinor(S2, i, o);
+   }
+   __syncthreads();
+   for (int i = threadIdx.x; i < a.bh; i += SWG)
+   {
+      int gx = imb[i];
+      float2 o[12];
+      if (a.masked && a.bnd[gx])
+      {
+         zero12(o);
+         store_spinor(S2, i, o);
+         continue;
+      }
+      float2 acc[12], v[12];
+      zero12(acc);
+      blk_hop(a, imb, S2, i, gx, acc);
+      load_spinor((const float2 *)S1, i, v);
+      pauli_site(a.sw, gx, mu, v, o);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+      store_spinor(S2, i, o);
+   }
+}
+
+// minimal residual steps on points [0, nv) (blk_solv.c:1117-1186)
+template <bool EO>
+__device__ void blk_mres_dev(const SArgs &a, const int *imb, float2 *S0, float2 *S1, float2 *S2, float mu)
+{
+   int nv = EO ? a.bh : a.bvol;
+   for (int i = threadIdx.x; i < nv; i += SWG) zero_spinor(S0, i);
+   __syncthreads();
+   for (int imr = 0; imr < a.nmr; imr++)
+   {
+      if (EO)
+         blk_dwhat(a, imb, S1, S2, mu);
+      else
+         blk_dw(a, imb, S1, S2, 0, a.bvol, mu);
+      __syncthreads();
+      double rr = 0.0, zr = 0.0, zi = 0.0;
+      for (int i = threadIdx.x; i < nv; i += SWG)
+      {
+         const float2 *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         const float2 *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+#pragma unroll
+         for (int k = 0; k < 4; k++)
+         {
+            float nr = 0.0f, pr = 0.0f, pi = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               float2 u = p2[(3 * k + c) * SB], v = p1[(3 * k + c) * SB];
+               nr += u.x * u.x + u.y * u.y;
+               pr += u.x * v.x + u.y * v.y;
+               pi += u.x * v.y - u.y * v.x;
+            }
+            rr += (double)nr;
+            zr += (double)pr;
+            zi += (double)pi;
+         }
+      }
+      wg_sum3(rr, zr, zi);
+      float r = (float)rr;
+      if (r < 2.0f * FLT_MIN) break;
+      r = 1.0f / r;
+      float2 z = make_float2((float)zr * r, (float)zi * r);
+      for (int i = threadIdx.x; i < nv; i += SWG)
+      {
+         float2 *p0 = S0 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         float2 *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         const float2 *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            float2 s1 = p1[c * SB], s2 = p2[c * SB], s0 = p0[c * SB];
+            p0[c * SB] = make_float2(s0.x + (z.x * s1.x - z.y * s1.y), s0.y + (z.x * s1.y + z.y * s1.x));
+            p1[c * SB] = make_float2(s1.x - (z.x * s2.x - z.y * s2.y), s1.y - (z.x * s2.y + z.y * s2.x));
+         }
+      }
+      __syncthreads();
+   }
+}
+
+// isolv=1 preparation: S1_o = A_oo^-1 S1_o, then S1_e -= D_eo S1_o
+// (Dwoo_blk, Dweo_blk of sap.c:759-761)
+__device__ void blk_eo_prepare(const SArgs &a, const int *imb, float2 *S1)
+{
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   {
+      int gx = imb[i];
+      float2 v[12], o[12];
+      if (a.masked && a.bnd[gx])
+         zero12(o);
+      else
+      {
+         load_spinor((const float2 *)S1, i, v);
+         pauli_site(a.sw, gx, 0.0f, v, o);
+      }
+      store_spinor(S1, i, o);
+   }
+   __syncthreads();
+   for (int i = threadIdx.x; i < a.bh; i += SWG)
+   {
+      int gx = imb[i];
+      float2 o[12];
+      if (a.masked && a.bnd[gx])
+         zero12(o);
+      else
+      {
+         float2 acc[12];
+         zero12(acc);
+         blk_hop(a, imb, S1, i, gx, acc);
+         load_spinor((const float2 *)S1, i, o);
+#pragma unroll
+         for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+      }
+      store_spinor(S1, i, o);
+   }
+   __syncthreads();
+}
+
+// isolv=1 completion: odd solution (Dwoe_blk, Dwoo_blk; update_flds1/2)
+__device__ void blk_eo_finish(const SArgs &a, const int *imb, float2 *S0, float2 *S1)
+{
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   {
+      int gx = imb[i];
+      float2 o[12];
+      if (a.masked && a.bnd[gx])
+         zero12(o);
+      else
+      {
+         float2 acc[12];
+         zero12(acc);
+         blk_hop(a, imb, S0, i, gx, acc);
+#pragma unroll
+         for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
+         pauli_site(a.sw, gx, 0.0f, acc, o);
+      }
+      // s0_o = s1_o - A_oo^-1 D_oe s0_e
+      float2 r1[12];
+      load_spinor((const float2 *)S1, i, r1);
+#pragma unroll
+      for (int c = 0; c < 12; c++) o[c] = csub(r1[c], o[c]);
+      store_spinor(S0, i, o);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// the Schwarz sweep over the blocks of one colour
+// ---------------------------------------------------------------------------
+
+template <int ISOLV>
+__global__ __launch_bounds__(SWG) void k_sap_blocks(SArgs a)
+{
+   int n = a.n0 + blockIdx.x;
+   const int *imb = a.imb + (size_t)n * a.bvol;
+   float2 *S0 = a.scr + (size_t)blockIdx.x * 3 * a.Ub;
+   float2 *S1 = S0 + a.Ub, *S2 = S1 + a.Ub;
+
+   for (int i = threadIdx.x; i < a.bvol; i += SWG)
+   {
+      int gx = imb[i];
+      float2 v[12];
+      if (!ISOLV && a.masked && a.bnd[gx])
+         zero12(v);
+      else
+         load_spinor((const float2 *)a.rho, gx, v);
+      store_spinor(S1, i, v);
+   }
+   __syncthreads();
+
+   if (ISOLV)
+   {
+      blk_eo_prepare(a, imb, S1);
+      blk_mres_dev<true>(a, imb, S0, S1, S2, a.mu);
+      blk_eo_finish(a, imb, S0, S1);
+      __syncthreads();
+   }
+   else
+      blk_mres_dev<false>(a, imb, S0, S1, S2, a.mu);
+
+   for (int i = threadIdx.x; i < a.bvol; i += SWG)
+   {
+      int gx = imb[i];
+      float2 s0[12], s1[12], p[12];
+      load_spinor((const float2 *)S0, i, s0);
+      load_spinor((const float2 *)S1, i, s1);
+      load_spinor((const float2 *)a.psi, gx, p);
+#pragma unroll
+      for (int c = 0; c < 12; c++) p[c] = cadd(p[c], s0[c]);
+      store_spinor(a.psi, gx, p);
+      if (ISOLV && i >= a.bh)
+      {
+         // odd residual -i mu gamma5 s0 (update_flds1) or 0 (eoflg, update_flds2)
+         float m = a.eoflg ? 0.0f : a.mu;
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            float cm = (c < 6) ? -m : m;
+            s1[c] = make_float2(-cm * s0[c].y, cm * s0[c].x);
+         }
+      }
+      store_spinor(a.rho, gx, s1);
+      store_spinor(a.s0g, gx, s0);
+   }
+}
+
+// rho(y) -= hopping terms into y from the block corrections of the
+// neighbouring blocks, for the face points y of the blocks of one colour
+__global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, const unsigned char *fmask, int nf)
+{
+   int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= nf) return;
+   int y = face[k];
+   int m = fmask[k];
+   float2 acc[12], sp[12];
+   zero12(acc);
+   int odd = y >= a.h;
+#define OQH_FACE(MU)                                                                   \
+   if (m & (1 << (2 * MU)))                                                           \
+   {                                                                                  \
+      int x = a.nbr[(size_t)(2 * MU) * a.V + y] & 0x7fffffff;                         \
+      load_spinor((const float2 *)a.s0g, x, sp);                                      \
+      const float2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU) : link_ptr(a.u, x - a.h, 2 * MU + 1); \
+      hop_one<float2, MU, true>(sp, up, acc);                                         \
+   }                                                                                  \
+   if (m & (2 << (2 * MU)))                                                           \
+   {                                                                                  \
+      int x = a.nbr[(size_t)(2 * MU + 1) * a.V + y] & 0x7fffffff;                     \
+      load_spinor((const float2 *)a.s0g, x, sp);                                      \
+      const float2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU + 1) : link_ptr(a.u, x - a.h, 2 * MU); \
+      hop_one<float2, MU, false>(sp, up, acc);                                        \
+   }
+   OQH_FACE(0)
+   OQH_FACE(1)
+   OQH_FACE(2)
+   OQH_FACE(3)
+#undef OQH_FACE
+   float2 r[12];
+   load_spinor((const float2 *)a.rho, y, r);
+#pragma unroll
+   for (int c = 0; c < 12; c++) r[c] = cadd(r[c], cscale(acc[c], 0.5f));
+   store_spinor(a.rho, y, r);
+}
+
+// ---------------------------------------------------------------------------
+// block operators on the block fields of the API (one work-group per call)
+// ---------------------------------------------------------------------------
+
+enum BlkOp { BOP_DW = 0, BOP_DWHAT, BOP_MRES, BOP_EO_MRES };
+
+__global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, int l, float mu)
+{
+   const int *imb = a.imb + (size_t)n * a.bvol;
+   float2 *F = a.scr + (size_t)n * 3 * a.Ub;  // the block's fields 0,1,2
+   float2 *Sk = F + (size_t)k * a.Ub, *Sl = F + (size_t)l * a.Ub;
+   // open bc: the operators zero their input on the boundary points (all of
+   // them for Dw_blk, the even ones for Dwhat_blk); in the MinRes solvers
+   // the input is field 1
+   if (a.masked)
+   {
+      bool eo = (op == BOP_DWHAT || op == BOP_EO_MRES);
+      float2 *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
+      for (int i = threadIdx.x; i < a.bvol; i += SWG)
+         if (a.bnd[imb[i]] && (!eo || i < a.bh)) zero_spinor(In, i);
+      __syncthreads();
+   }
+   if (op == BOP_DW)
+      blk_dw(a, imb, Sk, Sl, 0, a.bvol, mu);
+   else if (op == BOP_DWHAT)
+   {
+      // Dwhat_blk keeps the odd half of the output: use field 2 only if l != 2
+      blk_dwhat(a, imb, Sk, Sl, mu);
+   }
+   else if (op == BOP_MRES)
+      blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
+   else
+      blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
+}
+
+__global__ void k_blk_copy(float2 *glob, float2 *blk, const int *imb, int i0, int i1, int to_blk)
+{
+   int i = i0 + blockIdx.x * 256 + threadIdx.x;
+   if (i >= i1) return;
+   int gx = imb[i];
+   float2 v[12];
+   if (to_blk)
+   {
+      load_spinor((const float2 *)glob, gx, v);
+      store_spinor(blk, i, v);
+   }
+   else
+   {
+      load_spinor((const float2 *)blk, i, v);
+      store_spinor(glob, gx, v);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// host side: block grid
+// ---------------------------------------------------------------------------
+
+static int lexb(const int *bs, const int *y) { return y[3] + bs[3] * (y[2] + bs[2] * (y[1] + bs[1] * y[0])); }
+
+static void free_grid()
+{
+   dev_free(G.d_imb); dev_free(G.d_bnb);
+   for (int c = 0; c < 2; c++) { dev_free(G.d_face[c]); dev_free(G.d_fmask[c]); G.d_face[c] = nullptr; G.d_fmask[c] = nullptr; }
+   dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
+   G = SapGrid();
+}
+
+static bool build_grid(const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return false;
+   if (L.sap_ncy == 0)
+   {
+      fail(where, "SAP parameters are not set");
+      return false;
+   }
+   if (G.ready && memcmp(G.bs, L.sap_bs, sizeof(G.bs)) == 0) return true;
+   if (G.ready) free_grid();
+   const int *bs = L.sap_bs;
+   int nbl[4], cb[4], nbs[4];
+   for (int mu = 0; mu < 4; mu++) nbl[mu] = L.g.L[mu] / bs[mu];
+   memcpy(G.bs, bs, sizeof(G.bs));
+   G.nb = nbl[0] * nbl[1] * nbl[2] * nbl[3];
+   G.nbh = G.nb / 2;
+   G.bvol = bs[0] * bs[1] * bs[2] * bs[3];
+   G.bh = G.bvol / 2;
+   G.Ub = units_for(G.bvol, 12);
+
+   // block-local point order (blk_geometry.c:56-128): cache blocks, even first
+   std::vector<int> bipt(G.bvol);
+   cb[0] = bs[0];
+   for (int mu = 1; mu < 4; mu++) cb[mu] = (bs[mu] % 4 == 0) ? 4 : ((bs[mu] % 3 == 0) ? 3 : 2);
+   for (int mu = 0; mu < 4; mu++) nbs[mu] = bs[mu] / cb[mu];
+   {
+      int ofs[2] = {0, G.bh}, j[4], x[4], y[4];
+      for (j[0] = 0; j[0] < nbs[0]; j[0]++)
+      for (j[1] = 0; j[1] < nbs[1]; j[1]++)
+      for (j[2] = 0; j[2] < nbs[2]; j[2]++)
+      for (j[3] = 0; j[3] < nbs[3]; j[3]++)
+         for (x[0] = 0; x[0] < cb[0]; x[0]++)
+         for (x[1] = 0; x[1] < cb[1]; x[1]++)
+         for (x[2] = 0; x[2] < cb[2]; x[2]++)
+         for (x[3] = 0; x[3] < cb[3]; x[3]++)
+         {
+            for (int mu = 0; mu < 4; mu++) y[mu] = j[mu] * cb[mu] + x[mu];
+            bipt[lexb(bs, y)] = ofs[(y[0] + y[1] + y[2] + y[3]) & 1]++;
+         }
+   }
+   std::vector<int> bnb(8 * (size_t)G.bvol), bcoord(4 * (size_t)G.bvol);
+   {
+      int y[4];
+      for (y[0] = 0; y[0] < bs[0]; y[0]++)
+      for (y[1] = 0; y[1] < bs[1]; y[1]++)
+      for (y[2] = 0; y[2] < bs[2]; y[2]++)
+      for (y[3] = 0; y[3] < bs[3]; y[3]++)
+      {
+         int i = bipt[lexb(bs, y)];
+         for (int mu = 0; mu < 4; mu++) bcoord[4 * (size_t)i + mu] = y[mu];
+         for (int mu = 0; mu < 4; mu++)
+         {
+            int z[4];
+            memcpy(z, y, sizeof(z));
+            z[mu] = y[mu] + 1;
+            bnb[(size_t)(2 * mu) * G.bvol + i] = (z[mu] < bs[mu]) ? bipt[lexb(bs, z)] : G.bvol;
+            z[mu] = y[mu] - 1;
+            bnb[(size_t)(2 * mu + 1) * G.bvol + i] = (z[mu] >= 0) ? bipt[lexb(bs, z)] : G.bvol;
+         }
+      }
+   }
+   // blocks: colour-0 blocks first, lexicographic (blk_grid.c:63-113)
+   G.imb.assign((size_t)G.nb * G.bvol, 0);
+   G.bo.assign(4 * (size_t)G.nb, 0);
+   std::vector<int> colour_of_site(L.g.V, 0), inface(L.g.V, 0);
+   std::vector<unsigned char> fm(L.g.V, 0);
+   int ib = 0, m[4];
+   for (int col = 0; col < 2; col++)
+      for (m[0] = 0; m[0] < nbl[0]; m[0]++)
+      for (m[1] = 0; m[1] < nbl[1]; m[1]++)
+      for (m[2] = 0; m[2] < nbl[2]; m[2]++)
+      for (m[3] = 0; m[3] < nbl[3]; m[3]++)
+      {
+         if (((m[0] + m[1] + m[2] + m[3]) & 1) != col) continue;
+         for (int mu = 0; mu < 4; mu++) G.bo[4 * (size_t)ib + mu] = m[mu] * bs[mu];
+         for (int i = 0; i < G.bvol; i++)
+         {
+            int g[4];
+            for (int mu = 0; mu < 4; mu++) g[mu] = m[mu] * bs[mu] + bcoord[4 * (size_t)i + mu];
+            int gx = L.g.ipt[g[3] + L.g.L[3] * (g[2] + L.g.L[2] * (g[1] + L.g.L[1] * g[0]))];
+            G.imb[(size_t)ib * G.bvol + i] = gx;
+            colour_of_site[gx] = col;
+            unsigned char mk = 0;
+            for (int mu = 0; mu < 4; mu++)
+            {
+               if (bnb[(size_t)(2 * mu) * G.bvol + i] == G.bvol) mk |= (unsigned char)(1 << (2 * mu));
+               if (bnb[(size_t)(2 * mu + 1) * G.bvol + i] == G.bvol) mk |= (unsigned char)(2 << (2 * mu));
+            }
+            fm[gx] = mk;
+         }
+         ib++;
+      }
+   std::vector<int> face[2];
+   std::vector<unsigned char> fmask[2];
+   for (int ix = 0; ix < L.g.V; ix++)
+      if (fm[ix])
+      {
+         face[colour_of_site[ix]].push_back(ix);
+         fmask[colour_of_site[ix]].push_back(fm[ix]);
+      }
+   G.d_imb = (int *)dev_alloc(sizeof(int) * G.imb.size());
+   G.d_bnb = (int *)dev_alloc(sizeof(int) * bnb.size());
+   HIP_OK(hipMemcpy(G.d_imb, G.imb.data(), sizeof(int) * G.imb.size(), hipMemcpyHostToDevice));
+   HIP_OK(hipMemcpy(G.d_bnb, bnb.data(), sizeof(int) * bnb.size(), hipMemcpyHostToDevice));
+   for (int c = 0; c < 2; c++)
+   {
+      G.nface[c] = (int)face[c].size();
+      G.d_face[c] = (int *)dev_alloc(sizeof(int) * (face[c].size() + 1));
+      G.d_fmask[c] = (unsigned char *)dev_alloc(fmask[c].size() + 1);
+      if (G.nface[c])
+      {
+         HIP_OK(hipMemcpy(G.d_face[c], face[c].data(), sizeof(int) * face[c].size(), hipMemcpyHostToDevice));
+         HIP_OK(hipMemcpy(G.d_fmask[c], fmask[c].data(), fmask[c].size(), hipMemcpyHostToDevice));
+      }
+   }
+   G.d_swb = (float2 *)dev_alloc(units_for(L.g.V, 36) * sizeof(float2));
+   G.d_scr = (float2 *)dev_alloc((size_t)G.nbh * 3 * G.Ub * sizeof(float2));
+   G.d_s0g = (float2 *)dev_alloc(units_for(L.g.V, 12) * sizeof(float2));
+   HIP_OK(hipMemset(G.d_s0g, 0, units_for(L.g.V, 12) * sizeof(float2)));
+   G.swb_set = -1;
+   G.ready = true;
+   return true;
+}
+
+// FP32 SW field of the block solver: single-precision copy of swd with the
+// points of the set inverted in double precision first (map_sw2blk.c:53-100)
+static int assign_swb(int set)
+{
+   Lib &L = lib();
+   ensure_swd(0, 0);  // as sap_gcr.c:181-197: the SW field must not be inverted
+   int h = L.g.V / 2, ifail = 0;
+   size_t nun = units_for(L.g.V, 36);
+   double2 *tmp = (double2 *)tmp_buffer(nun * sizeof(double2));
+   HIP_OK(hipMemcpyAsync(tmp, L.d_swd, nun * sizeof(double2), hipMemcpyDeviceToDevice, L.stream));
+   if (set == ALL_PTS || set == EVEN_PTS) ifail |= invert_pauli(tmp, tmp, 0, h, 0.0);
+   if (set == ALL_PTS || set == ODD_PTS) ifail |= invert_pauli(tmp, tmp, h, h, 0.0);
+   convert_f64_to_f32(tmp, G.d_swb, nun, L.stream);
+   HIP_OK(hipStreamSynchronize(L.stream));
+   G.swb_set = set;
+   G.swb_epoch = L.swd_epoch;
+   return ifail;
+}
+
+static SArgs sargs(float mu, int nmr)
+{
+   Lib &L = lib();
+   SArgs a;
+   a.u = L.d_u;
+   a.sw = G.d_swb;
+   a.imb = G.d_imb;
+   a.bnb = G.d_bnb;
+   a.nbr = L.d_nbr;
+   a.bnd = L.d_bnd;
+   a.V = L.g.V;
+   a.h = L.g.V / 2;
+   a.bvol = G.bvol;
+   a.bh = G.bh;
+   a.masked = (L.bc != 3);
+   a.eoflg = L.eoflg;
+   a.nmr = nmr;
+   a.mu = mu;
+   a.muo = L.eoflg ? 0.0f : mu;
+   a.Ub = G.Ub;
+   a.psi = a.rho = nullptr;
+   a.s0g = G.d_s0g;
+   a.scr = G.d_scr;
+   a.n0 = 0;
+   return a;
+}
+
+// one SAP cycle on device fields (no synchronisation)
+void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
+{
+   Lib &L = lib();
+   SArgs a = sargs(mu, nmr);
+   a.psi = psi;
+   a.rho = rho;
+   for (int ic = 0; ic < 2; ic++)
+   {
+      a.n0 = ic ? G.nbh : 0;
+      if (isolv)
+         k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
+      else
+         k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
+      int c = 1 - ic;
+      if (G.nface[c]) k_sap_faces<<<(G.nface[c] + 255) / 256, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c]);
+   }
+}
+
+// grid, single-precision links and block SW field for the solver; returns
+// false if the odd-point inversion is unsafe (reported as ifail=-3 by
+// sap_gcr, as an error by sap)
+bool sap_prepare(int isolv, const char *where, bool quiet)
+{
+   Lib &L = lib();
+   if (!build_grid(where)) return false;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return false;
+   }
+   ensure_u();
+   int want = isolv ? ODD_PTS : NO_PTS;
+   if (!L.swd_valid || L.swd_inv[0] || L.swd_inv[1]) ensure_swd(0, 0);
+   if (G.swb_set != want || G.swb_epoch != L.swd_epoch)
+   {
+      if (assign_swb(want))
+      {
+         G.swb_set = -1;
+         if (!quiet) fail(where, "The inversion of the SW term was not safe");
+         return false;
+      }
+   }
+   return true;
+}
+
+int sap_grid_nblocks() { return G.ready ? G.nb : 0; }
+
+}  // namespace oqh
+
+using namespace oqh;
+
+extern "C" void sap(float mu, int isolv, int nmr, spinor *psi, spinor *eta)
+{
+   Lib &L = lib();
+   if (!sap_prepare(isolv, "sap [sap.c]", false)) return;
+   int V = L.g.V;
+   Res p = resolve(psi, F32, V, 0, true);
+   Res r = resolve(eta, F32, V, 1, true);
+   if (!p.dev || !r.dev) return;
+   if (p.off || r.off)
+   {
+      fail("sap [sap.c]", "the fields must be full fields");
+      return;
+   }
+   sap_device(mu, isolv, nmr, (float2 *)p.dev, (float2 *)r.dev);
+   finish(p, F32, V, true);
+   finish(r, F32, V, true);
+}
+
+extern "C" void alloc_bgr(blk_grid_t grid)
+{
+   if (grid != SAP_BLOCKS)
+   {
+      fail("alloc_bgr [blk_grid.c]", "only the SAP_BLOCKS grid is provided");
+      return;
+   }
+   build_grid("alloc_bgr [blk_grid.c]");
+}
+
+extern "C" void assign_ud2ubgr(blk_grid_t grid)
+{
+   (void)grid;
+   Lib &L = lib();
+   if (!require_geometry("assign_ud2ubgr [map_u2blk.c]")) return;
+   L.u_valid = false;
+   ensure_u();
+}
+
+extern "C" int assign_swd2swbgr(blk_grid_t grid, ptset_t set)
+{
+   if (grid != SAP_BLOCKS || !build_grid("assign_swd2swbgr [map_sw2blk.c]")) return 1;
+   return assign_swb((int)set);
+}
+
+static bool blk_check(int n, int k, const char *where)
+{
+   if (!build_grid(where)) return false;
+   if (n < 0 || n >= G.nb || k < 0 || k > 2)
+   {
+      fail(where, "Block number or field index is out of range");
+      return false;
+   }
+   if (!G.d_bs) G.d_bs = (float2 *)dev_alloc((size_t)G.nb * 3 * G.Ub * sizeof(float2));
+   return true;
+}
+
+static void blk_copy(int n, ptset_t set, int k, spinor *s, int to_blk, const char *where)
+{
+   Lib &L = lib();
+   if (!blk_check(n, k, where)) return;
+   Res r = resolve(s, F32, L.g.V, 0, true);
+   if (!r.dev) return;
+   int i0 = (set == ODD_PTS) ? G.bh : 0, i1 = (set == EVEN_PTS) ? G.bh : G.bvol;
+   if (set == NO_PTS) i1 = i0;
+   float2 *blk = G.d_bs + ((size_t)n * 3 + k) * G.Ub;
+   if (i1 > i0)
+      k_blk_copy<<<(i1 - i0 + 255) / 256, 256, 0, L.stream>>>((float2 *)r.dev, blk, G.d_imb + (size_t)n * G.bvol,
+                                                              i0, i1, to_blk);
+   finish(r, F32, L.g.V, !to_blk);
+}
+
+extern "C" void assign_s2sblk(blk_grid_t grid, int n, ptset_t set, spinor *s, int k)
+{
+   (void)grid;
+   blk_copy(n, set, k, s, 1, "assign_s2sblk [map_s2blk.c]");
+}
+
+extern "C" void assign_sblk2s(blk_grid_t grid, int n, ptset_t set, int k, spinor *s)
+{
+   (void)grid;
+   blk_copy(n, set, k, s, 0, "assign_sblk2s [map_s2blk.c]");
+}
+
+static void blk_op(int n, int op, int k, int l, float mu, int nmr, const char *where)
+{
+   Lib &L = lib();
+   if (!blk_check(n, k, where) || !blk_check(n, l, where)) return;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return;
+   }
+   ensure_u();
+   if (G.swb_set < 0) assign_swb(L.sap_isolv ? ODD_PTS : NO_PTS);
+   SArgs a = sargs(mu, nmr);
+   a.scr = G.d_bs;
+   k_blk_op<<<1, SWG, 0, L.stream>>>(a, n, op, k, l, mu);
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+extern "C" void Dw_blk(blk_grid_t grid, int n, float mu, int k, int l)
+{
+   (void)grid;
+   if (k == l)
+   {
+      fail("Dw_blk [Dw.c]", "Input and output fields must be different");
+      return;
+   }
+   blk_op(n, BOP_DW, k, l, mu, 0, "Dw_blk [Dw.c]");
+}
+
+extern "C" void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l)
+{
+   (void)grid;
+   if (k == l)
+   {
+      fail("Dwhat_blk [Dw.c]", "Input and output fields must be different");
+      return;
+   }
+   blk_op(n, BOP_DWHAT, k, l, mu, 0, "Dwhat_blk [Dw.c]");
+}
+
+extern "C" void blk_mres(int n, float mu, int nmr)
+{
+   blk_op(n, BOP_MRES, 0, 2, mu, nmr, "blk_mres [blk_solv.c]");
+}
+
+extern "C" void blk_eo_mres(int n, float mu, int nmr)
+{
+   blk_op(n, BOP_EO_MRES, 0, 2, mu, nmr, "blk_eo_mres [blk_solv.c]");
+}
+
+extern "C" void oqh_export_sblk(int n, int k, spinor *host)
+{
+   Lib &L = lib();
+   if (!blk_check(n, k, "oqh_export_sblk")) return;
+   size_t bytes = (size_t)G.bvol * sizeof(spinor);
+   void *aos = tmp_buffer(bytes);
+   blocks_to_aos_f32(G.d_bs + ((size_t)n * 3 + k) * G.Ub, (float2 *)aos, G.bvol, 12, L.stream);
+   HIP_OK(hipMemcpyAsync(host, aos, bytes, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+extern "C" void oqh_get_imb(int n, int *imb)
+{
+   if (!build_grid("oqh_get_imb")) return;
+   memcpy(imb, G.imb.data() + (size_t)n * G.bvol, sizeof(int) * G.bvol);
+}

@@ -1,0 +1,184 @@
+// solver.hip -- flexible GCR and the SAP+GCR driver.
+//
+// Replaces linsolv/fgcr.c:109-299 and sap/sap_gcr.c:120-254.  The algorithm
+// and its precision structure are those of the reference: single-precision
+// Krylov vectors phi (preconditioned directions) and chi (their images),
+// modified Gram-Schmidt with double-precision accumulation, restart after nkv
+// vectors with the solution update psi += sum c_l phi_l in double precision
+// and the residual recomputed as rho = eta - D psi with the double-precision
+// operator.  Stopping: rn <= res*rn0, stagnation rn < 100*FLT_EPSILON*rn_old,
+// k = nkv-1 or nmx Krylov vectors in total.  All vectors are device fields;
+// the host keeps only the small arrays a, b, c.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "oqh_internal.h"
+
+namespace oqh {
+bool sap_prepare(int isolv, const char *where, bool quiet);
+}
+
+using namespace oqh;
+
+static double get_res(int vol, int istop, spinor *rho)
+{
+   if (istop) return (double)unorm(vol, 3, rho);
+   double rn = (double)norm_square(vol, 3, rho);
+   return sqrt(rn);
+}
+
+extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
+                       void (*Mop)(int k, spinor *rho, spinor *phi, spinor *chi), spinor **ws,
+                       spinor_dble **wsd, int nkv, int nmx, int istop, double res, spinor_dble *eta,
+                       spinor_dble *psi, int *status)
+{
+   if (vol <= 0 || nkv < 1 || nmx < 1 || res <= DBL_EPSILON || istop < 0 || istop > 1)
+   {
+      fail("fgcr [fgcr.c]", "Parameters are out of range");
+      return 0.0;
+   }
+   std::vector<complex> a((size_t)nkv * nkv), c(nkv);
+   std::vector<float> b(nkv);
+   spinor **phi = ws, *rho = ws[nkv], **chi = ws + nkv + 1;
+   spinor_dble *wrk = wsd[0];
+   const double PRECISION_LIMIT = (double)(100.0f * FLT_EPSILON);
+
+   set_sd2zero(vol, 2, psi);
+   assign_sd2s(vol, 2, eta, rho);
+   double rn = get_res(vol, istop, rho), tol = res * rn, rn_old;
+   status[0] = 0;
+
+   while (rn > tol)
+   {
+      int k;
+      rn_old = rn;
+      for (k = 0;; k++)
+      {
+         // gcr_step (fgcr.c:157-178)
+         Mop(k, rho, phi[k], chi[k]);
+         for (int l = 0; l < k; l++)
+         {
+            a[(size_t)nkv * l + k] = spinor_prod(vol, 3, chi[l], chi[k]);
+            complex z;
+            z.re = -a[(size_t)nkv * l + k].re;
+            z.im = -a[(size_t)nkv * l + k].im;
+            mulc_spinor_add(vol, 2, chi[k], chi[l], z);
+         }
+         b[k] = normalize(vol, 3, chi[k]);
+         c[k] = spinor_prod(vol, 3, chi[k], rho);
+         complex z;
+         z.re = -c[k].re;
+         z.im = -c[k].im;
+         mulc_spinor_add(vol, 2, rho, chi[k], z);
+         rn = get_res(vol, istop, rho);
+         status[0] += 1;
+         if (rn <= tol || rn < PRECISION_LIMIT * rn_old || k == nkv - 1 || status[0] >= nmx) break;
+      }
+      // update_psi (fgcr.c:181-212)
+      for (int l = k; l >= 0; l--)
+      {
+         float zr = c[l].re, zi = c[l].im;
+         for (int i = l + 1; i <= k; i++)
+         {
+            zr -= (a[(size_t)l * nkv + i].re * c[i].re - a[(size_t)l * nkv + i].im * c[i].im);
+            zi -= (a[(size_t)l * nkv + i].re * c[i].im + a[(size_t)l * nkv + i].im * c[i].re);
+         }
+         float r = 1.0f / b[l];
+         c[l].re = zr * r;
+         c[l].im = zi * r;
+      }
+      set_s2zero(vol, 2, rho);
+      for (int l = k; l >= 0; l--) mulc_spinor_add(vol, 2, rho, phi[l], c[l]);
+      add_s2sd(vol, 2, rho, psi);
+      Dop(psi, wrk);
+      diff_sd2s(vol, 2, eta, wrk, rho);
+      rn = get_res(vol, istop, rho);
+      if (status[0] >= nmx && rn > tol)
+      {
+         status[0] = -1;
+         return rn;
+      }
+   }
+   return rn;
+}
+
+// ---------------------------------------------------------------------------
+// sap_gcr
+// ---------------------------------------------------------------------------
+
+static double mud;
+
+static void Dop(spinor_dble *s, spinor_dble *r) { Dw_dble(mud, s, r); }
+
+static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
+{
+   (void)k;
+   Lib &L = lib();
+   int vt = L.g.VT;
+   set_s2zero(vt, 2, phi);
+   assign_s2s(vt, 2, rho, chi);
+   for (int n = 0; n < L.sap_ncy; n++) sap((float)mud, L.sap_isolv, L.sap_nmr, phi, chi);
+   diff_s2s(vt, 2, rho, chi);
+}
+
+extern "C" double sap_gcr(int nkv, int nmx, int istop, double res, double mu, spinor_dble *eta,
+                          spinor_dble *psi, int *ifail, int *status)
+{
+   Lib &L = lib();
+   if (!require_geometry("sap_gcr [sap_gcr.c]")) return 0.0;
+   if (L.sap_ncy == 0)
+   {
+      fail("sap_gcr [sap_gcr.c]", "SAP parameters are not set");
+      return 0.0;
+   }
+   int vt = L.g.VT;
+   ifail[0] = 0;
+   status[0] = 0;
+   double rho = 0.0;
+
+   if (!sap_prepare(L.sap_isolv, "sap_gcr [sap_gcr.c]", true))
+      ifail[0] = -3;
+   else
+   {
+      double rho0 = unorm_dble(vt, 3, eta);
+      if (rho0 != 0.0)
+      {
+         alloc_ws(L.ws_used + 2 * nkv + 1);
+         alloc_wsd(L.wsd_used + 2);
+         spinor **ws = reserve_ws(2 * nkv + 1);
+         spinor_dble **wsd = reserve_wsd(2);
+         spinor_dble *rsd = wsd[1];
+         mud = mu;
+         assign_sd2sd(vt, 2, eta, rsd);
+         scale_dble(vt, 2, 1.0 / rho0, rsd);
+         rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+         scale_dble(vt, 2, rho0, psi);
+         rho *= rho0;
+         if (status[0] < 0)
+         {
+            ifail[0] = -1;
+            status[0] = nmx;
+         }
+         release_wsd();
+         release_ws();
+      }
+      else
+         set_sd2zero(vt, 2, psi);
+   }
+
+   if (ifail[0] <= -2)
+   {
+      if (psi != eta) set_sd2zero(vt, 2, psi);
+      if (istop)
+         rho = unorm_dble(vt, 3, eta);
+      else
+      {
+         qflt q = norm_square_dble(vt, 3, eta);
+         rho = sqrt(q.q[0]);
+      }
+   }
+   return rho;
+}

@@ -613,6 +613,7 @@ extern "C" void oqh_import_swd(const pauli_dble *swd)
    L.swd_valid = true;
    L.swd_inv[0] = L.swd_inv[1] = false;
    L.sw_valid = false;
+   L.swd_epoch++;
 }
 
 extern "C" spinor *oqh_alloc_s(int n)

@@ -364,6 +364,7 @@ static void compute_swd()
    L.swd_valid = true;
    L.swd_inv[0] = L.swd_inv[1] = false;
    L.sw_valid = false;
+   L.swd_epoch++;
 }
 
 // make the double-precision SW field current, with the even/odd parts
@@ -383,12 +384,14 @@ int ensure_swd_ret(int want_e, int want_o)
       ifail |= invert_pauli(L.d_swd, L.d_swd, 0, h, 0.0);
       L.swd_inv[0] = true;
       L.sw_valid = false;
+      L.swd_epoch++;
    }
    if (want_o && !L.swd_inv[1])
    {
       ifail |= invert_pauli(L.d_swd, L.d_swd, h, h, 0.0);
       L.swd_inv[1] = true;
       L.sw_valid = false;
+      L.swd_epoch++;
    }
    return ifail;
 }

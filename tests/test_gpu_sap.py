@@ -1,0 +1,155 @@
+"""GPU parity of the SAP block solver, the Schwarz sweep and SAP+GCR.
+
+FP32 block quantities are compared with the oracle to a relative 1e-5 after
+one operator application and to 1e-3 after several Schwarz cycles (rounding
+differences of the single-precision recursion grow with the number of steps).
+Solutions of SAP+GCR are judged like the reference's devel/sap/check3.c: the
+double-precision residual |eta - Dw_dble psi| / |eta|, recomputed by the
+oracle on the exported solution, must reach the requested tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoqh
+from helpers import make_oracle, site_rel
+
+P = C.c_void_p
+
+CFGS = [
+    dict(name="8^4-periodic-isolv0", L=(8, 8, 8, 8), bc=3, m0=-0.5, csw=1.0, seed=1, isolv=0, mu=0.0),
+    dict(name="8^4-periodic-isolv1-mu", L=(8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77, seed=3, isolv=1, mu=0.05),
+    dict(name="8^4-open-isolv0", L=(8, 8, 8, 8), bc=0, m0=-0.3, csw=1.77, seed=2, isolv=0, mu=0.0),
+    dict(name="16x8^3-open-isolv1", L=(16, 8, 8, 8), bc=0, m0=-0.3, csw=1.77, seed=5, isolv=1, mu=0.1),
+]
+
+
+def _sig(lib):
+    lib.sap.argtypes = [C.c_float, C.c_int, C.c_int, P, P]
+    lib.sap_gcr.restype = C.c_double
+    lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.oqh_set_sap_parms.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    lib.assign_s2sblk.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int]
+    lib.Dw_blk.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int]
+    lib.blk_mres.argtypes = [C.c_int, C.c_float, C.c_int]
+    lib.oqh_export_sblk.argtypes = [C.c_int, C.c_int, P]
+    lib.oqh_get_imb.argtypes = [C.c_int, P]
+
+
+@pytest.fixture(params=CFGS, ids=lambda c: c["name"])
+def setup(request):
+    c = request.param
+    print("config", c, flush=True)
+    o, eta, s32 = make_oracle(c["L"], None, bc=c["bc"], m0=c["m0"], csw=c["csw"], seed=c["seed"])
+    lat = pyoqh.Lattice(c["L"], None, bc=c["bc"], m0=c["m0"], csw=c["csw"])
+    _sig(lat.lib)
+    lat.import_ud(o.ud())
+    bs = np.array([4, 4, 4, 4], np.int32)
+    lat.lib.oqh_set_sap_parms(bs.ctypes.data_as(P), c["isolv"], 4, 5)
+    o.set_sap((4, 4, 4, 4), c["isolv"], 4, 5)
+    o.sw_term(3)
+    yield c, o, eta, s32, lat
+    lat.close()
+
+
+@pytest.mark.gpu
+def test_block_operator_and_mres(setup):
+    c, o, eta, s32, lat = setup
+    if c["isolv"]:
+        pytest.skip("block Dw/MinRes compared on the isolv=0 grid")
+    assert o.sap_prepare() == 0
+    s = lat.alloc_s()
+    lat.put_s(s, s32)
+    for n in (0, o.nblocks() - 1):
+        imb = np.empty(256, np.int32)
+        lat.lib.oqh_get_imb(n, imb.ctypes.data_as(P))
+        assert np.array_equal(imb, o.block_imb(n))
+        lat.lib.assign_s2sblk(0, n, 0, s, 1)
+        lat.lib.Dw_blk(0, n, c["mu"], 1, 2)
+        got = np.empty((256, 24), np.float32)
+        lat.lib.oqh_export_sblk(n, 2, got.ctypes.data_as(P))
+        s1 = np.vstack([s32[imb], np.zeros((1, 24), np.float32)])
+        ref = o.Dw_blk(n, np.float32(c["mu"]), s1.copy())[:-1]
+        rel, absz = site_rel(got, ref)
+        assert rel < 1e-5 and absz == 0.0
+        lat.lib.blk_mres(n, c["mu"], 4)
+        s0g = np.empty((256, 24), np.float32)
+        s1g = np.empty((256, 24), np.float32)
+        lat.lib.oqh_export_sblk(n, 0, s0g.ctypes.data_as(P))
+        lat.lib.oqh_export_sblk(n, 1, s1g.ctypes.data_as(P))
+        s0r, s1r = o.blk_mres(n, np.float32(c["mu"]), 4, s1.copy())
+        rel0, _ = site_rel(s0g, s0r[:-1])
+        rel1, _ = site_rel(s1g, s1r[:-1])
+        print("block", n, "Dw", rel, "mres", rel0, rel1)
+        assert rel0 < 1e-4 and rel1 < 1e-3
+    lat.free(s)
+
+
+@pytest.mark.gpu
+def test_sap_cycles(setup):
+    c, o, eta, s32, lat = setup
+    assert o.sap_prepare() == 0
+    psi, rho = lat.alloc_s(), lat.alloc_s()
+    lat.put_s(psi, np.zeros_like(s32))
+    lat.put_s(rho, s32)
+    psi_o = np.zeros_like(s32)
+    rho_o = s32.copy()
+    for cyc in range(3):
+        lat.lib.sap(c["mu"], c["isolv"], 4, psi, rho)
+        o.sap(np.float32(c["mu"]), c["isolv"], 4, psi_o, rho_o)
+    rel_p, _ = site_rel(lat.get_s(psi), psi_o)
+    rel_r, _ = site_rel(lat.get_s(rho), rho_o)
+    print("sap psi", rel_p, "rho", rel_r)
+    assert rel_p < 1e-3
+    # the residual itself becomes small; compare in norm relative to |eta|
+    g, r = lat.get_s(rho).astype(np.float64), rho_o.astype(np.float64)
+    assert np.linalg.norm(g - r) <= 1e-4 * np.linalg.norm(s32.astype(np.float64))
+    # SAP invariant (devel/sap/check2.c): rho = eta - D psi
+    o.assign_ud2u()
+    o.assign_swd2sw()
+    d = o.Dw(np.float32(c["mu"]), lat.get_s(psi).copy())
+    inv = np.linalg.norm(s32.astype(np.float64) - d - g) / np.linalg.norm(s32.astype(np.float64))
+    print("invariant", inv)
+    assert inv < 1e-5
+
+
+@pytest.mark.gpu
+def test_sap_gcr_solves(setup):
+    c, o, eta, s32, lat = setup
+    V = lat.V
+    e, p = lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(e, eta)
+    ifail, status = C.c_int(), C.c_int()
+    rho = lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    psi = lat.get_sd(p)
+    o.sw_term(3)
+    res = np.linalg.norm(eta - o.Dw_dble(c["mu"], psi.copy())) / np.linalg.norm(eta)
+    rho_o, ifail_o, status_o, psi_o = o.sap_gcr(16, 256, 0, 1e-10, c["mu"], eta.copy())
+    print("gcr", ifail.value, status.value, rho, "true residual", res, "oracle status", status_o)
+    assert ifail.value == 0 and ifail_o == 0
+    assert res <= 1.2e-10
+    assert abs(status.value - status_o) <= 1
+    assert np.array_equal(lat.get_sd(e), eta), "source must be preserved"
+    # psi == eta (in place), as allowed by sap_gcr
+    rho2 = lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, e, C.byref(ifail), C.byref(status))
+    assert ifail.value == 0
+    rel, _ = site_rel(lat.get_sd(e), psi)
+    assert rel < 1e-8
+
+
+@pytest.mark.gpu
+def test_sap_gcr_failure_modes(setup):
+    c, o, eta, s32, lat = setup
+    e, p = lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(e, eta)
+    ifail, status = C.c_int(), C.c_int()
+    # too few Krylov vectors: ifail = -1, status = nmx (sap_gcr.c:223-228)
+    lat.lib.sap_gcr(4, 2, 0, 1e-12, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    assert ifail.value == -1 and status.value == 2
+    # zero source: psi = 0
+    lat.put_sd(e, np.zeros_like(eta))
+    lat.put_sd(p, eta)
+    rho = lat.lib.sap_gcr(16, 64, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    assert rho == 0.0 and ifail.value == 0 and not lat.get_sd(p).any()
