@@ -103,6 +103,9 @@ void oqh_sync(void);
    with HIP events on the library stream; op: 0 Dw_dble, 1 Dw, 2 Dwhat_dble,
    3 Dwhat.  Returns the average milliseconds per call. */
 double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall);
+/* bandwidth (GB/s, bytes read + written) of a plain device copy of gbytes
+   in total, averaged over nrep launches */
+double oqh_copy_bandwidth(double gbytes, int nrep);
 
 /* SAP block grid access for tests: block-local field k (0..2) of block n in
    the block's own point order (vol spinors), and the embedding of the block

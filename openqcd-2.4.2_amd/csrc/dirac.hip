@@ -81,22 +81,53 @@ __device__ inline void hop_all(const DArgs<T> &a, const typename cplx_of<T>::t *
    hop_mu<T, 3>(a, in, x, odd, mask, acc);
 }
 
+// Work-groups are dispatched round-robin over the 8 XCDs of the chip, each
+// with its own L2.  Tile t of XCD j is made the t-th of a contiguous range of
+// tiles, so that the neighbour points a tile gathers were mostly brought into
+// the same L2 by the tiles processed next to it.
+#ifndef OQH_XCD_REMAP
+#define OQH_XCD_REMAP 1
+#endif
+__device__ inline int xcd_tile(int b, int nb)
+{
+#if OQH_XCD_REMAP
+   int q = nb >> 3, r = nb & 7, j = b & 7;
+   return j * q + min(j, r) + (b >> 3);
+#else
+   (void)nb;
+   return b;
+#endif
+}
+
+#ifndef OQH_DW_WAVES
+#define OQH_DW_WAVES 0
+#endif
+#ifndef OQH_NT_STORE
+#define OQH_NT_STORE 0
+#endif
+#if OQH_DW_WAVES
+#define DW_ATTR __attribute__((amdgpu_waves_per_eu(OQH_DW_WAVES)))
+#else
+#define DW_ATTR
+#endif
+
 template <typename T, int OP>
-__global__ __launch_bounds__(256) void k_dirac(DArgs<T> a)
+__global__ __launch_bounds__(256) DW_ATTR void k_dirac(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
    int x, odd;
+   int b = xcd_tile(blockIdx.x, gridDim.x);
    if (OP == OP_DW)
    {
       int tid = threadIdx.x;
-      int k = blockIdx.x * 128 + (tid & 127);
+      int k = b * 128 + (tid & 127);
       if (k >= a.h) return;
       odd = tid >> 7;
       x = k + odd * a.h;
    }
    else
    {
-      int k = blockIdx.x * 256 + threadIdx.x;
+      int k = b * 256 + threadIdx.x;
       if (k >= a.np) return;
       x = a.p0 + k;
       odd = x >= a.h;
@@ -120,7 +151,11 @@ __global__ __launch_bounds__(256) void k_dirac(DArgs<T> a)
       pauli_site(a.sw, x, odd ? a.muo : a.mu, v, o);
 #pragma unroll
       for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], a.coef));
+#if OQH_NT_STORE
+      store_spinor_nt(a.r, x, o);
+#else
       store_spinor(a.r, x, o);
+#endif
    }
    else if (OP == OP_DOE)
    {

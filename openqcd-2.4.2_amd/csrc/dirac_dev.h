@@ -23,6 +23,18 @@ template <typename C> __device__ inline void store_spinor(C *f, int y, const C *
    for (int c = 0; c < 12; c++) p[c * SB] = s[c];
 }
 
+// streaming store (non-temporal: the output is not read again by the kernel)
+template <typename C> __device__ inline void store_spinor_nt(C *f, int y, const C *s)
+{
+   C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      __builtin_nontemporal_store(s[c].x, &(p[c * SB].x));
+      __builtin_nontemporal_store(s[c].y, &(p[c * SB].y));
+   }
+}
+
 template <typename C> __device__ inline void zero_spinor(C *f, int y)
 {
    C *p = f + (size_t)(y >> 6) * (12 * SB) + (y & (SB - 1));

@@ -131,6 +131,7 @@ void finish(const Res &r, int prec, int nsites, bool download);
 // geometry and tables
 void build_geometry(Geometry &g, const int *L, const int *Ltrd);
 void upload_tables();
+void sap_release();  // frees the SAP block grid (made for the current geometry)
 
 // layout conversions (reference AoS <-> site blocks), n sites from site 0
 void aos_to_blocks_f64(const double2 *aos, double2 *blk, int n, int nc, hipStream_t st);

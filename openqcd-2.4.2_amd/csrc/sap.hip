@@ -750,6 +750,12 @@ bool sap_prepare(int isolv, const char *where, bool quiet)
 
 int sap_grid_nblocks() { return G.ready ? G.nb : 0; }
 
+// the grid belongs to the geometry: released with it
+void sap_release()
+{
+   if (G.ready || G.d_bs) free_grid();
+}
+
 }  // namespace oqh
 
 using namespace oqh;
@@ -846,7 +852,19 @@ static void blk_op(int n, int op, int k, int l, float mu, int nmr, const char *w
       return;
    }
    ensure_u();
-   if (G.swb_set < 0) assign_swb(L.sap_isolv ? ODD_PTS : NO_PTS);
+   // the block SW field follows the SW term of the current gauge field and
+   // parameters: it is (re)made with the last assigned point set (or the one
+   // of the SAP parameters) when it is missing or older than the SW field
+   if (G.swb_set < 0 || !L.swd_valid || G.swb_epoch != L.swd_epoch)
+   {
+      int set = (G.swb_set >= 0) ? G.swb_set : (L.sap_isolv ? ODD_PTS : NO_PTS);
+      if (assign_swb(set))
+      {
+         G.swb_set = -1;
+         fail(where, "The inversion of the SW term was not safe");
+         return;
+      }
+   }
    SArgs a = sargs(mu, nmr);
    a.scr = G.d_bs;
    k_blk_op<<<1, SWG, 0, L.stream>>>(a, n, op, k, l, mu);

@@ -431,11 +431,15 @@ extern "C" int oqh_init(int device)
    return 0;
 }
 
+static void clear_ws_marks();
+
 extern "C" void oqh_finalize(void)
 {
    Lib &L = lib();
    if (!L.inited) return;
    (void)hipStreamSynchronize(L.stream);
+   sap_release();
+   clear_ws_marks();
    for (auto &kv : L.reg) dev_free(kv.second.base);
    L.reg.clear();
    for (int k = 0; k < 4; k++)
@@ -653,11 +657,53 @@ extern "C" void oqh_sync(void)
    if (L.inited) HIP_OK(hipStreamSynchronize(L.stream));
 }
 
+// plain device copy (16-byte accesses, grid-stride) used to measure the
+// achievable HBM bandwidth and to calibrate the byte counters of profiles
+__global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n)
+{
+   size_t stride = (size_t)gridDim.x * 256;
+   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" double oqh_copy_bandwidth(double gbytes, int nrep)
+{
+   Lib &L = lib();
+   if (!L.inited && oqh_init(0)) return 0.0;
+   size_t n = (size_t)(gbytes * 1e9 / 2.0 / sizeof(double2));
+   if (n == 0 || nrep <= 0) return 0.0;
+   double2 *a = (double2 *)dev_alloc(n * sizeof(double2));
+   double2 *b = (double2 *)dev_alloc(n * sizeof(double2));
+   if (!a || !b) return 0.0;
+   HIP_OK(hipMemsetAsync(a, 0, n * sizeof(double2), L.stream));
+   unsigned nb = 256 * 16;
+   k_copy16<<<nb, 256, 0, L.stream>>>(a, b, n);
+   hipEvent_t e0, e1;
+   HIP_OK(hipEventCreate(&e0));
+   HIP_OK(hipEventCreate(&e1));
+   HIP_OK(hipEventRecord(e0, L.stream));
+   for (int k = 0; k < nrep; k++) k_copy16<<<nb, 256, 0, L.stream>>>(a, b, n);
+   HIP_OK(hipEventRecord(e1, L.stream));
+   HIP_OK(hipEventSynchronize(e1));
+   float ms = 0.0f;
+   HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+   HIP_OK(hipEventDestroy(e0));
+   HIP_OK(hipEventDestroy(e1));
+   dev_free(a);
+   dev_free(b);
+   return 2.0 * n * sizeof(double2) * nrep / (ms * 1e-3) / 1e9;
+}
+
 // ---------------------------------------------------------------------------
 // workspaces (utils/wspace.c)
 // ---------------------------------------------------------------------------
 
 static std::vector<int> ws_marks, wsd_marks;
+
+static void clear_ws_marks()
+{
+   ws_marks.clear();
+   wsd_marks.clear();
+}
 
 extern "C" void alloc_ws(int n)
 {

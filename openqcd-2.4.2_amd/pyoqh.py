@@ -19,7 +19,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIBPATH = os.path.join(HERE, "libopenqcd_hip.so")
+LIBPATH = os.environ.get("OQH_LIBRARY") or os.path.join(HERE, "libopenqcd_hip.so")
 HEADER = os.path.join(ROOT, "include", "openqcd_hip.h")
 
 P = C.c_void_p
@@ -102,9 +102,14 @@ def _p(a):
 
 
 class Lattice:
-    """Library state for one geometry (the library holds one global state)."""
+    """Library state for one geometry (the library holds one global state,
+    so at most one Lattice is open at a time)."""
+
+    _open = None
 
     def __init__(self, L, Ltrd=None, bc=3, cF=(1.0, 1.0), m0=0.0, csw=1.0, eoflg=0, device=0):
+        if Lattice._open is not None:
+            raise RuntimeError("another Lattice is open: close() it first (one global library state)")
         self.lib = load()
         self.L = tuple(int(x) for x in L)
         self.Ltrd = tuple(int(x) for x in (Ltrd or L))
@@ -119,9 +124,12 @@ class Lattice:
         self.lib.oqh_set_sw_parms(m0, csw)
         self.lib.oqh_set_tm_parms(eoflg)
         self._fields = []
+        Lattice._open = self
 
     def close(self):
         self.lib.oqh_finalize()
+        if Lattice._open is self:
+            Lattice._open = None
 
     def ipt(self):
         a = np.empty(self.V, np.int32)

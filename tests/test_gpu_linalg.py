@@ -94,7 +94,14 @@ def test_float_products_and_axpy(lat):
     z = lat.lib.spinor_prod(V, 0, sa, sb)
     z0 = np.zeros(2, np.float32)
     po.lib().oc_spinor_prod(V, po.ptr(a), po.ptr(b), po.ptr(z0))
-    assert abs(z.re - z0[0]) <= 1e-6 * abs(z0[0]) and abs(z.im - z0[1]) <= 1e-6 * abs(z0[0])
+    # each colour-vector product is a float sum of 6 float products, so each
+    # side carries a rounding error of at most ~7 ulp of sum_i |a_i b_i|
+    # (contracted or not); the sums over points are in double
+    bound = 1e-6 * float(np.abs(a.astype(np.float64) * b.astype(np.float64)).sum())
+    assert abs(z.re - z0[0]) <= bound and abs(z.im - z0[1]) <= bound
+    # exact double result of the same products, for scale
+    ex = (_cplx(a.astype(np.float64)).conj() * _cplx(b.astype(np.float64))).sum()
+    assert abs(z.re - ex.real) <= bound and abs(z.im - ex.imag) <= bound
     ns = lat.lib.norm_square(V, 0, sa)
     assert abs(ns - po.lib().oc_norm_square(V, po.ptr(a))) <= 1e-6 * ns
     lat.lib.mulc_spinor_add(V, 0, sa, sb, pyoqh.Complex(0.3, -1.1))
