@@ -45,6 +45,20 @@ def copy_bandwidth(lib):
     return bw if bw > 0 else None
 
 
+def measured_traffic():
+    """HBM bytes per Dw_dble launch from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated with the copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "Dw_dble_64x32x32x32_traffic.md")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == "Dw_dble":
+                return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(V):
     """Reference CPU Dw_dble (oracle/_ref/ref_64x32x32x32_avx: the
     reference's modules built with -DAVX -DFMA3, OpenMP, 16 threads)."""
@@ -147,7 +161,7 @@ def main():
                    "pct_of_copy": round(100 * value / copy_bw, 1) if copy_bw else None,
                    "copy_GBps": round(copy_bw, 1) if copy_bw else None, **others},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
