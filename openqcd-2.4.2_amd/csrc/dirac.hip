@@ -227,6 +227,235 @@ __global__ __launch_bounds__(256) DW_ATTR void k_dirac(DArgs<T> a)
    }
 }
 
+// ---------------------------------------------------------------------------
+// Dw / Dw_dble with link reuse in registers and spinor reuse in LDS.
+//
+// A work-group of TP lanes takes TP even points e0..e0+TP-1 and the TP odd
+// points with the same offsets; lane t owns the pair (e0+t, V/2+e0+t).  In
+// the site order these tiles are compact (a 4^4 sub-block for 8^4 thread
+// blocks), so 3/4 of all neighbours lie in the tile.  Every link is stored
+// with an odd point and connects it to one even point: the owning lane loads
+// it once and computes both hopping terms that use it, its own and the even
+// neighbour's.  The even neighbour's term (two colour vectors after the spin
+// projection) is passed through LDS; in each of the eight direction phases
+// every even point receives at most one such term, so no two lanes write the
+// same slot.  Even spinors are staged in LDS for the odd lanes.  Neighbours
+// outside the tile are gathered from global memory as in k_dirac.  Compared
+// with the plain gather this halves the link reads and removes 3/4 of the
+// neighbour-spinor reads from the memory system.
+// ---------------------------------------------------------------------------
+
+#ifndef OQH_DW_TPD
+#define OQH_DW_TPD 256
+#endif
+#ifndef OQH_DW_TPF
+#define OQH_DW_TPF 256
+#endif
+#ifndef OQH_NT_LINK
+#define OQH_NT_LINK 0
+#endif
+#ifndef OQH_NT_SW
+#define OQH_NT_SW 1
+#endif
+template <typename T> struct TileSize { static constexpr int n = OQH_DW_TPD; };
+template <> struct TileSize<float> { static constexpr int n = OQH_DW_TPF; };
+#ifndef OQH_DW_TILE
+#define OQH_DW_TILE 1
+#endif
+
+#ifndef OQH_NBR_PRELOAD
+#define OQH_NBR_PRELOAD 1
+#endif
+#ifndef OQH_X_DOUBLE
+#define OQH_X_DOUBLE 0
+#endif
+constexpr int XBUF = OQH_X_DOUBLE ? 2 : 1;
+
+// neighbour-table entry k (0..7 for the odd point, 8..15 for the even point)
+template <typename T> struct TileNbr
+{
+   int v[OQH_NBR_PRELOAD ? 16 : 1];
+};
+
+template <typename T, int K>
+__device__ inline int tile_nbr(const DArgs<T> &a, const TileNbr<T> &nb, int e, int o)
+{
+#if OQH_NBR_PRELOAD
+   (void)a; (void)e; (void)o;
+   return nb.v[K];
+#else
+   (void)nb;
+   return K < 8 ? a.nbr[(size_t)K * a.V + o] : a.nbr[(size_t)(K - 8) * a.V + e];
+#endif
+}
+
+template <typename T, int MU, bool FWD, int TP>
+__device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, const TileNbr<T> &nb,
+                                  typename cplx_of<T>::t (*S)[TP], typename cplx_of<T>::t (*X)[TP],
+                                  const typename cplx_of<T>::t *so, typename cplx_of<T>::t *acc_o,
+                                  typename cplx_of<T>::t *acc_e)
+{
+   typedef typename cplx_of<T>::t C;
+   const int e = e0 + t, o = a.h + e;
+   // odd point o: link of direction (MU, FWD), neighbour y = o +- mu (even)
+   if (valid)
+   {
+      int ny = tile_nbr<T, 2 * MU + (FWD ? 0 : 1)>(a, nb, e, o);
+      int y = ny & 0x7fffffff;
+      int ly = y - e0;
+      bool yin = (unsigned)ly < (unsigned)TP;
+      const C *up = link_ptr(a.u, e, FWD ? 2 * MU : 2 * MU + 1);
+      C u[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[k] = ld_stream<OQH_NT_LINK>(up + k * SB);
+      C h0[3], h1[3], c0[3], c1[3];
+      if (!(a.masked && ny < 0))
+      {
+         C sy[12];
+         if (yin)
+         {
+#pragma unroll
+            for (int c = 0; c < 12; c++) sy[c] = S[c][ly];
+         }
+         else
+            load_spinor(a.s, y, sy);
+         spin_project<C, MU, FWD>(sy, h0, h1);
+         su3_mul<C, !FWD>(u, h0, c0);
+         su3_mul<C, !FWD>(u, h1, c1);
+         spin_expand_add<C, MU, FWD>(c0, c1, acc_o);
+      }
+      if (yin)
+      {
+         // the term of y from o, as y's neighbour in the opposite direction
+         spin_project<C, MU, !FWD>(so, h0, h1);
+         su3_mul<C, FWD>(u, h0, c0);
+         su3_mul<C, FWD>(u, h1, c1);
+#pragma unroll
+         for (int c = 0; c < 3; c++)
+         {
+            X[c][ly] = c0[c];
+            X[3 + c][ly] = c1[c];
+         }
+      }
+   }
+   __syncthreads();
+   // even point e: neighbour z = e -+ mu (odd), the opposite direction
+   if (valid)
+   {
+      int nz = tile_nbr<T, 8 + 2 * MU + (FWD ? 1 : 0)>(a, nb, e, o);
+      if (!(a.masked && nz < 0))
+      {
+         int z = nz & 0x7fffffff;
+         if ((unsigned)(z - a.h - e0) < (unsigned)TP)
+         {
+            C c0[3], c1[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               c0[c] = X[c][t];
+               c1[c] = X[3 + c][t];
+            }
+            spin_expand_add<C, MU, !FWD>(c0, c1, acc_e);
+         }
+         else
+         {
+            C sz[12];
+            load_spinor(a.s, z, sz);
+            hop_one<C, MU, !FWD>(sz, link_ptr(a.u, z - a.h, FWD ? 2 * MU : 2 * MU + 1), acc_e);
+         }
+      }
+   }
+   // with a single exchange buffer the next phase may overwrite it only
+   // after every lane has read it; with two buffers the next phase's barrier
+   // gives that order
+   if (XBUF == 1) __syncthreads();
+}
+
+template <typename T, int TP = TileSize<T>::n>
+__global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
+{
+   typedef typename cplx_of<T>::t C;
+   __shared__ C S[12][TP];
+   __shared__ C X[XBUF][6][TP];
+   const int t = threadIdx.x;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e = e0 + t, o = a.h + e;
+   const bool valid = e < a.h;
+   bool eb = false, ob = false;
+   TileNbr<T> nb;
+   C so[12], acc_o[12], acc_e[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      acc_o[c] = cmk<C>(0, 0);
+      acc_e[c] = cmk<C>(0, 0);
+      so[c] = cmk<C>(0, 0);
+   }
+   if (valid)
+   {
+#if OQH_NBR_PRELOAD
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+      {
+         nb.v[k] = a.nbr[(size_t)k * a.V + o];
+         nb.v[8 + k] = a.nbr[(size_t)k * a.V + e];
+      }
+#endif
+      eb = a.masked && a.bnd[e];
+      ob = a.masked && a.bnd[o];
+      C se[12];
+      if (eb)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++) se[c] = cmk<C>(0, 0);
+      }
+      else
+         load_spinor(a.s, e, se);
+#pragma unroll
+      for (int c = 0; c < 12; c++) S[c][t] = se[c];
+      if (!ob) load_spinor(a.s, o, so);
+   }
+   __syncthreads();
+   tile_phase<T, 0, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
+   tile_phase<T, 0, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
+   tile_phase<T, 1, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
+   tile_phase<T, 1, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
+   tile_phase<T, 2, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
+   tile_phase<T, 2, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
+   tile_phase<T, 3, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
+   tile_phase<T, 3, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
+   if (!valid) return;
+   C out[12];
+   // boundary points of open bc are zeroed in output and input
+   if (ob)
+   {
+      zero_spinor(a.r, o);
+      zero_spinor(a.s_w, o);
+   }
+   else
+   {
+      pauli_site<C, OQH_NT_SW>(a.sw, o, a.muo, so, out);
+#pragma unroll
+      for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_o[c], a.coef));
+      store_spinor(a.r, o, out);
+   }
+   if (eb)
+   {
+      zero_spinor(a.r, e);
+      zero_spinor(a.s_w, e);
+   }
+   else
+   {
+      C se[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) se[c] = S[c][t];
+      pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, out);
+#pragma unroll
+      for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_e[c], a.coef));
+      store_spinor(a.r, e, out);
+   }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_bnd(typename cplx_of<T>::t *s, const unsigned char *bnd,
                                                   int p0, int np)
@@ -258,6 +487,15 @@ template <typename T> static DArgs<T> base_args()
 template <typename T, int OP> static void launch(const DArgs<T> &a)
 {
    Lib &L = lib();
+#if OQH_DW_TILE
+   if (OP == OP_DW)
+   {
+      constexpr int TP = TileSize<T>::n;
+      unsigned nt = (unsigned)((a.h + TP - 1) / TP);
+      if (nt) k_dw_tile<T><<<nt, TP, 0, L.stream>>>(a);
+      return;
+   }
+#endif
    unsigned nb = (OP == OP_DW) ? (unsigned)((a.h + 127) / 128) : (unsigned)((a.np + 255) / 256);
    if (nb == 0) return;
    k_dirac<T, OP><<<nb, 256, 0, L.stream>>>(a);

@@ -23,6 +23,16 @@ template <typename C> __device__ inline void store_spinor(C *f, int y, const C *
    for (int c = 0; c < 12; c++) p[c * SB] = s[c];
 }
 
+// load of data used once (NT: non-temporal, not kept in the caches)
+template <bool NT, typename C> __device__ inline C ld_stream(const C *p)
+{
+   if (!NT) return *p;
+   C v;
+   v.x = __builtin_nontemporal_load(&p->x);
+   v.y = __builtin_nontemporal_load(&p->y);
+   return v;
+}
+
 // streaming store (non-temporal: the output is not read again by the kernel)
 template <typename C> __device__ inline void store_spinor_nt(C *f, int y, const C *s)
 {
@@ -59,11 +69,23 @@ template <typename C, bool DAG> __device__ inline void su3_apply(const C *up, co
    }
 }
 
-// one direction mu, forward (FWD) or backward: project s, multiply, expand
-template <typename C, int MU, bool FWD>
-__device__ inline void hop_one(const C *sp, const C *up, C *acc)
+// c = U h  or  c = U^dag h with U held in registers
+template <typename C, bool DAG> __device__ inline void su3_mul(const C *u, const C *h, C *c)
 {
-   C h0[3], h1[3], c0[3], c1[3];
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      if (!DAG)
+         c[i] = cadd(cadd(cmul(u[3 * i], h[0]), cmul(u[3 * i + 1], h[1])), cmul(u[3 * i + 2], h[2]));
+      else
+         c[i] = cadd(cadd(cmulc(u[i], h[0]), cmulc(u[3 + i], h[1])), cmulc(u[6 + i], h[2]));
+   }
+}
+
+// spin projection (1 -+ gamma_mu) s of the hopping term towards x+mu (FWD)
+// or x-mu: the two independent colour vectors h0, h1
+template <typename C, int MU, bool FWD> __device__ inline void spin_project(const C *sp, C *h0, C *h1)
+{
    // spin components s_a are sp[3a..3a+2]
 #pragma unroll
    for (int i = 0; i < 3; i++)
@@ -74,8 +96,12 @@ __device__ inline void hop_one(const C *sp, const C *up, C *acc)
       if (MU == 2) { h0[i] = FWD ? cadd(s0, s3) : csub(s0, s3); h1[i] = FWD ? csub(s1, s2) : cadd(s1, s2); }
       if (MU == 3) { h0[i] = FWD ? caddi(s0, s2) : csubi(s0, s2); h1[i] = FWD ? csubi(s1, s3) : caddi(s1, s3); }
    }
-   su3_apply<C, !FWD>(up, h0, c0);
-   su3_apply<C, !FWD>(up, h1, c1);
+}
+
+// acc += the full spinor rebuilt from the colour products c0, c1 of the
+// projected components (inverse of spin_project)
+template <typename C, int MU, bool FWD> __device__ inline void spin_expand_add(const C *c0, const C *c1, C *acc)
+{
 #pragma unroll
    for (int i = 0; i < 3; i++)
    {
@@ -104,6 +130,18 @@ __device__ inline void hop_one(const C *sp, const C *up, C *acc)
    }
 }
 
+// one direction mu, forward (FWD: neighbour x+mu, link U(x,mu)) or backward
+// (neighbour x-mu, link U(x-mu,mu)^dag): project s, multiply, expand
+template <typename C, int MU, bool FWD>
+__device__ inline void hop_one(const C *sp, const C *up, C *acc)
+{
+   C h0[3], h1[3], c0[3], c1[3];
+   spin_project<C, MU, FWD>(sp, h0, h1);
+   su3_apply<C, !FWD>(up, h0, c0);
+   su3_apply<C, !FWD>(up, h1, c1);
+   spin_expand_add<C, MU, FWD>(c0, c1, acc);
+}
+
 // address of link l (0..7, udfld order) stored with odd point j = y - V/2
 template <typename C> __device__ inline const C *link_ptr(const C *u, int j, int l)
 {
@@ -113,10 +151,10 @@ template <typename C> __device__ inline const C *link_ptr(const C *u, int j, int
 // out = (A + i mu) v on one Weyl half (6 complex); m points at unit 0 of the
 // half (stride SB); units 0-2 hold the real diagonal, 3-17 the upper
 // off-diagonal entries (0,1),(0,2),...,(4,5)
-template <typename C>
+template <typename C, bool NT = false>
 __device__ inline void pauli_half(const C *m, decltype(C::x) mu, const C *v, C *out)
 {
-   C d0 = m[0], d1 = m[SB], d2 = m[2 * SB];
+   C d0 = ld_stream<NT>(m), d1 = ld_stream<NT>(m + SB), d2 = ld_stream<NT>(m + 2 * SB);
    decltype(C::x) d[6] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y};
 #pragma unroll
    for (int i = 0; i < 6; i++) out[i] = cmk<C>(d[i] * v[i].x - mu * v[i].y, d[i] * v[i].y + mu * v[i].x);
@@ -126,19 +164,19 @@ __device__ inline void pauli_half(const C *m, decltype(C::x) mu, const C *v, C *
 #pragma unroll
       for (int j = i + 1; j < 6; j++)
       {
-         C hij = m[p * SB];
+         C hij = ld_stream<NT>(m + p * SB);
          out[i] = cadd(out[i], cmul(hij, v[j]));
          out[j] = cadd(out[j], cmulc(hij, v[i]));
          p++;
       }
 }
 
-template <typename C>
+template <typename C, bool NT = false>
 __device__ inline void pauli_site(const C *sw, int x, decltype(C::x) mu, const C *v, C *out)
 {
    const C *m = sw + (size_t)(x >> 6) * (36 * SB) + (x & (SB - 1));
-   pauli_half(m, mu, v, out);
-   pauli_half(m + 18 * SB, -mu, v + 6, out + 6);
+   pauli_half<C, NT>(m, mu, v, out);
+   pauli_half<C, NT>(m + 18 * SB, -mu, v + 6, out + 6);
 }
 
 }  // namespace oqh

@@ -657,12 +657,19 @@ extern "C" void oqh_sync(void)
    if (L.inited) HIP_OK(hipStreamSynchronize(L.stream));
 }
 
-// plain device copy (16-byte accesses, grid-stride) used to measure the
-// achievable HBM bandwidth and to calibrate the byte counters of profiles
+// plain device copy (16-byte accesses, four per lane with the loads issued
+// before the stores) used to measure the achievable HBM bandwidth and to
+// calibrate the byte counters of profiles
 __global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n)
 {
-   size_t stride = (size_t)gridDim.x * 256;
-   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) b[i] = a[i];
+   size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+   double2 v[4];
+#pragma unroll
+   for (int k = 0; k < 4; k++)
+      if (base + 256 * k < n) v[k] = a[base + 256 * k];
+#pragma unroll
+   for (int k = 0; k < 4; k++)
+      if (base + 256 * k < n) b[base + 256 * k] = v[k];
 }
 
 extern "C" double oqh_copy_bandwidth(double gbytes, int nrep)
@@ -675,7 +682,7 @@ extern "C" double oqh_copy_bandwidth(double gbytes, int nrep)
    double2 *b = (double2 *)dev_alloc(n * sizeof(double2));
    if (!a || !b) return 0.0;
    HIP_OK(hipMemsetAsync(a, 0, n * sizeof(double2), L.stream));
-   unsigned nb = 256 * 16;
+   unsigned nb = (unsigned)((n + 1023) / 1024);
    k_copy16<<<nb, 256, 0, L.stream>>>(a, b, n);
    hipEvent_t e0, e1;
    HIP_OK(hipEventCreate(&e0));
