@@ -82,7 +82,8 @@ void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
 
 /* gauge field in the layout of udfld() (uflds/uflds.c:113-156; 8 links per
    odd point, README.global).  Import invalidates the SW term and the
-   single-precision copy. */
+   single-precision copy; the imported field is taken as given, with the
+   library's phase flag of set_ud_phase() cleared. */
 void oqh_import_ud(const su3_dble *ud);
 void oqh_export_ud(su3_dble *ud);
 /* SW field in the layout of swdfld() (2 pauli_dble per point) */
@@ -106,6 +107,11 @@ double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall);
 /* bandwidth (GB/s, bytes read + written) of a plain device copy of gbytes
    in total, averaged over nrep launches */
 double oqh_copy_bandwidth(double gbytes, int nrep);
+
+/* force field of the last plaq_frc() (4V su3_alg_dble = 8 doubles per
+   link, layout of udfld(); the MD force field mdflds()->frc of the
+   reference, mdflds/mdflds.c:269) */
+void oqh_export_frc(double *frc);
 
 /* SAP block grid access for tests: block-local field k (0..2) of block n in
    the block's own point order (vol spinors), and the embedding of the block
@@ -134,6 +140,17 @@ int sw_term(ptset_t set);
 void assign_swd2sw(void);
 /* include/uflds.h:59 (uflds/uflds.c:479) */
 void assign_ud2u(void);
+
+/* include/uflds.h:56-57 (uflds/uflds.c:338-400): sign of the time-like
+   links at x0 = N0-1 for periodic bc (theta = 0) */
+void set_ud_phase(void);
+void unset_ud_phase(void);
+/* include/forces.h:27 (forces/force0.c:281): force of the Wilson plaquette
+   action, into the library's force field (oqh_export_frc) */
+void plaq_frc(void);
+/* include/wflow.h:16 (wflow/wflow.c:217): n forward-Euler steps of the
+   Wilson flow with step eps on the double-precision gauge field */
+void fwd_euler(int n, double eps);
 
 /* include/linalg.h:70-79 (linalg/salg.c:1223-1429) */
 complex spinor_prod(int vol, int icom, spinor *s, spinor *r);

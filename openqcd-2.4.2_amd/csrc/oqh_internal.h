@@ -79,6 +79,7 @@ struct Lib
    double2 *d_swd = nullptr;  // 36 units per site
    float2 *d_sw = nullptr;
    bool ud_set = false, u_valid = false, swd_valid = false, sw_valid = false;
+   bool ud_phase = false;  // quark phase of set_ud_phase() applied to d_ud
    bool swd_inv[2] = {false, false};
    int swd_epoch = 0;  // incremented whenever the contents of d_swd change
 
@@ -132,6 +133,7 @@ void finish(const Res &r, int prec, int nsites, bool download);
 void build_geometry(Geometry &g, const int *L, const int *Ltrd);
 void upload_tables();
 void sap_release();  // frees the SAP block grid (made for the current geometry)
+void release_flow(); // frees the force field of the Wilson flow
 
 // layout conversions (reference AoS <-> site blocks), n sites from site 0
 void aos_to_blocks_f64(const double2 *aos, double2 *blk, int n, int nc, hipStream_t st);

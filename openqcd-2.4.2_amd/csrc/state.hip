@@ -439,6 +439,7 @@ extern "C" void oqh_finalize(void)
    if (!L.inited) return;
    (void)hipStreamSynchronize(L.stream);
    sap_release();
+   release_flow();
    clear_ws_marks();
    for (auto &kv : L.reg) dev_free(kv.second.base);
    L.reg.clear();
@@ -578,6 +579,7 @@ extern "C" void oqh_import_ud(const su3_dble *ud)
    aos_to_blocks_f64((const double2 *)aos, L.d_ud, h, 72, L.stream);
    HIP_OK(hipStreamSynchronize(L.stream));
    L.ud_set = true;
+   L.ud_phase = false;
    L.u_valid = false;
    L.swd_valid = L.sw_valid = false;
 }

@@ -55,6 +55,9 @@ _SIG = {
     "oqh_import_sd": (None, [P, P, _I]), "oqh_export_sd": (None, [P, P, _I]),
     "oqh_sync": (None, []), "oqh_time_dirac": (_D, [_I, _D, P, P, _I]),
     "sw_term": (_I, [_I]), "assign_swd2sw": (None, []), "assign_ud2u": (None, []),
+    "oqh_copy_bandwidth": (_D, [_D, _I]), "oqh_export_frc": (None, [P]),
+    "plaq_frc": (None, []), "fwd_euler": (None, [_I, _D]),
+    "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
 }
 for _n in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
     _SIG[_n] = (None, [_D, P, P])
@@ -144,6 +147,12 @@ class Lattice:
     def export_ud(self):
         a = np.empty((4 * self.V, 18), np.float64)
         self.lib.oqh_export_ud(_p(a))
+        return a
+
+    def export_frc(self):
+        """Force field of the last plaq_frc(), (4V, 8) su3_alg_dble."""
+        a = np.empty((4 * self.V, 8), np.float64)
+        self.lib.oqh_export_frc(_p(a))
         return a
 
     def export_swd(self):

@@ -62,6 +62,11 @@ void oc_set_ud(oc_ctx *c, const double *ud);
 void oc_assign_ud2u(oc_ctx *c);         /* FP64 -> FP32 copy (ufld)     */
 float *oc_ufld(oc_ctx *c);
 
+/* ---- Wilson flow (wflow/wflow.c, forces/force0.c, su3fcts/chexp.c) ---- */
+void oc_plaq_frc(oc_ctx *c, double *frc);            /* 4V su3_alg_dble (8 doubles) */
+void oc_expXsu3(double eps, const double *X, double *u);
+void oc_fwd_euler(oc_ctx *c, int n, double eps);     /* acts on the FP64 field  */
+
 /* ---- spinor fields (sflds/sflds.c, sflds/unorm.c, lattice/bcnds.c) ---- */
 void oc_random_sd(oc_ctx *c, double *sd, double sigma);
 void oc_random_s(oc_ctx *c, float *s, float sigma);

@@ -58,6 +58,8 @@ def lib():
             "oc_blk_mres": (None, [P, i, f, i, P, P]),
             "oc_sap": (None, [P, f, i, i, P, P]),
             "oc_sap_gcr": (d, [P, i, i, i, d, d, P, P, P, P]),
+            "oc_plaq_frc": (None, [P, P]), "oc_expXsu3": (None, [d, P, P]),
+            "oc_fwd_euler": (None, [P, i, d]),
         }
         for op in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
             sig["oc_" + op] = (None, [P, d, P, P])
@@ -139,6 +141,15 @@ class Oracle:
 
     def assign_ud2u(self):
         lib().oc_assign_ud2u(self.h)
+
+    def plaq_frc(self):
+        """Plaquette force of the current FP64 field, (4V, 8) su3_alg_dble."""
+        f = np.empty((4 * self.V, 8), np.float64)
+        lib().oc_plaq_frc(self.h, ptr(f))
+        return f
+
+    def fwd_euler(self, n, eps):
+        lib().oc_fwd_euler(self.h, n, eps)
 
     def random_sd(self, sigma=1.0):
         a = np.empty((self.V, 24), np.float64)
@@ -305,6 +316,14 @@ def ref_exe(tag):
     """Path of a reference driver executable, or None when not built."""
     p = os.path.join(HERE, "_ref", "ref_" + tag)
     return p if os.path.exists(p) else None
+
+
+def expXsu3(eps, X, u):
+    """exp(eps X) u for one su3_alg_dble X (8) and su3_dble u (18)."""
+    X = np.ascontiguousarray(X, np.float64)
+    u = np.array(u, np.float64)
+    lib().oc_expXsu3(eps, ptr(X), ptr(u))
+    return u
 
 
 def ref_dump(tag, outdir, **kw):

@@ -11,9 +11,12 @@
  * Options: -bc T -cF a b -csw c -m0 m -seed s -mu mu -eoflg e
  *          -bs b0 b1 b2 b3 -isolv i -nmr n -ncy n -nkv n -nmx n -res r
  *          -secs t (bench: target seconds per operator)
+ *          -flow n eps (dump: plaquette force of the random field to frc0,
+ *          then n forward-Euler Wilson-flow steps, flowed field to ud_flow;
+ *          all later outputs use the flowed field)
  *
  * The random-number sequence of "dump" is fixed:
- *   start_ranlux(0,seed); geometry(); random_ud(); set_ud_phase();
+ *   start_ranlux(0,seed); geometry(); random_ud(); [flow]; set_ud_phase();
  *   random_sd(VOLUME_TRD,2,eta,1.0) [bnd_sd2zero(ALL_PTS) for bc!=3];
  *   random_s (VOLUME_TRD,2,s32,1.0) [bnd_s2zero].
  */
@@ -35,6 +38,9 @@
 #include "dirac.h"
 #include "block.h"
 #include "sap.h"
+#include "mdflds.h"
+#include "forces.h"
+#include "wflow.h"
 #include "global.h"
 
 static char outdir[512];
@@ -54,7 +60,8 @@ static double wtime(void) { return MPI_Wtime(); }
 int main(int argc, char *argv[])
 {
    int bc = 3, seed = 1, eoflg = 0, isolv = 0, nmr = 4, ncy = 5, nkv = 16, nmx = 256;
-   int bs[4] = {4, 4, 4, 4}, dump = 1, i;
+   int bs[4] = {4, 4, 4, 4}, dump = 1, i, nflow = 0;
+   double eflow = 0.0;
    double cF[2] = {1.0, 1.0}, csw = 1.0, m0 = -0.5, mu = 0.0, res = 1.0e-10, secs = 2.0;
    double phi[2] = {0.0, 0.0}, phip[2] = {0.0, 0.0}, theta[3] = {0.0, 0.0, 0.0};
 
@@ -85,6 +92,7 @@ int main(int argc, char *argv[])
       else if (!strcmp(argv[i], "-nmx")) nmx = atoi(argv[++i]);
       else if (!strcmp(argv[i], "-res")) res = atof(argv[++i]);
       else if (!strcmp(argv[i], "-secs")) secs = atof(argv[++i]);
+      else if (!strcmp(argv[i], "-flow")) { nflow = atoi(argv[++i]); eflow = atof(argv[++i]); }
    }
 
    mpi_init(argc, argv);
@@ -103,6 +111,13 @@ int main(int argc, char *argv[])
    spinor **ws = reserve_ws(8);
 
    random_ud();
+   if (dump && nflow > 0)
+   {
+      plaq_frc();
+      wr("frc0.f64", (*mdflds()).frc, 4 * VOLUME * sizeof(su3_alg_dble));
+      fwd_euler(nflow, eflow);
+      wr("ud_flow.f64", udfld(), 4 * VOLUME * sizeof(su3_dble));
+   }
    set_ud_phase();
 
    if (!dump)

@@ -65,7 +65,30 @@ def ref_samples():
         np.savez_compressed(os.path.join(HERE, "ref_8x8x8x8_%s.npz" % name), **arrs)
 
 
+def flow_samples():
+    """Plaquette force of random_ud (seed 2) and the field after 4 Euler
+    steps of the Wilson flow (eps 0.02), sampled at 96 links, for periodic and
+    open bc on 8^4."""
+    import pyoracle as po
+    rng = np.random.default_rng(1)
+    for bc, name in ((3, "periodic"), (0, "open")):
+        d = tempfile.mkdtemp()
+        ld = po.ref_dump("8x8x8x8_t8", d, bc=bc, seed=2, flow=(4, 0.02))
+        if ld is None:
+            print("reference build not present")
+            return
+        nl = 4 * 4096
+        links = np.sort(rng.choice(nl, 96, replace=False)).astype(np.int32)
+        frc = ld("frc0.f64", np.float64, (nl, 8))
+        udf = ld("ud_flow.f64", np.float64, (nl, 18))
+        np.savez_compressed(os.path.join(HERE, "ref_8x8x8x8_flow_%s.npz" % name), links=links,
+                            info=json.dumps({"bc": bc, "seed": 2, "nflow": 4, "eps": 0.02}),
+                            frc0=frc[links], frc0_norm=np.linalg.norm(frc),
+                            ud_flow=udf[links], ud_flow_norm=np.linalg.norm(udf))
+
+
 if __name__ == "__main__":
     kat()
     ref_samples()
+    flow_samples()
     print("fixtures written")

@@ -240,3 +240,76 @@ def test_reference_samples(name):
     rho, ifail, status, psi = o.sap_gcr(16, 256, 0, 1e-10, mu, eta.copy()) if o.set_sap((4, 4, 4, 4), info["isolv"], 4, 5) is None else (None,) * 4
     assert ifail == info["gcr_ifail"] and abs(status - info["gcr_status"]) <= 1
     assert site_rel(psi[pts], z["gcr_psi_f64"])[0] < 1e-8
+
+
+def test_expXsu3_is_the_matrix_exponential():
+    """expXsu3 (su3fcts/chexp.c:878) against the matrix exponential of scipy,
+    including norms large enough for the halving-and-squaring branch."""
+    from scipy.linalg import expm
+    from helpers import alg_matrix
+    rng = np.random.default_rng(11)
+    for scale in (0.01, 0.3, 1.0, 5.0, 40.0):
+        X = rng.standard_normal(8) * scale
+        u = _random_gauge(1, rng)[0]
+        ud = np.empty(18)
+        ud[0::2] = u.ravel().real
+        ud[1::2] = u.ravel().imag
+        for eps in (0.02, -0.7):
+            out = po.expXsu3(eps, X, ud)
+            ref = expm(eps * alg_matrix(X)) @ u
+            got = (out[0::2] + 1j * out[1::2]).reshape(3, 3)
+            assert np.abs(got - ref).max() < 2e-14 * max(1.0, abs(eps) * scale), (scale, eps)
+
+
+@pytest.mark.parametrize("name", ["periodic", "open"])
+def test_flow_against_reference_samples(name):
+    """The oracle's force and 4 flow steps from its own random field agree with
+    the reference's at the sampled links (the two random fields differ in the
+    last bit, see test_fields)."""
+    g = np.load(os.path.join(GOLD, "ref_8x8x8x8_flow_%s.npz" % name))
+    info = json.loads(str(g["info"]))
+    o = po.Oracle((8, 8, 8, 8), None, bc=info["bc"])
+    o.start_ranlux(0, info["seed"])
+    o.random_ud()
+    f = o.plaq_frc()
+    assert np.abs(f[g["links"]] - g["frc0"]).max() < 1e-13 * float(g["frc0_norm"]) / math.sqrt(f.size)
+    o.fwd_euler(info["nflow"], info["eps"])
+    u = o.ud()
+    assert np.abs(u[g["links"]] - g["ud_flow"]).max() < 1e-13
+
+
+def test_flow_invariants():
+    """The flow keeps the links in SU(3), increases the plaquette sum (it
+    descends along the Wilson action) and its force is gauge covariant."""
+    from helpers import alg_matrix, gauge_transform, link_base_points, plaquette_sum
+    for bc in (3, 0):
+        o = po.Oracle((8, 8, 8, 8), (4, 4, 4, 4), bc=bc)
+        o.start_ranlux(0, 5)
+        o.random_ud()
+        iup, idn = o.iupdn()
+        V, h = o.V, o.V // 2
+        ud0 = o.ud()
+        # gauge covariance of the force: F[U^g](x,mu) = g(x) F[U](x,mu) g(x)^+
+        F0 = alg_matrix(o.plaq_frc()).reshape(h, 8, 3, 3)
+        g = _random_gauge(V, np.random.default_rng(2))
+        o.set_ud(gauge_transform(ud0, g, iup, idn))
+        F1 = alg_matrix(o.plaq_frc()).reshape(h, 8, 3, 3)
+        base = link_base_points(iup, idn, h)
+        gb = g[base]
+        rot = gb @ F0 @ np.conj(np.transpose(gb, (0, 1, 3, 2)))
+        assert np.abs(F1 - rot).max() < 1e-13
+        # monotone smoothing, unitarity
+        o.set_ud(ud0)
+        s = [plaquette_sum(o.ud(), iup)]
+        for _ in range(3):
+            o.fwd_euler(2, 0.02)
+            s.append(plaquette_sum(o.ud(), iup))
+        assert all(b > a for a, b in zip(s, s[1:])), s
+        m = (o.ud()[:, 0::2] + 1j * o.ud()[:, 1::2]).reshape(-1, 3, 3)
+        # open bc: the time-like links at x0 = N0-1 are zero and stay so
+        live = np.abs(m).max(axis=(1, 2)) > 0
+        assert live.sum() == (4 * V if bc == 3 else 4 * V - V // 8)
+        m = m[live]
+        eye = np.eye(3)
+        assert np.abs(m @ np.conj(np.transpose(m, (0, 2, 1))) - eye).max() < 1e-13
+        assert np.abs(np.linalg.det(m) - 1).max() < 1e-13

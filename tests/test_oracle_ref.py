@@ -128,3 +128,38 @@ def test_sap_and_solver(case):
     assert abs(status - info["gcr_status"]) <= 1
     rel, _ = site_rel(psi, ld("gcr_psi.f64", np.float64, (V, 24)))
     assert rel < 1e-8
+
+
+def _unphase(o, ud, L):
+    """Undo set_ud_phase of periodic bc exactly: the time-like links at
+    x0 = N0-1 (slot 0 of odd points there, slot 1 of odd points at x0 = 0)."""
+    V = o.V
+    h = V // 2
+    lex = np.empty(V, np.int64)
+    lex[o.ipt()] = np.arange(V)
+    t = lex[h:] // (L[1] * L[2] * L[3])
+    ud = ud.reshape(h, 8, 18).copy()
+    ud[t == L[0] - 1, 0] *= -1.0
+    ud[t == 0, 1] *= -1.0
+    return ud.reshape(4 * V, 18)
+
+
+@pytest.mark.parametrize("tag,bc", [("8x8x8x8_t8", 3), ("8x8x8x8_t8", 0), ("8x8x8x8_t4", 3),
+                                    ("16x8x8x8_t8", 0)])
+def test_flow_matches_reference(tag, bc, tmp_path):
+    """plaq_frc and fwd_euler of the oracle on the reference's own random
+    field agree with the reference build to the last bit."""
+    if po.ref_exe(tag) is None:
+        pytest.skip("reference build not present")
+    L, T = GEOM[tag]
+    l0 = po.ref_dump(tag, str(tmp_path / "a"), bc=bc, seed=2)
+    l1 = po.ref_dump(tag, str(tmp_path / "b"), bc=bc, seed=2, flow=(4, 0.02))
+    o = po.Oracle(L, T, bc=bc)
+    V = o.V
+    ud = l0("ud.f64", np.float64, (4 * V, 18))
+    if bc == 3:
+        ud = _unphase(o, ud, L)
+    o.set_ud(ud)
+    assert np.array_equal(o.plaq_frc(), l1("frc0.f64", np.float64, (4 * V, 8)))
+    o.fwd_euler(4, 0.02)
+    assert np.array_equal(o.ud(), l1("ud_flow.f64", np.float64, (4 * V, 18)))
