@@ -262,6 +262,9 @@ template <> struct TileSize<float> { static constexpr int n = OQH_DW_TPF; };
 #ifndef OQH_DW_TILE
 #define OQH_DW_TILE 1
 #endif
+#ifndef OQH_HAT_TILE
+#define OQH_HAT_TILE 1
+#endif
 
 #ifndef OQH_NBR_PRELOAD
 #define OQH_NBR_PRELOAD 1
@@ -292,11 +295,15 @@ __device__ inline int tile_nbr(const DArgs<T> &a, const TileNbr<T> &nb, int e, i
 #endif
 }
 
-template <typename T, int MU, bool FWD, int TP>
+// OWN: the odd lane adds its own hopping term (from the even neighbour, S or
+// global a.s) to acc_o; XCH: the odd lane passes the term of its even
+// neighbour through X, and even lanes gather the terms of neighbours outside
+// the tile from the odd-point field zsrc
+template <typename T, int MU, bool FWD, int TP, bool OWN = true, bool XCH = true>
 __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, const TileNbr<T> &nb,
                                   typename cplx_of<T>::t (*S)[TP], typename cplx_of<T>::t (*X)[TP],
                                   const typename cplx_of<T>::t *so, typename cplx_of<T>::t *acc_o,
-                                  typename cplx_of<T>::t *acc_e)
+                                  typename cplx_of<T>::t *acc_e, const typename cplx_of<T>::t *zsrc)
 {
    typedef typename cplx_of<T>::t C;
    const int e = e0 + t, o = a.h + e;
@@ -312,7 +319,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
 #pragma unroll
       for (int k = 0; k < 9; k++) u[k] = ld_stream<OQH_NT_LINK>(up + k * SB);
       C h0[3], h1[3], c0[3], c1[3];
-      if (!(a.masked && ny < 0))
+      if (OWN && !(a.masked && ny < 0))
       {
          C sy[12];
          if (yin)
@@ -327,7 +334,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
          su3_mul<C, !FWD>(u, h1, c1);
          spin_expand_add<C, MU, FWD>(c0, c1, acc_o);
       }
-      if (yin)
+      if (XCH && yin)
       {
          // the term of y from o, as y's neighbour in the opposite direction
          spin_project<C, MU, !FWD>(so, h0, h1);
@@ -341,6 +348,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
          }
       }
    }
+   if (!XCH) return;
    __syncthreads();
    // even point e: neighbour z = e -+ mu (odd), the opposite direction
    if (valid)
@@ -363,7 +371,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
          else
          {
             C sz[12];
-            load_spinor(a.s, z, sz);
+            load_spinor(zsrc, z, sz);
             hop_one<C, MU, !FWD>(sz, link_ptr(a.u, z - a.h, FWD ? 2 * MU : 2 * MU + 1), acc_e);
          }
       }
@@ -436,14 +444,14 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 #endif
    }
    __syncthreads();
-   tile_phase<T, 0, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
-   tile_phase<T, 0, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
-   tile_phase<T, 1, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
-   tile_phase<T, 1, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
-   tile_phase<T, 2, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
-   tile_phase<T, 2, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
-   tile_phase<T, 3, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e);
-   tile_phase<T, 3, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e);
+   tile_phase<T, 0, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 0, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 1, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 1, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 2, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 2, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 3, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 3, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
    if (!valid) return;
    C out[12];
    // boundary points of open bc are zeroed in output and input
@@ -484,6 +492,120 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 #endif
       store_spinor(a.r, e, out);
    }
+}
+
+// Dwhat / Dwhat_dble in two tile passes (Dw_dble.c:1792-1888):
+//   pass 1 (odd points):  t_o = A_oo^-1 (-1/2 sum of hops from s_e), the even
+//                         spinors of the tile staged in LDS;
+//   pass 2 (even points): r_e = (A_ee + i mu g5) s_e + 1/2 sum of hops from
+//                         t_o, each link loaded once by its odd lane, which
+//                         passes the even neighbour's term through LDS.
+template <typename T, int TP = TileSize<T>::n>
+__global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
+{
+   typedef typename cplx_of<T>::t C;
+   __shared__ C S[12][TP];
+   const int t = threadIdx.x;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e = e0 + t, o = a.h + e;
+   const bool valid = e < a.h;
+   bool ob = false;
+   TileNbr<T> nb;
+   C acc[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) acc[c] = cmk<C>(0, 0);
+   if (valid)
+   {
+#if OQH_NBR_PRELOAD
+#pragma unroll
+      for (int k = 0; k < 8; k++) nb.v[k] = a.nbr[(size_t)k * a.V + o];
+#endif
+      ob = a.masked && a.bnd[o];
+      C se[12];
+      if (a.masked && a.bnd[e])
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++) se[c] = cmk<C>(0, 0);
+      }
+      else
+         load_spinor(a.s, e, se);
+#pragma unroll
+      for (int c = 0; c < 12; c++) S[c][t] = se[c];
+   }
+   __syncthreads();
+   tile_phase<T, 0, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 0, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 1, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 1, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 2, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 2, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 3, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 3, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   if (!valid) return;
+   if (ob)
+   {
+      zero_spinor(a.t, o);
+      return;
+   }
+   C out[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], a.coef);
+   pauli_site<C, OQH_NT_SW>(a.sw, o, (T)0, acc, out);
+   store_spinor(a.t, o, out);
+}
+
+template <typename T, int TP = TileSize<T>::n>
+__global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
+{
+   typedef typename cplx_of<T>::t C;
+   __shared__ C X[XBUF][6][TP];
+   const int t = threadIdx.x;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e = e0 + t, o = a.h + e;
+   const bool valid = e < a.h;
+   bool eb = false;
+   TileNbr<T> nb;
+   C to[12], acc[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      acc[c] = cmk<C>(0, 0);
+      to[c] = cmk<C>(0, 0);
+   }
+   if (valid)
+   {
+#if OQH_NBR_PRELOAD
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+      {
+         nb.v[k] = a.nbr[(size_t)k * a.V + o];
+         nb.v[8 + k] = a.nbr[(size_t)k * a.V + e];
+      }
+#endif
+      eb = a.masked && a.bnd[e];
+      if (!(a.masked && a.bnd[o])) load_spinor((const C *)a.t, o, to);
+   }
+   const C *tz = a.t;
+   tile_phase<T, 0, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 0, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 1, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 1, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 2, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 2, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 3, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 3, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   if (!valid) return;
+   if (eb)
+   {
+      zero_spinor(a.r, e);
+      return;
+   }
+   C se[12], out[12];
+   load_spinor(a.s, e, se);
+   pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, out);
+#pragma unroll
+   for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc[c], a.coef));
+   store_spinor(a.r, e, out);
 }
 
 template <typename T>
@@ -626,6 +748,17 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
    {
       // odd intermediate t = A_oo^-1 (-1/2 hop(s_e)) in a scratch field
       a.t = (C *)tmp_buffer(units_for(V, 12) * sizeof(C));
+#if OQH_HAT_TILE
+      {
+         constexpr int TPn = TileSize<T>::n;
+         unsigned nt = (unsigned)((h + TPn - 1) / TPn);
+         a.coef = (T)-0.5;
+         k_hat1_tile<T><<<nt, TPn, 0, L.stream>>>(a);
+         a.coef = (T)0.5;
+         a.mu = mu;
+         k_hat2_tile<T><<<nt, TPn, 0, L.stream>>>(a);
+      }
+#else
       a.coef = (T)-0.5;
       a.p0 = h;
       a.np = h;
@@ -635,6 +768,7 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
       a.p0 = 0;
       a.np = h;
       launch<T, OP_HAT2>(a);
+#endif
       zero_bnd<T>((C *)s.dev, 0, h);   // cps(d)_int_bnd side effect
       break;
    }
