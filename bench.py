@@ -60,6 +60,69 @@ def measured_traffic():
     return None
 
 
+def solve_times(args):
+    """Time-to-solution of SAP+GCR on the 48^4 solver workload: periodic bc,
+    Haar-random links (numpy, seed 3) smoothed by 50 Euler steps (eps 0.01) of
+    the Wilson flow, csw=1.77, m0=-0.35, Gaussian source; FGCR with nkv=24 to
+    relative residual 1e-10, SAP on 4^4 blocks (isolv=1, nmr=4, ncy=5).  The
+    plain solver (FP32 SAP, FP64 GCR) and the variant with FP16 storage of the
+    block solver's links and SW matrices are timed on the same field and
+    source; the true residual |eta - Dw_dble psi|/|eta| is recomputed."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    L4 = (48, 48, 48, 48)
+    lat = pyoqh.Lattice(L4, (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
+    out = {"workload": "SAP+GCR 48^4 periodic, Wilson-flowed (50 x 0.01) Haar links, csw=1.77, "
+                       "m0=-0.35, nkv=24, res=1e-10, SAP 4^4 blocks isolv=1 nmr=4 ncy=5"}
+    try:
+        V = lat.V
+        rng = np.random.default_rng(3)
+        log("bench: solver workload, generating %d SU(3) links" % (4 * V))
+        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        lib = lat.lib
+        lib.fwd_euler(50, 0.01)
+        lib.set_ud_phase()
+        lib.oqh_set_sap_parms.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        lib.sap_gcr.restype = C.c_double
+        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.norm_square_dble.restype = pyoqh.Qflt
+        lib.norm_square_dble.argtypes = [C.c_int, C.c_int, P]
+        lib.mulr_spinor_add_dble.argtypes = [C.c_int, C.c_int, P, P, C.c_double]
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
+        lat.put_sd(e, rng.standard_normal((V, 24)))
+
+        def nsq(f):
+            q = lib.norm_square_dble(V, 0, f)
+            return q.q[0] + q.q[1]
+
+        for name, prec in (("plain_fp32_sap", 0), ("fp16_storage_sap", 1)):
+            lib.oqh_set_sap_prec(prec)
+            ifail, status = C.c_int(), C.c_int()
+            lat.sync()
+            t0 = time.perf_counter()
+            lib.sap_gcr(24, 1024, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+            lat.sync()
+            dt = time.perf_counter() - t0
+            lib.Dw_dble(0.0, p, r)
+            lib.mulr_spinor_add_dble(V, 0, r, e, -1.0)
+            res = (nsq(r) / nsq(e)) ** 0.5
+            out[name] = {"seconds": round(dt, 3), "status": status.value, "ifail": ifail.value,
+                         "true_residual": float("%.3e" % res)}
+            log("bench: %s %.3f s status %d residual %.2e" % (name, dt, status.value, res))
+        lib.oqh_set_sap_prec(0)
+        if out["plain_fp32_sap"]["ifail"] == 0 and out["fp16_storage_sap"]["ifail"] == 0:
+            out["speedup_fp16_vs_plain"] = round(out["plain_fp32_sap"]["seconds"] /
+                                                 out["fp16_storage_sap"]["seconds"], 3)
+    finally:
+        lat.close()
+    return out
+
+
 def cpu_baseline(V):
     """Reference CPU Dw_dble (oracle/_ref/ref_64x32x32x32_avx: the
     reference's modules built with -DAVX -DFMA3, OpenMP, 16 threads)."""
@@ -92,6 +155,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -139,9 +203,10 @@ def main():
     others["Dwhat_dble_GBps"] = round(BYTES["Dwhat_dble"] * V / ms / 1e6, 1)
     ms = lib.oqh_time_dirac(3, 0.0, s32, r32, 50)
     others["Dwhat_GBps"] = round(BYTES["Dwhat"] * V / ms / 1e6, 1)
-    lat.close()
 
     copy_bw = copy_bandwidth(lib)
+    lat.close()
+    solve = None if args.no_solve else solve_times(args)
     cpu = None if args.no_cpu else cpu_baseline(V)
 
     out = {
@@ -166,6 +231,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,
+        "solve": solve,
     }
     print(json.dumps(out))
 

@@ -79,6 +79,11 @@ void oqh_set_bc_parms(int type, double cF, double cF_prime);
 void oqh_set_sw_parms(double m0, double csw);
 void oqh_set_tm_parms(int eoflg);
 void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
+/* storage precision of the SAP block solver (sap, sap_gcr; no counterpart in
+   the reference): 0 = FP32 links and SW matrices as in sap/sap.c (default),
+   1 = FP16 links and SW matrices with FP32 arithmetic (blocks of 256 points;
+   other block sizes use FP32).  The outer FGCR keeps its precision. */
+void oqh_set_sap_prec(int prec);
 
 /* gauge field in the layout of udfld() (uflds/uflds.c:113-156; 8 links per
    odd point, README.global).  Import invalidates the SW term and the

@@ -67,6 +67,7 @@ struct Lib
    double m0 = 0.0, csw = 0.0;
    int eoflg = 0;
    int sap_bs[4] = {0, 0, 0, 0}, sap_isolv = 0, sap_nmr = 0, sap_ncy = 0;
+   int sap_prec = 0;  // storage of links/SW in the SAP block solver: 0 FP32, 1 FP16
 
    // geometry tables on the device
    int *d_nbr = nullptr;            // [8][V]: x+mu at 2mu, x-mu at 2mu+1; bit 31: boundary point
@@ -82,6 +83,7 @@ struct Lib
    bool ud_phase = false;  // quark phase of set_ud_phase() applied to d_ud
    bool swd_inv[2] = {false, false};
    int swd_epoch = 0;  // incremented whenever the contents of d_swd change
+   int u_epoch = 0;    // incremented whenever d_u is remade
 
    std::map<uintptr_t, Field> reg;
 

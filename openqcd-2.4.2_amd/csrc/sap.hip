@@ -29,11 +29,16 @@
 #include <cstdio>
 #include <cstring>
 
+#include <hip/hip_fp16.h>
+
 #include "dirac_dev.h"
 
 namespace oqh {
 
 constexpr int SWG = 256;  // work-group size of the block kernels
+#ifndef OQH_SAP_REG
+#define OQH_SAP_REG 1
+#endif
 
 struct SapGrid
 {
@@ -52,6 +57,10 @@ struct SapGrid
    float2 *d_scr = nullptr;              // [nbh][3][Ub]
    float2 *d_s0g = nullptr;              // global field of block corrections
    float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
+   __half2 *d_u16 = nullptr;             // FP16 links (FP16 variant of the solver)
+   __half2 *d_swb16 = nullptr;           // FP16 block SW field
+   int u16_epoch = -1, swb16_epoch = -1; // versions they were made from
+   int swb_version = 0;                  // incremented by every assign_swb
    std::vector<int> imb;                 // host copy
    std::vector<int> bo;                  // [nb][4]
 };
@@ -426,6 +435,348 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgs a)
    }
 }
 
+// ---------------------------------------------------------------------------
+// the Schwarz sweep with the block held in registers (blocks of 256 points)
+//
+// Lane x of the work-group owns block point x: its Pauli pair, the links
+// U(x,mu) to its in-block forward neighbours and the values of the block
+// fields s0, s1, s2 at x stay in registers for the whole block solve, so the
+// global memory is read once (links, SW, residual) and written once per
+// block.  A hopping pass exchanges only spinors: each lane puts its input
+// value into LDS (F), adds the forward terms of its neighbours x+mu from F and
+// computes with its own link the backward term of x+mu, which it passes
+// through LDS (X, two colour vectors after the spin projection).  The terms
+// are added in the order of blk_hop (mu = 0..3, forward then backward), so
+// the arithmetic is that of k_sap_blocks.
+// ---------------------------------------------------------------------------
+
+constexpr int RBV = 256;
+
+// storage of links and SW matrices in the block solver: float2, or __half2
+// in the FP16 variant (arithmetic in FP32 either way)
+__device__ inline float2 to_f2(float2 v) { return v; }
+__device__ inline float2 to_f2(__half2 v) { return __half22float2(v); }
+template <typename S> __device__ inline S from_f2(float2 v);
+template <> __device__ inline float2 from_f2<float2>(float2 v) { return v; }
+template <> __device__ inline __half2 from_f2<__half2>(float2 v) { return __float22half2_rn(v); }
+
+template <typename S> struct RegBlock
+{
+   S u[4][9];  // U(x,mu) if x+mu is in the block
+   S m[36];    // Pauli pair at x (units as in the SW field)
+   int fwd[4], bwd[4];
+   bool bp;    // open-bc boundary point
+};
+
+// out = (A + i mu) v on the two Weyl halves, A from registers (pauli_half
+// with unit stride)
+template <typename S>
+__device__ inline void pauli_reg(const S *m, float mu, const float2 *v, float2 *out, int stride = 1)
+{
+#pragma unroll
+   for (int hh = 0; hh < 2; hh++)
+   {
+      float2 q[18];
+#pragma unroll
+      for (int k = 0; k < 18; k++) q[k] = to_f2(m[(18 * hh + k) * stride]);
+      const float2 *w = v + 6 * hh;
+      float2 *o = out + 6 * hh;
+      float mm = hh ? -mu : mu;
+      float d[6] = {q[0].x, q[0].y, q[1].x, q[1].y, q[2].x, q[2].y};
+#pragma unroll
+      for (int i = 0; i < 6; i++) o[i] = make_float2(d[i] * w[i].x - mm * w[i].y, d[i] * w[i].y + mm * w[i].x);
+      int p = 3;
+#pragma unroll
+      for (int i = 0; i < 6; i++)
+#pragma unroll
+         for (int j = i + 1; j < 6; j++)
+         {
+            float2 hij = q[p];
+            o[i] = cadd(o[i], cmul(hij, w[j]));
+            o[j] = cadd(o[j], cmulc(hij, w[i]));
+            p++;
+         }
+   }
+}
+
+// OUT: parity of the points whose terms are needed (0 even, 1 odd, 2 all);
+// lane x computes its own forward term if its output is needed and the
+// backward term of x+mu if the output of x+mu is needed, so that for one
+// parity the even and the odd lanes (separate wavefronts) share the work
+template <int MU, int OUT, typename S>
+__device__ inline void rb_hop_mu(const RegBlock<S> &b, bool odd, const float2 *v, float2 *acc, float2 (*F)[RBV],
+                                 float2 (*X)[RBV])
+{
+   const int x = threadIdx.x;
+   const int y = b.fwd[MU];
+   const bool own = (OUT == 2) || (OUT == (int)odd);
+   const bool other = (OUT == 2) || (OUT != (int)odd);
+   if (y < RBV)
+   {
+      float2 h0[3], h1[3], c0[3], c1[3], u[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[k] = to_f2(b.u[MU][k]);
+      if (own)
+      {
+         float2 sp[12];
+#pragma unroll
+         for (int c = 0; c < 12; c++) sp[c] = F[c][y];
+         spin_project<float2, MU, true>(sp, h0, h1);
+         su3_mul<float2, false>(u, h0, c0);
+         su3_mul<float2, false>(u, h1, c1);
+         spin_expand_add<float2, MU, true>(c0, c1, acc);
+      }
+      if (other)
+      {
+         // backward term of y from x through the same link
+         spin_project<float2, MU, false>(v, h0, h1);
+         su3_mul<float2, true>(u, h0, c0);
+         su3_mul<float2, true>(u, h1, c1);
+#pragma unroll
+         for (int c = 0; c < 3; c++)
+         {
+            X[c][y] = c0[c];
+            X[3 + c][y] = c1[c];
+         }
+      }
+   }
+   __syncthreads();
+   if (own && b.bwd[MU] < RBV)
+   {
+      float2 c0[3], c1[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+      {
+         c0[c] = X[c][x];
+         c1[c] = X[3 + c][x];
+      }
+      spin_expand_add<float2, MU, false>(c0, c1, acc);
+   }
+   __syncthreads();
+}
+
+// acc = sum of the hopping terms of the field v over in-block neighbours
+// (at the points of parity OUT); values at boundary points of open bc count
+// as zero (blk_hop skips them)
+template <int OUT, typename S>
+__device__ inline void rb_hop(const RegBlock<S> &b, bool odd, const float2 *vin, float2 *acc, float2 (*F)[RBV],
+                              float2 (*X)[RBV])
+{
+   const int x = threadIdx.x;
+   float2 v[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      v[c] = b.bp ? make_float2(0.0f, 0.0f) : vin[c];
+      if (OUT == 2 || OUT != (int)odd) F[c][x] = v[c];
+      acc[c] = make_float2(0.0f, 0.0f);
+   }
+   __syncthreads();
+   rb_hop_mu<0, OUT>(b, odd, v, acc, F, X);
+   rb_hop_mu<1, OUT>(b, odd, v, acc, F, X);
+   rb_hop_mu<2, OUT>(b, odd, v, acc, F, X);
+   rb_hop_mu<3, OUT>(b, odd, v, acc, F, X);
+}
+
+// block sums of |s2|^2 and (s2, s1) as in blk_mres_dev, lanes with act
+__device__ inline void rb_sums(bool act, const float2 *s2, const float2 *s1, double &rr, double &zr, double &zi)
+{
+   rr = zr = zi = 0.0;
+   if (act)
+   {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+      {
+         float nr = 0.0f, pr = 0.0f, pi = 0.0f;
+#pragma unroll
+         for (int c = 0; c < 3; c++)
+         {
+            float2 u = s2[3 * k + c], v = s1[3 * k + c];
+            nr += u.x * u.x + u.y * u.y;
+            pr += u.x * v.x + u.y * v.y;
+            pi += u.x * v.y - u.y * v.x;
+         }
+         rr += (double)nr;
+         zr += (double)pr;
+         zi += (double)pi;
+      }
+   }
+   wg_sum3(rr, zr, zi);
+}
+
+#ifndef OQH_SAP_WAVES16
+#define OQH_SAP_WAVES16 2
+#endif
+#ifndef OQH_SAP_WAVES32
+#define OQH_SAP_WAVES32 1
+#endif
+// FP16 storage frees enough registers for two wavefronts per SIMD
+template <int ISOLV, typename S>
+__global__ __launch_bounds__(RBV)
+__attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? OQH_SAP_WAVES16 : OQH_SAP_WAVES32)))
+void k_sap_reg(SArgs a, const S *u16, const S *sw16)
+{
+   __shared__ float2 F[12][RBV];
+   __shared__ float2 X[6][RBV];
+   // FP16 variant: the Pauli matrices of the block live in LDS
+   constexpr bool MLDS = sizeof(S) == 4;
+   __shared__ S MS[MLDS ? 36 : 1][RBV];
+   const int x = threadIdx.x;
+   const int n = a.n0 + blockIdx.x;
+   const int *imb = a.imb + (size_t)n * RBV;
+   const int gx = imb[x];
+   const bool even = x < a.bh;
+   RegBlock<S> b;
+   b.bp = a.masked && a.bnd[gx];
+   // links and SW matrices: the FP32 fields, or their FP16 copies
+   const S *ub = sizeof(S) == sizeof(float2) ? (const S *)a.u : u16;
+   const S *swb = sizeof(S) == sizeof(float2) ? (const S *)a.sw : sw16;
+#pragma unroll
+   for (int mu = 0; mu < 4; mu++)
+   {
+      b.fwd[mu] = a.bnb[(2 * mu) * RBV + x];
+      b.bwd[mu] = a.bnb[(2 * mu + 1) * RBV + x];
+      if (b.fwd[mu] < RBV)
+      {
+         const S *up = ub + (blk_link(a, gx, imb[b.fwd[mu]], mu, true) - a.u);
+#pragma unroll
+         for (int k = 0; k < 9; k++) b.u[mu][k] = up[k * SB];
+      }
+   }
+   {
+      const S *m = swb + (size_t)(gx >> 6) * (36 * SB) + (gx & (SB - 1));
+#pragma unroll
+      for (int k = 0; k < 36; k++)
+      {
+         if (MLDS)
+            MS[k][x] = m[k * SB];
+         else
+            b.m[k] = m[k * SB];
+      }
+   }
+   const S *mp = MLDS ? &MS[0][x] : b.m;
+   const int mst = MLDS ? RBV : 1;
+   float2 s0[12], s1[12], s2[12], acc[12];
+   load_spinor((const float2 *)a.rho, gx, s1);
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      s0[c] = make_float2(0.0f, 0.0f);
+      s2[c] = make_float2(0.0f, 0.0f);
+      if (!ISOLV && b.bp) s1[c] = make_float2(0.0f, 0.0f);
+   }
+
+   if (ISOLV)
+   {
+      // odd points: s1 = A_oo^-1 s1, then even points: s1 += 1/2 hops
+      // (Dwoo_blk, Dweo_blk of sap.c:759-761)
+      if (!even)
+      {
+         float2 o[12];
+         pauli_reg(mp, 0.0f, s1, o, mst);
+#pragma unroll
+         for (int c = 0; c < 12; c++) s1[c] = b.bp ? make_float2(0.0f, 0.0f) : o[c];
+      }
+      rb_hop<0>(b, !even, s1, acc, F, X);
+      if (even)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+            s1[c] = b.bp ? make_float2(0.0f, 0.0f) : cadd(s1[c], cscale(acc[c], 0.5f));
+      }
+   }
+
+   // minimal residual steps (blk_solv.c:1117-1186)
+   for (int imr = 0; imr < a.nmr; imr++)
+   {
+      if (ISOLV)
+      {
+         // s2 (odd) = A_oo^-1 (-1/2 hops of s1_e); s2 (even) = A_ee s1 + 1/2 hops of s2_o
+         rb_hop<1>(b, !even, s1, acc, F, X);
+         if (!even)
+         {
+#pragma unroll
+            for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
+            pauli_reg(mp, 0.0f, acc, s2, mst);
+            if (b.bp)
+            {
+#pragma unroll
+               for (int c = 0; c < 12; c++) s2[c] = make_float2(0.0f, 0.0f);
+            }
+         }
+         rb_hop<0>(b, !even, s2, acc, F, X);
+         if (even)
+         {
+            pauli_reg(mp, a.mu, s1, s2, mst);
+#pragma unroll
+            for (int c = 0; c < 12; c++)
+               s2[c] = b.bp ? make_float2(0.0f, 0.0f) : cadd(s2[c], cscale(acc[c], 0.5f));
+         }
+      }
+      else
+      {
+         rb_hop<2>(b, !even, s1, acc, F, X);
+         pauli_reg(mp, even ? a.mu : a.muo, s1, s2, mst);
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+            s2[c] = b.bp ? make_float2(0.0f, 0.0f) : cadd(s2[c], cscale(acc[c], -0.5f));
+      }
+      double rr, zr, zi;
+      rb_sums(ISOLV ? even : true, s2, s1, rr, zr, zi);
+      float r = (float)rr;
+      if (r < 2.0f * FLT_MIN) break;
+      r = 1.0f / r;
+      float2 z = make_float2((float)zr * r, (float)zi * r);
+      if (!ISOLV || even)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            s0[c] = make_float2(s0[c].x + (z.x * s1[c].x - z.y * s1[c].y), s0[c].y + (z.x * s1[c].y + z.y * s1[c].x));
+            s1[c] = make_float2(s1[c].x - (z.x * s2[c].x - z.y * s2[c].y), s1[c].y - (z.x * s2[c].y + z.y * s2[c].x));
+         }
+      }
+   }
+
+   if (ISOLV)
+   {
+      // odd solution s0_o = s1_o - A_oo^-1 (-1/2 hops of s0_e) (Dwoe_blk, Dwoo_blk)
+      rb_hop<1>(b, !even, s0, acc, F, X);
+      if (!even)
+      {
+         float2 o[12];
+#pragma unroll
+         for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
+         pauli_reg(mp, 0.0f, acc, o, mst);
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            if (b.bp) o[c] = make_float2(0.0f, 0.0f);
+            s0[c] = csub(s1[c], o[c]);
+         }
+      }
+   }
+
+   float2 p[12];
+   load_spinor((const float2 *)a.psi, gx, p);
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c] = cadd(p[c], s0[c]);
+   store_spinor(a.psi, gx, p);
+   if (ISOLV && !even)
+   {
+      // odd residual -i mu gamma5 s0 (update_flds1) or 0 (eoflg, update_flds2)
+      float m = a.eoflg ? 0.0f : a.mu;
+#pragma unroll
+      for (int c = 0; c < 12; c++)
+      {
+         float cm = (c < 6) ? -m : m;
+         s1[c] = make_float2(-cm * s0[c].y, cm * s0[c].x);
+      }
+   }
+   store_spinor(a.rho, gx, s1);
+   store_spinor(a.s0g, gx, s0);
+}
+
 // rho(y) -= hopping terms into y from the block corrections of the
 // neighbouring blocks, for the face points y of the blocks of one colour
 __global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, const unsigned char *fmask, int nf)
@@ -528,6 +879,7 @@ static void free_grid()
    dev_free(G.d_imb); dev_free(G.d_bnb);
    for (int c = 0; c < 2; c++) { dev_free(G.d_face[c]); dev_free(G.d_fmask[c]); G.d_face[c] = nullptr; G.d_fmask[c] = nullptr; }
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
+   dev_free(G.d_u16); dev_free(G.d_swb16);
    G = SapGrid();
 }
 
@@ -672,6 +1024,7 @@ static int assign_swb(int set)
    HIP_OK(hipStreamSynchronize(L.stream));
    G.swb_set = set;
    G.swb_epoch = L.swd_epoch;
+   G.swb_version++;
    return ifail;
 }
 
@@ -712,12 +1065,52 @@ void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
    for (int ic = 0; ic < 2; ic++)
    {
       a.n0 = ic ? G.nbh : 0;
-      if (isolv)
+      if (OQH_SAP_REG && G.bvol == RBV && L.sap_prec == 1)
+      {
+         if (isolv)
+            k_sap_reg<1, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+         else
+            k_sap_reg<0, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+      }
+      else if (OQH_SAP_REG && G.bvol == RBV)
+      {
+         if (isolv)
+            k_sap_reg<1, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+         else
+            k_sap_reg<0, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      }
+      else if (isolv)
          k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
       else
          k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
       int c = 1 - ic;
       if (G.nface[c]) k_sap_faces<<<(G.nface[c] + 255) / 256, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c]);
+   }
+}
+
+__global__ void k_f32_to_f16(const float2 *__restrict__ a, __half2 *__restrict__ b, size_t n)
+{
+   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i < n) b[i] = __float22half2_rn(a[i]);
+}
+
+// FP16 copies of the links and of the block SW field (FP16 variant)
+static void ensure_half_fields()
+{
+   Lib &L = lib();
+   size_t nu = units_for(L.g.V / 2, 72), ns = units_for(L.g.V, 36);
+   if (!G.d_u16) G.d_u16 = (__half2 *)dev_alloc(nu * sizeof(__half2));
+   if (!G.d_swb16) G.d_swb16 = (__half2 *)dev_alloc(ns * sizeof(__half2));
+   if (!G.d_u16 || !G.d_swb16) return;
+   if (G.u16_epoch != L.u_epoch)
+   {
+      k_f32_to_f16<<<(unsigned)((nu + 255) / 256), 256, 0, L.stream>>>(L.d_u, G.d_u16, nu);
+      G.u16_epoch = L.u_epoch;
+   }
+   if (G.swb16_epoch != G.swb_version)
+   {
+      k_f32_to_f16<<<(unsigned)((ns + 255) / 256), 256, 0, L.stream>>>(G.d_swb, G.d_swb16, ns);
+      G.swb16_epoch = G.swb_version;
    }
 }
 
@@ -745,6 +1138,7 @@ bool sap_prepare(int isolv, const char *where, bool quiet)
          return false;
       }
    }
+   if (L.sap_prec == 1 && G.bvol == RBV) ensure_half_fields();
    return true;
 }
 
@@ -776,6 +1170,16 @@ extern "C" void sap(float mu, int isolv, int nmr, spinor *psi, spinor *eta)
    sap_device(mu, isolv, nmr, (float2 *)p.dev, (float2 *)r.dev);
    finish(p, F32, V, true);
    finish(r, F32, V, true);
+}
+
+extern "C" void oqh_set_sap_prec(int prec)
+{
+   if (prec != 0 && prec != 1)
+   {
+      fail("oqh_set_sap_prec", "the precision must be 0 (FP32) or 1 (FP16 storage)");
+      return;
+   }
+   lib().sap_prec = prec;
 }
 
 extern "C" void alloc_bgr(blk_grid_t grid)

@@ -441,6 +441,7 @@ extern "C" void oqh_finalize(void)
    sap_release();
    release_flow();
    clear_ws_marks();
+   L.sap_prec = 0;
    for (auto &kv : L.reg) dev_free(kv.second.base);
    L.reg.clear();
    for (int k = 0; k < 4; k++)

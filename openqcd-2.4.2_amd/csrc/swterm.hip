@@ -413,6 +413,7 @@ void ensure_u()
    if (L.u_valid) return;
    convert_f64_to_f32(L.d_ud, L.d_u, units_for(L.g.V / 2, 72), L.stream);
    L.u_valid = true;
+   L.u_epoch++;
 }
 
 }  // namespace oqh

@@ -58,6 +58,7 @@ _SIG = {
     "oqh_copy_bandwidth": (_D, [_D, _I]), "oqh_export_frc": (None, [P]),
     "plaq_frc": (None, []), "fwd_euler": (None, [_I, _D]),
     "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
+    "oqh_set_sap_prec": (None, [_I]),
 }
 for _n in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
     _SIG[_n] = (None, [_D, P, P])

@@ -153,3 +153,35 @@ def test_sap_gcr_failure_modes(setup):
     lat.put_sd(p, eta)
     rho = lat.lib.sap_gcr(16, 64, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
     assert rho == 0.0 and ifail.value == 0 and not lat.get_sd(p).any()
+
+
+@pytest.mark.gpu
+def test_fp16_storage_variant(setup):
+    """The FP16-storage block solver (links and SW matrices in FP16, FP32
+    arithmetic) approximates the FP32 Schwarz cycle to the FP16 rounding
+    level, and SAP+GCR with it still reaches the requested double-precision
+    residual (the outer FGCR corrects the preconditioner's rounding)."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    out = {}
+    for prec in (0, 1):
+        lib.oqh_set_sap_prec(prec)
+        psi, rho = lat.alloc_s(), lat.alloc_s()
+        lat.put_s(psi, np.zeros_like(s32))
+        lat.put_s(rho, s32)
+        lib.sap(c["mu"], c["isolv"], 4, psi, rho)
+        out[prec] = lat.get_s(psi).astype(np.float64)
+        lat.free(psi)
+        lat.free(rho)
+    rel = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
+    print("fp16 vs fp32 SAP cycle", rel)
+    assert rel < 5e-3
+    e, p = lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(e, eta)
+    ifail, status = C.c_int(), C.c_int()
+    lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    o.sw_term(3)
+    res = np.linalg.norm(eta - o.Dw_dble(c["mu"], lat.get_sd(p).copy())) / np.linalg.norm(eta)
+    print("fp16 SAP+GCR status", status.value, "true residual", res)
+    assert ifail.value == 0 and res <= 1.2e-10
+    lib.oqh_set_sap_prec(0)

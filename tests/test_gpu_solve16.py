@@ -25,16 +25,19 @@ def test_sap_gcr_16_4():
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
         V = lat.V
         eta = rng.standard_normal((V, 24))
-        for isolv, bs in ((0, (4, 4, 4, 4)), (1, (8, 4, 4, 4))):
+        for isolv, bs, prec in ((0, (4, 4, 4, 4), 0), (1, (8, 4, 4, 4), 0), (1, (4, 4, 4, 4), 0),
+                                (1, (4, 4, 4, 4), 1), (0, (4, 4, 4, 4), 1)):
             b = np.array(bs, np.int32)
             lib.oqh_set_sap_parms(b.ctypes.data_as(P), isolv, 4, 5)
+            lib.oqh_set_sap_prec(prec)
             e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
             lat.put_sd(e, eta)
             ifail, status = C.c_int(), C.c_int()
             rho = lib.sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
             lib.Dw_dble(0.0, p, r)
             res = np.linalg.norm(eta - lat.get_sd(r)) / np.linalg.norm(eta)
-            print("16^4 isolv", isolv, "bs", bs, "status", status.value, "rho", rho, "true residual", res)
+            print("16^4 isolv", isolv, "bs", bs, "prec", prec, "status", status.value, "rho", rho,
+                  "true residual", res)
             assert ifail.value == 0 and res <= 1.2e-10
             for f in (e, p, r):
                 lat.free(f)
