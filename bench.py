@@ -204,6 +204,20 @@ def main():
     ms = lib.oqh_time_dirac(3, 0.0, s32, r32, 50)
     others["Dwhat_GBps"] = round(BYTES["Dwhat"] * V / ms / 1e6, 1)
 
+    # batched operator: 12 fields (the spin-colour components of a point
+    # source) per call, links and Pauli matrices loaded once for 8 of them
+    import ctypes as C
+    assert lat.sw_term(3) == 0
+    nm = 12
+    fs = [lat.alloc_sd() for _ in range(nm)]
+    fr = [lat.alloc_sd() for _ in range(nm)]
+    for f in fs:
+        lat.put_sd(f, rng.standard_normal((V, 24)))
+    arr_s, arr_r = (C.c_void_p * nm)(*fs), (C.c_void_p * nm)(*fr)
+    ms = lib.oqh_time_dirac_multi(1, 0.0, nm, arr_s, arr_r, 20)
+    others["Dw_dble_multi12_ms_per_field"] = round(ms / nm, 4)
+    others["Dw_dble_multi12_effective_GBps"] = round(nm * BYTES["Dw_dble"] * V / ms / 1e6, 1)
+
     copy_bw = copy_bandwidth(lib)
     lat.close()
     solve = None if args.no_solve else solve_times(args)

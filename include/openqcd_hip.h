@@ -109,6 +109,15 @@ void oqh_sync(void);
    with HIP events on the library stream; op: 0 Dw_dble, 1 Dw, 2 Dwhat_dble,
    3 Dwhat.  Returns the average milliseconds per call. */
 double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall);
+/* Dw_dble / Dw applied to n fields at once, r[k] = D s[k] (no counterpart in
+   the reference: a batched form of dirac/Dw_dble.c:1377 and dirac/Dw.c:973
+   that loads every link and Pauli matrix once for up to 8 fields).  The
+   fields must be full device fields of the library and s[k] != r[k]; unlike
+   Dw_dble, the inputs are left unchanged on the boundary points of open bc. */
+void oqh_Dw_dble_multi(double mu, int n, spinor_dble **s, spinor_dble **r);
+void oqh_Dw_multi(float mu, int n, spinor **s, spinor **r);
+/* average milliseconds of ncall batched applications (prec 1: FP64, 0: FP32) */
+double oqh_time_dirac_multi(int prec, double mu, int n, void **s, void **r, int ncall);
 /* bandwidth (GB/s, bytes read + written) of a plain device copy of gbytes
    in total, averaged over nrep launches */
 double oqh_copy_bandwidth(double gbytes, int nrep);

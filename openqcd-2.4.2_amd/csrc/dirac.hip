@@ -608,6 +608,184 @@ __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
    store_spinor(a.r, e, out);
 }
 
+// ---------------------------------------------------------------------------
+// Dw / Dw_dble on several fields at once (no reference counterpart): the
+// tile kernel with NR lanes per even-odd pair, one per right-hand side.  The
+// NR lanes of a pair are adjacent in a wavefront and load the same links and
+// Pauli matrices, so each such load is fetched once for all right-hand sides;
+// the spinor staging and exchange are per right-hand side.
+// ---------------------------------------------------------------------------
+
+constexpr int NRMAX = 8;
+
+template <typename T> struct MArgs
+{
+   typedef typename cplx_of<T>::t C;
+   const C *s[NRMAX];
+   C *r[NRMAX];
+   const C *u, *sw;
+   const int *nbr;
+   const unsigned char *bnd;
+   int V, h, n;  // n: number of fields of this launch
+   T mu, muo, coef;
+   int masked;
+};
+
+template <typename T, int MU, bool FWD, int TPP, int NR>
+__device__ inline void multi_phase(const MArgs<T> &a, int t, int p, int k, int e0, bool valid, const int *nb,
+                                   typename cplx_of<T>::t (*S)[TPP * NR], typename cplx_of<T>::t (*X)[TPP * NR],
+                                   const typename cplx_of<T>::t *sk, const typename cplx_of<T>::t *so,
+                                   typename cplx_of<T>::t *acc_o, typename cplx_of<T>::t *acc_e)
+{
+   typedef typename cplx_of<T>::t C;
+   const int e = e0 + p;
+   if (valid)
+   {
+      int ny = nb[2 * MU + (FWD ? 0 : 1)];
+      int y = ny & 0x7fffffff;
+      int ly = y - e0;
+      bool yin = (unsigned)ly < (unsigned)TPP;
+      const C *up = link_ptr(a.u, e, FWD ? 2 * MU : 2 * MU + 1);
+      C u[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) u[q] = up[q * SB];
+      C h0[3], h1[3], c0[3], c1[3];
+      if (!(a.masked && ny < 0))
+      {
+         C sy[12];
+         if (yin)
+         {
+#pragma unroll
+            for (int c = 0; c < 12; c++) sy[c] = S[c][ly * NR + k];
+         }
+         else
+            load_spinor(sk, y, sy);
+         spin_project<C, MU, FWD>(sy, h0, h1);
+         su3_mul<C, !FWD>(u, h0, c0);
+         su3_mul<C, !FWD>(u, h1, c1);
+         spin_expand_add<C, MU, FWD>(c0, c1, acc_o);
+      }
+      if (yin)
+      {
+         spin_project<C, MU, !FWD>(so, h0, h1);
+         su3_mul<C, FWD>(u, h0, c0);
+         su3_mul<C, FWD>(u, h1, c1);
+#pragma unroll
+         for (int c = 0; c < 3; c++)
+         {
+            X[c][ly * NR + k] = c0[c];
+            X[3 + c][ly * NR + k] = c1[c];
+         }
+      }
+   }
+   __syncthreads();
+   if (valid)
+   {
+      int nz = nb[8 + 2 * MU + (FWD ? 1 : 0)];
+      if (!(a.masked && nz < 0))
+      {
+         int z = nz & 0x7fffffff;
+         if ((unsigned)(z - a.h - e0) < (unsigned)TPP)
+         {
+            C c0[3], c1[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               c0[c] = X[c][t];
+               c1[c] = X[3 + c][t];
+            }
+            spin_expand_add<C, MU, !FWD>(c0, c1, acc_e);
+         }
+         else
+         {
+            C sz[12];
+            load_spinor(sk, z, sz);
+            hop_one<C, MU, !FWD>(sz, link_ptr(a.u, z - a.h, FWD ? 2 * MU : 2 * MU + 1), acc_e);
+         }
+      }
+   }
+   __syncthreads();
+}
+
+template <typename T, int TPP, int NR>
+__global__ __launch_bounds__(TPP * NR) void k_dw_multi(MArgs<T> a)
+{
+   typedef typename cplx_of<T>::t C;
+   constexpr int NL = TPP * NR;
+   __shared__ C S[12][NL];
+   __shared__ C X[6][NL];
+   const int t = threadIdx.x, p = t / NR, k = t % NR;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TPP;
+   const int e = e0 + p, o = a.h + e;
+   const bool valid = e < a.h && k < a.n;
+   const C *sk = a.s[k < a.n ? k : 0];
+   C *rk = a.r[k < a.n ? k : 0];
+   bool eb = false, ob = false;
+   int nb[16];
+   C so[12], acc_o[12], acc_e[12];
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      acc_o[c] = cmk<C>(0, 0);
+      acc_e[c] = cmk<C>(0, 0);
+      so[c] = cmk<C>(0, 0);
+   }
+   if (valid)
+   {
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+      {
+         nb[q] = a.nbr[(size_t)q * a.V + o];
+         nb[8 + q] = a.nbr[(size_t)q * a.V + e];
+      }
+      eb = a.masked && a.bnd[e];
+      ob = a.masked && a.bnd[o];
+      C se[12];
+      if (eb)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++) se[c] = cmk<C>(0, 0);
+      }
+      else
+         load_spinor(sk, e, se);
+#pragma unroll
+      for (int c = 0; c < 12; c++) S[c][t] = se[c];
+      if (!ob) load_spinor(sk, o, so);
+   }
+   __syncthreads();
+   multi_phase<T, 0, true, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 0, false, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 1, true, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 1, false, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 2, true, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 2, false, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 3, true, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   multi_phase<T, 3, false, TPP, NR>(a, t, p, k, e0, valid, nb, S, X, sk, so, acc_o, acc_e);
+   if (!valid) return;
+   C out[12];
+   if (ob)
+      zero_spinor(rk, o);
+   else
+   {
+      pauli_site<C, OQH_NT_SW>(a.sw, o, a.muo, so, out);
+#pragma unroll
+      for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_o[c], a.coef));
+      store_spinor(rk, o, out);
+   }
+   if (eb)
+      zero_spinor(rk, e);
+   else
+   {
+      C se[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) se[c] = S[c][t];
+      pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, out);
+#pragma unroll
+      for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_e[c], a.coef));
+      store_spinor(rk, e, out);
+   }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_bnd(typename cplx_of<T>::t *s, const unsigned char *bnd,
                                                   int p0, int np)
@@ -794,6 +972,108 @@ extern "C" void Dwoo(float mu, spinor *s, spinor *r) { apply<float>(K_DOO, mu, s
 extern "C" void Dwoe(spinor *s, spinor *r) { apply<float>(K_DOE, 0.0f, s, r, "Dwoe [Dw.c]"); }
 extern "C" void Dweo(spinor *s, spinor *r) { apply<float>(K_DEO, 0.0f, s, r, "Dweo [Dw.c]"); }
 extern "C" void Dwhat(float mu, spinor *s, spinor *r) { apply<float>(K_HAT, mu, s, r, "Dwhat [Dw.c]"); }
+
+namespace oqh {
+
+template <typename T, int NR> static void launch_multi(const MArgs<T> &a)
+{
+   constexpr int TPP = 256 / NR;
+   unsigned nt = (unsigned)((a.h + TPP - 1) / TPP);
+   if (nt) k_dw_multi<T, TPP, NR><<<nt, 256, 0, lib().stream>>>(a);
+}
+
+// n fields in chunks of up to NRMAX, each with the smallest power of two of
+// lanes per pair that holds it
+template <typename T> static void apply_multi(T mu, int n, void **sp, void **rp, const char *where)
+{
+   typedef typename cplx_of<T>::t C;
+   Lib &L = lib();
+   int prec = sizeof(T) == 8 ? F64 : F32;
+   if (!prepare(where, prec)) return;
+   if (n < 1 || !sp || !rp)
+   {
+      fail(where, "the number of fields must be positive");
+      return;
+   }
+   MArgs<T> a{};
+   a.u = (const C *)(sizeof(T) == 8 ? (const void *)L.d_ud : (const void *)L.d_u);
+   a.sw = (const C *)(sizeof(T) == 8 ? (const void *)L.d_swd : (const void *)L.d_sw);
+   a.nbr = L.d_nbr;
+   a.bnd = L.d_bnd;
+   a.V = L.g.V;
+   a.h = L.g.V / 2;
+   a.mu = mu;
+   a.muo = L.eoflg ? (T)0 : mu;
+   a.coef = (T)-0.5;
+   a.masked = (L.bc != 3);
+   for (int k0 = 0; k0 < n; k0 += NRMAX)
+   {
+      int m = (n - k0 < NRMAX) ? n - k0 : NRMAX;
+      for (int k = 0; k < m; k++)
+      {
+         int o1 = 0, o2 = 0;
+         void *s = device_field(sp[k0 + k], prec, &o1), *r = device_field(rp[k0 + k], prec, &o2);
+         if (!s || !r || o1 || o2)
+         {
+            fail(where, "the fields must be full device fields of the library");
+            return;
+         }
+         if (s == r)
+         {
+            fail(where, "Input and output fields must be different");
+            return;
+         }
+         a.s[k] = (const C *)s;
+         a.r[k] = (C *)r;
+      }
+      a.n = m;
+      if (m == 1)
+         launch_multi<T, 1>(a);
+      else if (m == 2)
+         launch_multi<T, 2>(a);
+      else if (m <= 4)
+         launch_multi<T, 4>(a);
+      else
+         launch_multi<T, 8>(a);
+   }
+}
+
+}  // namespace oqh
+
+extern "C" void oqh_Dw_dble_multi(double mu, int n, spinor_dble **s, spinor_dble **r)
+{
+   apply_multi<double>(mu, n, (void **)s, (void **)r, "oqh_Dw_dble_multi");
+}
+
+extern "C" void oqh_Dw_multi(float mu, int n, spinor **s, spinor **r)
+{
+   apply_multi<float>(mu, n, (void **)s, (void **)r, "oqh_Dw_multi");
+}
+
+extern "C" double oqh_time_dirac_multi(int prec, double mu, int n, void **s, void **r, int ncall)
+{
+   Lib &L = lib();
+   if (ncall <= 0) return 0.0;
+   if (prec) apply_multi<double>(mu, n, s, r, "oqh_time_dirac_multi");
+   else apply_multi<float>((float)mu, n, s, r, "oqh_time_dirac_multi");
+   HIP_OK(hipStreamSynchronize(L.stream));
+   hipEvent_t e0, e1;
+   HIP_OK(hipEventCreate(&e0));
+   HIP_OK(hipEventCreate(&e1));
+   HIP_OK(hipEventRecord(e0, L.stream));
+   for (int k = 0; k < ncall; k++)
+   {
+      if (prec) apply_multi<double>(mu, n, s, r, "oqh_time_dirac_multi");
+      else apply_multi<float>((float)mu, n, s, r, "oqh_time_dirac_multi");
+   }
+   HIP_OK(hipEventRecord(e1, L.stream));
+   HIP_OK(hipEventSynchronize(e1));
+   float ms = 0.0f;
+   HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+   HIP_OK(hipEventDestroy(e0));
+   HIP_OK(hipEventDestroy(e1));
+   return (double)ms / ncall;
+}
 
 extern "C" double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall)
 {

@@ -129,6 +129,7 @@ struct Res
    int slot = -1;
 };
 Res resolve(const void *p, int prec, int nsites, int slot, bool upload);
+void *device_field(const void *p, int prec, int *off);
 void finish(const Res &r, int prec, int nsites, bool download);
 
 // geometry and tables

@@ -295,6 +295,17 @@ static const Field *lookup(const void *p, int *off)
    return &f;
 }
 
+// device base of a registered field of the given precision (nullptr if p is
+// not one), with the site offset addressed by p
+void *device_field(const void *p, int prec, int *off)
+{
+   int o = 0;
+   const Field *f = p ? lookup(p, &o) : nullptr;
+   if (!f || f->prec != prec) return nullptr;
+   *off = o;
+   return f->base;
+}
+
 Res resolve(const void *p, int prec, int nsites, int slot, bool upload)
 {
    Lib &L = lib();

@@ -59,6 +59,8 @@ _SIG = {
     "plaq_frc": (None, []), "fwd_euler": (None, [_I, _D]),
     "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
     "oqh_set_sap_prec": (None, [_I]),
+    "oqh_Dw_dble_multi": (None, [_D, _I, P, P]), "oqh_Dw_multi": (None, [_F, _I, P, P]),
+    "oqh_time_dirac_multi": (_D, [_I, _D, _I, P, P, _I]),
 }
 for _n in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
     _SIG[_n] = (None, [_D, P, P])
