@@ -330,6 +330,143 @@ __global__ void k_max_final(const double *part, int nb, double *out)
 }
 
 // ---------------------------------------------------------------------------
+// fused Gram-Schmidt steps of fgcr (single precision, device coefficients)
+//
+// Each kernel applies one vector update and forms the partial sums of the
+// next scalar product in the same pass, with the point partition of
+// k_dot_partial (grid-stride over the points, RB-lane work-groups, the same
+// number of work-groups), so the products are bitwise those of the separate
+// salg.c calls.  The coefficients stay on the device: the update of a kernel
+// reads the value the previous reduction stored, and the host reads them back
+// once per Krylov vector.
+// ---------------------------------------------------------------------------
+
+// conj(u).v per point as salg.c (float per colour vector, then double)
+__device__ inline void pdot_reg(const float2 *u, const float2 *v, bool norm, double *re, double *im)
+{
+#pragma unroll
+   for (int k = 0; k < 4; k++)
+   {
+      float pr = 0.0f, pi = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+      {
+         float2 a = u[3 * k + i], b = norm ? a : v[3 * k + i];
+         pr += a.x * b.x + a.y * b.y;
+         if (!norm) pi += a.x * b.y - a.y * b.x;
+      }
+      *re += (double)pr;
+      *im += (double)pi;
+   }
+}
+
+// s, r, q: fields of n points from site 0.  GS_DOT: (q, s).  GS_AXPY_DOT:
+// s -= coef[ic] r, then (q, s).  GS_AXPY_NORM: s -= coef[ic] r, then |s|^2.
+// GS_SCALE_DOT: s *= 1/coef[ic].x (the norm of s), then (s, q).  GS_NORM: |s|^2.
+template <int MODE>
+__global__ __launch_bounds__(RB) void k_gs_partial(float2 *s, const float2 *r, const float2 *q, const float2 *coef,
+                                                   int ic, int n, dd *part)
+{
+   double re = 0.0, im = 0.0;
+   float zr = 0.0f, zi = 0.0f, sc = 0.0f;
+   if (MODE == GS_AXPY_DOT || MODE == GS_AXPY_NORM)
+   {
+      zr = -coef[ic].x;
+      zi = -coef[ic].y;
+   }
+   if (MODE == GS_SCALE_DOT) sc = 1.0f / coef[ic].x;
+   int stride = gridDim.x * RB;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+   {
+      size_t i0 = (size_t)(k >> 6) * (12 * SB) + (k & (SB - 1));
+      float2 a[12], b[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) a[c] = s[i0 + c * SB];
+      if (MODE == GS_AXPY_DOT || MODE == GS_AXPY_NORM)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            float2 w = r[i0 + c * SB];
+            a[c] = make_float2(a[c].x + (zr * w.x - zi * w.y), a[c].y + (zr * w.y + zi * w.x));
+            s[i0 + c * SB] = a[c];
+         }
+      }
+      if (MODE == GS_SCALE_DOT)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            a[c] = make_float2(sc * a[c].x, sc * a[c].y);
+            s[i0 + c * SB] = a[c];
+         }
+      }
+      if (MODE == GS_DOT || MODE == GS_AXPY_DOT || MODE == GS_SCALE_DOT)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++) b[c] = q[i0 + c * SB];
+      }
+      if (MODE == GS_DOT || MODE == GS_AXPY_DOT)
+         pdot_reg(b, a, false, &re, &im);
+      else if (MODE == GS_SCALE_DOT)
+         pdot_reg(a, b, false, &re, &im);
+      else
+         pdot_reg(a, a, true, &re, &im);
+   }
+   dd v[2] = {{re, 0.0}, {im, 0.0}};
+   block_reduce_dd<2>(v);
+   if (threadIdx.x == 0)
+   {
+      part[2 * blockIdx.x] = v[0];
+      part[2 * blockIdx.x + 1] = v[1];
+   }
+}
+
+// final sum; stores the float result in coef[io]: the complex product, or
+// for a norm (NORM=1) the value sqrt(float(|s|^2)) of normalize() in .x and
+// float(|s|^2) in .y
+template <int NORM>
+__global__ __launch_bounds__(RB) void k_gs_final(const dd *part, int nb, float2 *coef, int io)
+{
+   dd v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+   for (int k = threadIdx.x; k < nb; k += RB)
+   {
+      v[0] = dd_add(v[0], part[2 * k]);
+      v[1] = dd_add(v[1], part[2 * k + 1]);
+   }
+   block_reduce_dd<2>(v);
+   if (threadIdx.x == 0)
+   {
+      if (NORM)
+      {
+         float r = (float)(v[0].hi + v[0].lo);
+         coef[io] = make_float2((float)sqrt((double)r), r);
+      }
+      else
+         coef[io] = make_float2((float)(v[0].hi + v[0].lo), (float)(v[1].hi + v[1].lo));
+   }
+}
+
+template <int MODE>
+void gs_step(float2 *s, const float2 *r, const float2 *q, float2 *coef, int ic, int io, int n)
+{
+   Lib &L = lib();
+   if (!d_part) d_part = (dd *)dev_alloc(sizeof(dd) * 2 * (RMAXB + 8));
+   int nb = red_blocks(n);
+   k_gs_partial<MODE><<<nb, RB, 0, L.stream>>>(s, r, q, coef, ic, n, d_part);
+   if (MODE == GS_AXPY_NORM || MODE == GS_NORM)
+      k_gs_final<1><<<1, RB, 0, L.stream>>>(d_part, nb, coef, io);
+   else
+      k_gs_final<0><<<1, RB, 0, L.stream>>>(d_part, nb, coef, io);
+}
+
+template void gs_step<GS_DOT>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+template void gs_step<GS_AXPY_DOT>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+template void gs_step<GS_AXPY_NORM>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+template void gs_step<GS_SCALE_DOT>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+template void gs_step<GS_NORM>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 

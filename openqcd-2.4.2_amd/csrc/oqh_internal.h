@@ -145,6 +145,13 @@ void aos_to_blocks_f32(const float2 *aos, float2 *blk, int n, int nc, hipStream_
 void blocks_to_aos_f32(const float2 *blk, float2 *aos, int n, int nc, hipStream_t st);
 void convert_f64_to_f32(const double2 *a, float2 *b, size_t nunits, hipStream_t st);
 
+// fused Gram-Schmidt steps of fgcr on single-precision device fields of n
+// points (linalg.hip); coefficients in device slots coef[ic] (input) and
+// coef[io] (output)
+enum GsMode { GS_DOT = 0, GS_AXPY_DOT, GS_AXPY_NORM, GS_SCALE_DOT, GS_NORM };
+template <int MODE>
+void gs_step(float2 *s, const float2 *r, const float2 *q, float2 *coef, int ic, int io, int n);
+
 // SW field management (swterm.hip)
 void ensure_swd(int want_inv_e, int want_inv_o);
 int ensure_swd_ret(int want_inv_e, int want_inv_o);

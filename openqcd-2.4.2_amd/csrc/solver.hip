@@ -23,6 +23,27 @@ bool sap_prepare(int isolv, const char *where, bool quiet);
 
 using namespace oqh;
 
+static int dummy_off = 0;
+
+// device slots (and a pinned host mirror) for the coefficients of the fused
+// Gram-Schmidt steps
+static void *tmp_gs_coef(int n, float2 **host)
+{
+   static float2 *d = nullptr, *h = nullptr;
+   static int cap = 0;
+   if (n > cap)
+   {
+      dev_free(d);
+      if (h) (void)hipHostFree(h);
+      d = (float2 *)dev_alloc(sizeof(float2) * n);
+      h = nullptr;
+      if (hipHostMalloc((void **)&h, sizeof(float2) * n, 0) != hipSuccess) h = nullptr;
+      cap = (d && h) ? n : 0;
+   }
+   *host = h;
+   return cap ? d : nullptr;
+}
+
 static double get_res(int vol, int istop, spinor *rho)
 {
    if (istop) return (double)unorm(vol, 3, rho);
@@ -51,12 +72,68 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
    double rn = get_res(vol, istop, rho), tol = res * rn, rn_old;
    status[0] = 0;
 
+   // device path for the Gram-Schmidt steps: all Krylov vectors and rho are
+   // full device fields and the stopping norm is the 2-norm
+   Lib &L = lib();
+   int npts = (int)((long)vol * L.g.nthread);
+   bool fused = (istop == 0 && npts == L.g.V);
+   for (int l = 0; fused && l < 2 * nkv + 1; l++)
+   {
+      int off = 0;
+      fused = device_field(ws[l], F32, &off) && off == 0;
+   }
+   const int SB_ = nkv, SC_ = nkv + 1, SR_ = nkv + 2;
+   float2 *dcoef = nullptr, *hcoef = nullptr;
+   if (fused)
+   {
+      dcoef = (float2 *)tmp_gs_coef(nkv + 3, &hcoef);
+      fused = dcoef != nullptr;
+   }
+   auto F = [](spinor *p) { return (float2 *)device_field(p, F32, &dummy_off); };
+
    while (rn > tol)
    {
       int k;
       rn_old = rn;
       for (k = 0;; k++)
       {
+         if (fused)
+         {
+            // gcr_step with the coefficients kept on the device: one fused
+            // update-and-product pass per Gram-Schmidt term, one read-back
+            Mop(k, rho, phi[k], chi[k]);
+            float2 *ck = F(chi[k]);
+            if (k > 0)
+            {
+               gs_step<GS_DOT>(ck, nullptr, F(chi[0]), dcoef, 0, 0, npts);
+               for (int l = 0; l < k; l++)
+               {
+                  if (l + 1 < k)
+                     gs_step<GS_AXPY_DOT>(ck, F(chi[l]), F(chi[l + 1]), dcoef, l, l + 1, npts);
+                  else
+                     gs_step<GS_AXPY_NORM>(ck, F(chi[l]), nullptr, dcoef, l, SB_, npts);
+               }
+            }
+            else
+               gs_step<GS_NORM>(ck, nullptr, nullptr, dcoef, 0, SB_, npts);
+            gs_step<GS_SCALE_DOT>(ck, nullptr, F(rho), dcoef, SB_, SC_, npts);
+            gs_step<GS_AXPY_NORM>(F(rho), ck, nullptr, dcoef, SC_, SR_, npts);
+            HIP_OK(hipMemcpyAsync(hcoef, dcoef, sizeof(float2) * (nkv + 3), hipMemcpyDeviceToHost, L.stream));
+            HIP_OK(hipStreamSynchronize(L.stream));
+            for (int l = 0; l < k; l++)
+            {
+               a[(size_t)nkv * l + k].re = hcoef[l].x;
+               a[(size_t)nkv * l + k].im = hcoef[l].y;
+            }
+            b[k] = hcoef[SB_].x;
+            if (b[k] == 0.0f) fail("normalize [salg.c]", "Vector has vanishing norm");
+            c[k].re = hcoef[SC_].x;
+            c[k].im = hcoef[SC_].y;
+            rn = sqrt((double)hcoef[SR_].y);
+            status[0] += 1;
+            if (rn <= tol || rn < PRECISION_LIMIT * rn_old || k == nkv - 1 || status[0] >= nmx) break;
+            continue;
+         }
          // gcr_step (fgcr.c:157-178)
          Mop(k, rho, phi[k], chi[k]);
          for (int l = 0; l < k; l++)
