@@ -59,6 +59,8 @@ struct SapGrid
    float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
    __half2 *d_u16 = nullptr;             // FP16 links (FP16 variant of the solver)
    __half2 *d_swb16 = nullptr;           // FP16 block SW field
+   __half2 *d_h16[3] = {nullptr, nullptr, nullptr};  // FP16 psi, rho, s0g
+   float *d_hsc[3] = {nullptr, nullptr, nullptr};    // their per-site scales
    int u16_epoch = -1, swb16_epoch = -1; // versions they were made from
    int swb_version = 0;                  // incremented by every assign_swb
    std::vector<int> imb;                 // host copy
@@ -77,7 +79,38 @@ struct SArgs
    size_t Ub;
    float2 *psi, *rho, *s0g, *scr;
    int n0;             // first block of the colour
+   // FP16 variant: psi, rho and s0g as FP16 spinors with per-site scales
+   __half2 *psi16, *rho16, *s0g16;
+   float *psc, *rsc, *s0sc;
+   int pz;             // psi is zero on entry (not read)
 };
+
+// FP16 spinor with per-site scale: the 12 colour components of site x are
+// stored as v / m in FP16 (same unit layout as the FP32 fields), m = the
+// largest modulus of a real component of the spinor at x
+__device__ inline void load_h16(const __half2 *h, const float *sc, int x, float2 *v)
+{
+   const float m = sc[x];
+   const __half2 *p = h + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      float2 f = __half22float2(p[c * SB]);
+      v[c] = make_float2(f.x * m, f.y * m);
+   }
+}
+
+__device__ inline void store_h16(__half2 *h, float *sc, int x, const float2 *v)
+{
+   float m = 0.0f;
+#pragma unroll
+   for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
+   const float r = m > 0.0f ? 1.0f / m : 0.0f;
+   sc[x] = m;
+   __half2 *p = h + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+#pragma unroll
+   for (int c = 0; c < 12; c++) p[c * SB] = __float22half2_rn(make_float2(v[c].x * r, v[c].y * r));
+}
 
 // link of direction mu, forward (x->x+mu) or backward (x-mu->x), seen from
 // global point gx with global neighbour gy
@@ -618,7 +651,8 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
 {
    __shared__ float2 F[12][RBV];
    __shared__ float2 X[6][RBV];
-   // FP16 variant: the Pauli matrices of the block live in LDS
+   // FP16 variant: the Pauli matrices of the block live in LDS, and psi,
+   // rho, s0g are FP16 spinors with per-site scales
    constexpr bool MLDS = sizeof(S) == 4;
    __shared__ S MS[MLDS ? 36 : 1][RBV];
    const int x = threadIdx.x;
@@ -657,7 +691,10 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
    const S *mp = MLDS ? &MS[0][x] : b.m;
    const int mst = MLDS ? RBV : 1;
    float2 s0[12], s1[12], s2[12], acc[12];
-   load_spinor((const float2 *)a.rho, gx, s1);
+   if (MLDS)
+      load_h16(a.rho16, a.rsc, gx, s1);
+   else
+      load_spinor((const float2 *)a.rho, gx, s1);
 #pragma unroll
    for (int c = 0; c < 12; c++)
    {
@@ -758,10 +795,18 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
    }
 
    float2 p[12];
-   load_spinor((const float2 *)a.psi, gx, p);
+   if (a.pz)
+      zero12(p);
+   else if (MLDS)
+      load_h16(a.psi16, a.psc, gx, p);
+   else
+      load_spinor((const float2 *)a.psi, gx, p);
 #pragma unroll
    for (int c = 0; c < 12; c++) p[c] = cadd(p[c], s0[c]);
-   store_spinor(a.psi, gx, p);
+   if (MLDS)
+      store_h16(a.psi16, a.psc, gx, p);
+   else
+      store_spinor(a.psi, gx, p);
    if (ISOLV && !even)
    {
       // odd residual -i mu gamma5 s0 (update_flds1) or 0 (eoflg, update_flds2)
@@ -773,46 +818,83 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
          s1[c] = make_float2(-cm * s0[c].y, cm * s0[c].x);
       }
    }
-   store_spinor(a.rho, gx, s1);
-   store_spinor(a.s0g, gx, s0);
+   if (MLDS)
+   {
+      store_h16(a.rho16, a.rsc, gx, s1);
+      store_h16(a.s0g16, a.s0sc, gx, s0);
+   }
+   else
+   {
+      store_spinor(a.rho, gx, s1);
+      store_spinor(a.s0g, gx, s0);
+   }
 }
 
 // rho(y) -= hopping terms into y from the block corrections of the
 // neighbouring blocks, for the face points y of the blocks of one colour
-__global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, const unsigned char *fmask, int nf)
+// (S = __half2: FP16 links, FP16 s0g and rho with per-site scales)
+template <typename S>
+__global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, const unsigned char *fmask, int nf,
+                                                   const S *u16)
 {
+   constexpr bool H = sizeof(S) == 4;
    int k = blockIdx.x * 256 + threadIdx.x;
    if (k >= nf) return;
    int y = face[k];
    int m = fmask[k];
-   float2 acc[12], sp[12];
+   float2 acc[12], sp[12], u[9], h0[3], h1[3], c0[3], c1[3];
    zero12(acc);
    int odd = y >= a.h;
+   const S *ub = H ? u16 : (const S *)a.u;
+#define OQH_LDU(J, L)                                                                  \
+   {                                                                                  \
+      const S *up = ub + ((J) >> 6) * (size_t)(72 * SB) + ((J) & (SB - 1)) + (size_t)(9 * (L)) * SB; \
+      _Pragma("unroll") for (int q = 0; q < 9; q++) u[q] = to_f2(up[q * SB]);          \
+   }
 #define OQH_FACE(MU)                                                                   \
    if (m & (1 << (2 * MU)))                                                           \
    {                                                                                  \
       int x = a.nbr[(size_t)(2 * MU) * a.V + y] & 0x7fffffff;                         \
-      load_spinor((const float2 *)a.s0g, x, sp);                                      \
-      const float2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU) : link_ptr(a.u, x - a.h, 2 * MU + 1); \
-      hop_one<float2, MU, true>(sp, up, acc);                                         \
+      if (H)                                                                          \
+         load_h16(a.s0g16, a.s0sc, x, sp);                                            \
+      else                                                                            \
+         load_spinor((const float2 *)a.s0g, x, sp);                                   \
+      if (odd) OQH_LDU(y - a.h, 2 * MU) else OQH_LDU(x - a.h, 2 * MU + 1)               \
+      spin_project<float2, MU, true>(sp, h0, h1);                                     \
+      su3_mul<float2, false>(u, h0, c0);                                              \
+      su3_mul<float2, false>(u, h1, c1);                                              \
+      spin_expand_add<float2, MU, true>(c0, c1, acc);                                 \
    }                                                                                  \
    if (m & (2 << (2 * MU)))                                                           \
    {                                                                                  \
       int x = a.nbr[(size_t)(2 * MU + 1) * a.V + y] & 0x7fffffff;                     \
-      load_spinor((const float2 *)a.s0g, x, sp);                                      \
-      const float2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU + 1) : link_ptr(a.u, x - a.h, 2 * MU); \
-      hop_one<float2, MU, false>(sp, up, acc);                                        \
+      if (H)                                                                          \
+         load_h16(a.s0g16, a.s0sc, x, sp);                                            \
+      else                                                                            \
+         load_spinor((const float2 *)a.s0g, x, sp);                                   \
+      if (odd) OQH_LDU(y - a.h, 2 * MU + 1) else OQH_LDU(x - a.h, 2 * MU)               \
+      spin_project<float2, MU, false>(sp, h0, h1);                                    \
+      su3_mul<float2, true>(u, h0, c0);                                               \
+      su3_mul<float2, true>(u, h1, c1);                                               \
+      spin_expand_add<float2, MU, false>(c0, c1, acc);                                \
    }
    OQH_FACE(0)
    OQH_FACE(1)
    OQH_FACE(2)
    OQH_FACE(3)
 #undef OQH_FACE
+#undef OQH_LDU
    float2 r[12];
-   load_spinor((const float2 *)a.rho, y, r);
+   if (H)
+      load_h16(a.rho16, a.rsc, y, r);
+   else
+      load_spinor((const float2 *)a.rho, y, r);
 #pragma unroll
    for (int c = 0; c < 12; c++) r[c] = cadd(r[c], cscale(acc[c], 0.5f));
-   store_spinor(a.rho, y, r);
+   if (H)
+      store_h16(a.rho16, a.rsc, y, r);
+   else
+      store_spinor(a.rho, y, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -880,6 +962,7 @@ static void free_grid()
    for (int c = 0; c < 2; c++) { dev_free(G.d_face[c]); dev_free(G.d_fmask[c]); G.d_face[c] = nullptr; G.d_fmask[c] = nullptr; }
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
    dev_free(G.d_u16); dev_free(G.d_swb16);
+   for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); }
    G = SapGrid();
 }
 
@@ -1052,40 +1135,120 @@ static SArgs sargs(float mu, int nmr)
    a.s0g = G.d_s0g;
    a.scr = G.d_scr;
    a.n0 = 0;
+   a.psi16 = G.d_h16[0];
+   a.rho16 = G.d_h16[1];
+   a.s0g16 = G.d_h16[2];
+   a.psc = G.d_hsc[0];
+   a.rsc = G.d_hsc[1];
+   a.s0sc = G.d_hsc[2];
+   a.pz = 0;
    return a;
 }
 
-// one SAP cycle on device fields (no synchronisation)
+static bool half_variant()
+{
+   return OQH_SAP_REG && G.bvol == RBV && lib().sap_prec == 1 && G.d_h16[0] && G.d_h16[1] && G.d_h16[2] && G.d_u16 &&
+          G.d_swb16;
+}
+
+// one half-sweep over the blocks of colour ic followed by the face update of
+// the other colour's blocks
+static void sap_colour(const SArgs &a0, int ic, int isolv)
+{
+   Lib &L = lib();
+   SArgs a = a0;
+   a.n0 = ic ? G.nbh : 0;
+   const bool half = half_variant();
+   if (half)
+   {
+      if (isolv)
+         k_sap_reg<1, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+      else
+         k_sap_reg<0, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+   }
+   else if (OQH_SAP_REG && G.bvol == RBV)
+   {
+      if (isolv)
+         k_sap_reg<1, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else
+         k_sap_reg<0, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+   }
+   else if (isolv)
+      k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
+   else
+      k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
+   int c = 1 - ic;
+   if (!G.nface[c]) return;
+   unsigned nbk = (G.nface[c] + 255) / 256;
+   if (half)
+      k_sap_faces<__half2><<<nbk, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], G.d_u16);
+   else
+      k_sap_faces<float2><<<nbk, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], nullptr);
+}
+
+// FP32 field <-> FP16 field with per-site scales
+__global__ void k_s32_to_h16(const float2 *__restrict__ f, __half2 *h, float *sc, int V)
+{
+   int x = blockIdx.x * 256 + threadIdx.x;
+   if (x >= V) return;
+   float2 v[12];
+   load_spinor(f, x, v);
+   store_h16(h, sc, x, v);
+}
+
+__global__ void k_h16_to_s32(const __half2 *h, const float *sc, float2 *__restrict__ f, int V)
+{
+   int x = blockIdx.x * 256 + threadIdx.x;
+   if (x >= V) return;
+   float2 v[12];
+   load_h16(h, sc, x, v);
+   store_spinor(f, x, v);
+}
+
+// one SAP cycle on device fields (no synchronisation); the FP16 variant
+// converts psi and rho to FP16 spinors and back
 void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
 {
    Lib &L = lib();
    SArgs a = sargs(mu, nmr);
    a.psi = psi;
    a.rho = rho;
-   for (int ic = 0; ic < 2; ic++)
+   const int V = L.g.V;
+   const unsigned nbk = (V + 255) / 256;
+   if (half_variant())
    {
-      a.n0 = ic ? G.nbh : 0;
-      if (OQH_SAP_REG && G.bvol == RBV && L.sap_prec == 1)
-      {
-         if (isolv)
-            k_sap_reg<1, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
-         else
-            k_sap_reg<0, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
-      }
-      else if (OQH_SAP_REG && G.bvol == RBV)
-      {
-         if (isolv)
-            k_sap_reg<1, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
-         else
-            k_sap_reg<0, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
-      }
-      else if (isolv)
-         k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
-      else
-         k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
-      int c = 1 - ic;
-      if (G.nface[c]) k_sap_faces<<<(G.nface[c] + 255) / 256, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c]);
+      k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(psi, G.d_h16[0], G.d_hsc[0], V);
+      k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(rho, G.d_h16[1], G.d_hsc[1], V);
    }
+   for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+   if (half_variant())
+   {
+      k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], psi, V);
+      k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[1], G.d_hsc[1], rho, V);
+   }
+}
+
+// the SAP preconditioner of sap_gcr in the FP16 variant: ncy cycles on FP16
+// fields starting from psi = 0, rho = rho_in, then phi = psi (sap_gcr.c:99-117
+// with the fields kept in FP16).  The residual carried in FP16 no longer
+// equals rho_in - D psi to FP32 accuracy, so the caller computes chi = D phi
+// with the FP32 operator instead of taking rho_in - rho.
+bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi)
+{
+   if (!half_variant()) return false;
+   Lib &L = lib();
+   const int V = L.g.V;
+   const unsigned nbk = (V + 255) / 256;
+   SArgs a = sargs(mu, nmr);
+   k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(rho, G.d_h16[1], G.d_hsc[1], V);
+   for (int n = 0; n < ncy; n++)
+   {
+      // the blocks of both colours see psi = 0 in the first cycle
+      a.pz = (n == 0);
+      for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+   }
+   k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], phi, V);
+   return true;
 }
 
 __global__ void k_f32_to_f16(const float2 *__restrict__ a, __half2 *__restrict__ b, size_t n)
@@ -1101,6 +1264,12 @@ static void ensure_half_fields()
    size_t nu = units_for(L.g.V / 2, 72), ns = units_for(L.g.V, 36);
    if (!G.d_u16) G.d_u16 = (__half2 *)dev_alloc(nu * sizeof(__half2));
    if (!G.d_swb16) G.d_swb16 = (__half2 *)dev_alloc(ns * sizeof(__half2));
+   for (int k = 0; k < 3; k++)
+   {
+      if (!G.d_h16[k]) G.d_h16[k] = (__half2 *)dev_alloc(units_for(L.g.V, 12) * sizeof(__half2));
+      if (!G.d_hsc[k]) G.d_hsc[k] = (float *)dev_alloc((size_t)L.g.V * sizeof(float));
+      if (!G.d_h16[k] || !G.d_hsc[k]) return;
+   }
    if (!G.d_u16 || !G.d_swb16) return;
    if (G.u16_epoch != L.u_epoch)
    {

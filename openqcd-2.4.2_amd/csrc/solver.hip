@@ -19,6 +19,7 @@
 
 namespace oqh {
 bool sap_prepare(int isolv, const char *where, bool quiet);
+bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi);
 }
 
 using namespace oqh;
@@ -195,6 +196,18 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
    (void)k;
    Lib &L = lib();
    int vt = L.g.VT;
+   if (L.sap_prec == 1)
+   {
+      // FP16 variant: the whole preconditioner on FP16 fields, chi = D phi
+      // with the FP32 operator
+      int o0 = 0, o1 = 0;
+      float2 *r = (float2 *)device_field(rho, F32, &o0), *p = (float2 *)device_field(phi, F32, &o1);
+      if (r && p && !o0 && !o1 && sap_mixed_mop((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p))
+      {
+         Dw((float)mud, phi, chi);
+         return;
+      }
+   }
    set_s2zero(vt, 2, phi);
    assign_s2s(vt, 2, rho, chi);
    for (int n = 0; n < L.sap_ncy; n++) sap((float)mud, L.sap_isolv, L.sap_nmr, phi, chi);

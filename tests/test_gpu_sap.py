@@ -157,10 +157,12 @@ def test_sap_gcr_failure_modes(setup):
 
 @pytest.mark.gpu
 def test_fp16_storage_variant(setup):
-    """The FP16-storage block solver (links and SW matrices in FP16, FP32
-    arithmetic) approximates the FP32 Schwarz cycle to the FP16 rounding
-    level, and SAP+GCR with it still reaches the requested double-precision
-    residual (the outer FGCR corrects the preconditioner's rounding)."""
+    """The FP16-storage Schwarz cycle (links, SW matrices and the spinor
+    fields psi, rho, s0g in FP16, the spinors with per-site scales; FP32
+    arithmetic) approximates the FP32 cycle to the FP16 rounding level, and
+    SAP+GCR with it, where the whole preconditioner runs on FP16 fields,
+    still reaches the requested double-precision residual (the outer FGCR
+    corrects the preconditioner's rounding)."""
     c, o, eta, s32, lat = setup
     lib = lat.lib
     out = {}
@@ -185,3 +187,38 @@ def test_fp16_storage_variant(setup):
     print("fp16 SAP+GCR status", status.value, "true residual", res)
     assert ifail.value == 0 and res <= 1.2e-10
     lib.oqh_set_sap_prec(0)
+
+
+@pytest.mark.gpu
+def test_fp16_spinor_format(setup):
+    """With nmr=0 a Schwarz cycle leaves psi as it is (with isolv=1 at the
+    even points: the odd ones receive A_oo^-1 rho), and with isolv=0 also
+    rho (apart from the open-bc boundary points), so in the FP16 variant it
+    returns them after one trip through the FP16 format: per site, every real
+    component is within 2^-12 of the site's largest one (FP16 rounding of
+    values scaled into [-1, 1]), up to the FP32 rounding of the scaling
+    (a few units of 2^-24 of the site maximum)."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    rng = np.random.default_rng(11)
+    # components spread over six decades within a site
+    psi0 = (s32 * np.exp(rng.uniform(-7, 0, s32.shape))).astype(np.float32)
+    out = {}
+    for prec in (0, 1):
+        lib.oqh_set_sap_prec(prec)
+        psi, rho = lat.alloc_s(), lat.alloc_s()
+        lat.put_s(psi, psi0)
+        lat.put_s(rho, s32)
+        lib.sap(c["mu"], c["isolv"], 0, psi, rho)
+        out[prec] = (lat.get_s(psi), lat.get_s(rho))
+        lat.free(psi)
+        lat.free(rho)
+    lib.oqh_set_sap_prec(0)
+    n = lat.V // 2 if c["isolv"] else lat.V
+    for k in ((0,) if c["isolv"] else (0, 1)):
+        ref = out[0][k][:n].astype(np.float64)
+        m = np.abs(ref).max(axis=1, keepdims=True)
+        dev = np.abs(out[1][k][:n] - ref)
+        print("field", k, "max deviation / site max", (dev / np.maximum(m, 1e-300)).max())
+        assert np.all(dev <= (2.0 ** -12 + 2.0 ** -22) * m)
+    assert np.array_equal(out[0][0][:n], psi0[:n])
