@@ -80,10 +80,20 @@ void oqh_set_sw_parms(double m0, double csw);
 void oqh_set_tm_parms(int eoflg);
 void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
 /* storage precision of the SAP block solver (sap, sap_gcr; no counterpart in
-   the reference): 0 = FP32 links and SW matrices as in sap/sap.c (default),
-   1 = FP16 links and SW matrices with FP32 arithmetic (blocks of 256 points;
-   other block sizes use FP32).  The outer FGCR keeps its precision. */
+   the reference): 0 = FP32 fields as in sap/sap.c (default), 1 = FP16 links,
+   SW matrices and spinor fields (psi, rho and the block corrections, with a
+   per-site scale), FP32 arithmetic, for blocks of 256 points (other block
+   sizes use FP32).  With 1, sap_gcr runs its ncy Schwarz cycles on FP16
+   fields and computes chi = Dw(phi) in FP32; the outer FGCR keeps its
+   precision. */
 void oqh_set_sap_prec(int prec);
+/* orthogonalisation of the Krylov vectors in fgcr on device fields (no
+   counterpart in the reference): 1 = modified Gram-Schmidt in its inverse
+   compact WY form, one pass for all scalar products (with the Gram entries
+   of the newest vector) and one for all updates (default); 0 = one pass per
+   term in the order of linsolv/fgcr.c:157-178.  Both give the coefficients
+   of the modified process in exact arithmetic. */
+void oqh_set_gcr_orth(int mode);
 
 /* gauge field in the layout of udfld() (uflds/uflds.c:113-156; 8 links per
    odd point, README.global).  Import invalidates the SW term and the

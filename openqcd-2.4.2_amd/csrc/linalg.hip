@@ -14,6 +14,7 @@
 // products follow salg.c: per point the colour-vector products are formed in
 // float, converted to double and summed in double; the result is rounded to
 // float.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -465,6 +466,264 @@ template void gs_step<GS_AXPY_DOT>(float2 *, const float2 *, const float2 *, flo
 template void gs_step<GS_AXPY_NORM>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
 template void gs_step<GS_SCALE_DOT>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
 template void gs_step<GS_NORM>(float2 *, const float2 *, const float2 *, float2 *, int, int, int);
+
+// ---------------------------------------------------------------------------
+// Gram-Schmidt step of fgcr with two passes over the Krylov vectors (the
+// inverse compact WY form of modified Gram-Schmidt, Swirydowicz et al.,
+// Numer. Linear Algebra Appl. 28 (2021) e2343).  The coefficients of the
+// modified process of fgcr.c:157-178,
+//    d_i = (chi_i, s - sum_{l<i} d_l chi_l) = p_i - sum_{l<i} G_il d_l,
+// with p_i = (chi_i, s) and G_il = (chi_i, chi_l), follow from one pass
+// that forms all p_i together with the Gram entries of the newest Krylov
+// vector, and a triangular solve of size k; one more pass applies all
+// updates.  Unlike classical Gram-Schmidt the non-orthogonality of the
+// stored FP32 vectors is corrected as in the modified process.
+// ---------------------------------------------------------------------------
+
+constexpr int NQD = 12;  // vectors per product pass
+constexpr int NQA = 32;  // vectors per update pass
+
+struct QList
+{
+   const float2 *q[NQA];
+};
+
+static dd *d_mpart = nullptr;
+static float2 *d_gram = nullptr;
+static int gram_ld = 0;
+
+// partial sums of (q_j, s) and, if c != null, (q_j, c), j < nq <= NQD;
+// part[blockIdx][4 j + (0,1: with s; 2,3: with c)]
+__global__ __launch_bounds__(RB) void k_mdot_partial(const float2 *__restrict__ s, const float2 *__restrict__ c,
+                                                     QList ql, int nq, int n, dd *part)
+{
+   double re[NQD], im[NQD], gr[NQD], gi[NQD];
+#pragma unroll
+   for (int j = 0; j < NQD; j++) re[j] = im[j] = gr[j] = gi[j] = 0.0;
+   int stride = gridDim.x * RB;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+   {
+      size_t i0 = (size_t)(k >> 6) * (12 * SB) + (k & (SB - 1));
+      float2 a[12], g[12];
+#pragma unroll
+      for (int u = 0; u < 12; u++)
+      {
+         a[u] = s[i0 + u * SB];
+         g[u] = c ? c[i0 + u * SB] : make_float2(0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int j = 0; j < NQD; j++)
+      {
+         if (j < nq)
+         {
+            float2 b[12];
+            const float2 *q = ql.q[j];
+#pragma unroll
+            for (int u = 0; u < 12; u++) b[u] = q[i0 + u * SB];
+            pdot_reg(b, a, false, &re[j], &im[j]);
+            if (c) pdot_reg(b, g, false, &gr[j], &gi[j]);
+         }
+      }
+   }
+   // work-group sums, one value at a time (few registers in flight)
+   __shared__ dd sh[RB / 64][4 * NQD];
+   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+   for (int j = 0; j < NQD; j++)
+   {
+      const double src[4] = {re[j], im[j], gr[j], gi[j]};
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+      {
+         dd x = {src[u], 0.0};
+         for (int m = 32; m >= 1; m >>= 1) x = dd_add(x, shfl_xor_dd(x, m));
+         if (l == 0) sh[w][4 * j + u] = x;
+      }
+   }
+   __syncthreads();
+   const int t = threadIdx.x;
+   if (t < 4 * nq)
+   {
+      dd x = sh[0][t];
+      for (int ww = 1; ww < RB / 64; ww++) x = dd_add(x, sh[ww][t]);
+      part[(size_t)blockIdx.x * 4 * NQD + t] = x;
+   }
+}
+
+// one work-group per vector j of the pass: p_j = coef[j0 + j]; the Gram
+// entry G_{kn,j} = conj((q_j, q_kn)) of the newest vector kn goes to
+// gram[kn * ld + j0 + j] if j0 + j < kn
+__global__ __launch_bounds__(RB) void k_mdot_final(const dd *part, int nb, float2 *coef, int j0, float2 *gram, int ld,
+                                                   int kn)
+{
+   const int j = blockIdx.x;
+   dd v[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+   for (int k = threadIdx.x; k < nb; k += RB)
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = dd_add(v[u], part[(size_t)k * 4 * NQD + 4 * j + u]);
+   block_reduce_dd<4>(v);
+   if (threadIdx.x == 0)
+   {
+      coef[j0 + j] = make_float2((float)(v[0].hi + v[0].lo), (float)(v[1].hi + v[1].lo));
+      if (kn > 0 && j0 + j < kn)
+         gram[(size_t)kn * ld + j0 + j] = make_float2((float)(v[2].hi + v[2].lo), -(float)(v[3].hi + v[3].lo));
+   }
+}
+
+// d_i = p_i - sum_{l<i} G_il d_l (in place in coef[0..k-1])
+__global__ void k_gram_solve(float2 *coef, const float2 *gram, int ld, int k)
+{
+   if (threadIdx.x != 0) return;
+   for (int i = 1; i < k; i++)
+   {
+      double zr = coef[i].x, zi = coef[i].y;
+      for (int l = 0; l < i; l++)
+      {
+         float2 g = gram[(size_t)i * ld + l], d = coef[l];
+         zr -= (double)g.x * d.x - (double)g.y * d.y;
+         zi -= (double)g.x * d.y + (double)g.y * d.x;
+      }
+      coef[i] = make_float2((float)zr, (float)zi);
+   }
+}
+
+// s -= sum_j coef[j0 + j] q_j (j < nq); in the last pass (fin) also the
+// partial sums of |s|^2 and (s, r)
+__global__ __launch_bounds__(RB) void k_maxpy_partial(float2 *s, QList ql, int nq, const float2 *coef, int j0,
+                                                      const float2 *__restrict__ r, int fin, int n, dd *part)
+{
+   __shared__ float2 z[NQA];
+   if (threadIdx.x < nq) z[threadIdx.x] = coef[j0 + threadIdx.x];
+   __syncthreads();
+   double nr = 0.0, pr = 0.0, pi = 0.0;
+   int stride = gridDim.x * RB;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+   {
+      size_t i0 = (size_t)(k >> 6) * (12 * SB) + (k & (SB - 1));
+      float2 a[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) a[c] = s[i0 + c * SB];
+      for (int j = 0; j < nq; j++)
+      {
+         const float2 *q = ql.q[j];
+         const float2 zj = z[j];
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+         {
+            float2 w = q[i0 + c * SB];
+            a[c] = make_float2(a[c].x - (zj.x * w.x - zj.y * w.y), a[c].y - (zj.x * w.y + zj.y * w.x));
+         }
+      }
+      if (nq)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++) s[i0 + c * SB] = a[c];
+      }
+      if (fin)
+      {
+         float2 b[12];
+#pragma unroll
+         for (int c = 0; c < 12; c++) b[c] = r[i0 + c * SB];
+         double ni = 0.0;
+         pdot_reg(a, a, true, &nr, &ni);
+         pdot_reg(a, b, false, &pr, &pi);
+      }
+   }
+   if (!fin) return;
+   dd v[3] = {{nr, 0.0}, {pr, 0.0}, {pi, 0.0}};
+   block_reduce_dd<3>(v);
+   if (threadIdx.x == 0)
+      for (int j = 0; j < 3; j++) part[(size_t)blockIdx.x * 3 + j] = v[j];
+}
+
+// coef[ib] = (|s|, |s|^2) as normalize() (salg.c), coef[ic] = (s, r) / |s|
+__global__ __launch_bounds__(RB) void k_maxpy_final(const dd *part, int nb, float2 *coef, int ib, int ic)
+{
+   dd v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+   for (int k = threadIdx.x; k < nb; k += RB)
+#pragma unroll
+      for (int j = 0; j < 3; j++) v[j] = dd_add(v[j], part[(size_t)k * 3 + j]);
+   block_reduce_dd<3>(v);
+   if (threadIdx.x == 0)
+   {
+      float r = (float)(v[0].hi + v[0].lo);
+      float b = (float)sqrt((double)r);
+      coef[ib] = make_float2(b, r);
+      float rb = b > 0.0f ? 1.0f / b : 0.0f;
+      coef[ic] = make_float2((float)(v[1].hi + v[1].lo) * rb, (float)(v[2].hi + v[2].lo) * rb);
+   }
+}
+
+// s *= 1 / coef[ib].x, r -= coef[ic] s, partial sums of |r|^2
+__global__ __launch_bounds__(RB) void k_scale_axpy_partial(float2 *s, float2 *r, const float2 *coef, int ib, int ic,
+                                                           int n, dd *part)
+{
+   const float sc = 1.0f / coef[ib].x;
+   const float zr = -coef[ic].x, zi = -coef[ic].y;
+   double nr = 0.0, ni = 0.0;
+   int stride = gridDim.x * RB;
+   for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += stride)
+   {
+      size_t i0 = (size_t)(k >> 6) * (12 * SB) + (k & (SB - 1));
+      float2 a[12], b[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++)
+      {
+         float2 u = s[i0 + c * SB];
+         a[c] = make_float2(sc * u.x, sc * u.y);
+         s[i0 + c * SB] = a[c];
+         float2 w = r[i0 + c * SB];
+         b[c] = make_float2(w.x + (zr * a[c].x - zi * a[c].y), w.y + (zr * a[c].y + zi * a[c].x));
+         r[i0 + c * SB] = b[c];
+      }
+      pdot_reg(b, b, true, &nr, &ni);
+   }
+   dd v[2] = {{nr, 0.0}, {0.0, 0.0}};
+   block_reduce_dd<2>(v);
+   if (threadIdx.x == 0)
+   {
+      part[2 * blockIdx.x] = v[0];
+      part[2 * blockIdx.x + 1] = v[1];
+   }
+}
+
+void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n)
+{
+   Lib &L = lib();
+   if (!d_part) d_part = (dd *)dev_alloc(sizeof(dd) * 2 * (RMAXB + 8));
+   if (!d_mpart) d_mpart = (dd *)dev_alloc(sizeof(dd) * 4 * NQD * (RMAXB + 8));
+   if (k + 1 > gram_ld)
+   {
+      dev_free(d_gram);
+      gram_ld = std::max(k + 1, 64);
+      d_gram = (float2 *)dev_alloc(sizeof(float2) * gram_ld * gram_ld);
+   }
+   if (!d_part || !d_mpart || !d_gram) return;
+   const int nb = red_blocks(n);
+   // products with s and the Gram row of the newest vector q[k-1]
+   for (int j0 = 0; j0 < k; j0 += NQD)
+   {
+      QList ql = {};
+      int nq = std::min(NQD, k - j0);
+      for (int j = 0; j < nq; j++) ql.q[j] = q[j0 + j];
+      const float2 *c = (k >= 2 && j0 < k - 1) ? q[k - 1] : nullptr;
+      k_mdot_partial<<<nb, RB, 0, L.stream>>>(s, c, ql, nq, n, d_mpart);
+      k_mdot_final<<<nq, RB, 0, L.stream>>>(d_mpart, nb, coef, j0, d_gram, gram_ld, c ? k - 1 : 0);
+   }
+   if (k > 1) k_gram_solve<<<1, 64, 0, L.stream>>>(coef, d_gram, gram_ld, k);
+   for (int j0 = 0;; j0 += NQA)
+   {
+      QList ql = {};
+      int nq = std::max(0, std::min(NQA, k - j0));
+      for (int j = 0; j < nq; j++) ql.q[j] = q[j0 + j];
+      int fin = (j0 + NQA >= k);
+      k_maxpy_partial<<<nb, RB, 0, L.stream>>>(s, ql, nq, coef, j0, rho, fin, n, d_mpart);
+      if (fin) break;
+   }
+   k_maxpy_final<<<1, RB, 0, L.stream>>>(d_mpart, nb, coef, ib, ic);
+   k_scale_axpy_partial<<<nb, RB, 0, L.stream>>>(s, rho, coef, ib, ic, n, d_part);
+   k_gs_final<1><<<1, RB, 0, L.stream>>>(d_part, nb, coef, ir);
+}
 
 // ---------------------------------------------------------------------------
 // host helpers

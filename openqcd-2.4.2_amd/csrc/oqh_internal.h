@@ -67,7 +67,8 @@ struct Lib
    double m0 = 0.0, csw = 0.0;
    int eoflg = 0;
    int sap_bs[4] = {0, 0, 0, 0}, sap_isolv = 0, sap_nmr = 0, sap_ncy = 0;
-   int sap_prec = 0;  // storage of links/SW in the SAP block solver: 0 FP32, 1 FP16
+   int sap_prec = 0;  // SAP storage: 0 FP32, 1 FP16 links, SW and spinors (per-site scales)
+   int gcr_cgs = 1;   // fgcr orthogonalisation on the device: 1 two-pass (compact WY) MGS, 0 per-term MGS
 
    // geometry tables on the device
    int *d_nbr = nullptr;            // [8][V]: x+mu at 2mu, x-mu at 2mu+1; bit 31: boundary point
@@ -151,6 +152,12 @@ void convert_f64_to_f32(const double2 *a, float2 *b, size_t nunits, hipStream_t 
 enum GsMode { GS_DOT = 0, GS_AXPY_DOT, GS_AXPY_NORM, GS_SCALE_DOT, GS_NORM };
 template <int MODE>
 void gs_step(float2 *s, const float2 *r, const float2 *q, float2 *coef, int ic, int io, int n);
+// Gram-Schmidt step of fgcr in two passes (compact WY form of modified GS):
+// coef[j] = d_j (the modified-GS coefficients) for j < k, s -= sum_j d_j q_j,
+// coef[ib] = (|s|, |s|^2), s /= |s|, coef[ic] = (s, rho), rho -= coef[ic] s,
+// coef[ir] = (|rho|, |rho|^2); the Gram entries of q[k-1] are formed on the
+// way (q[0..k-2] must be the vectors of the previous calls)
+void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n);
 
 // SW field management (swterm.hip)
 void ensure_swd(int want_inv_e, int want_inv_o);

@@ -91,6 +91,7 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
       fused = dcoef != nullptr;
    }
    auto F = [](spinor *p) { return (float2 *)device_field(p, F32, &dummy_off); };
+   std::vector<float2 *> qv(nkv);
 
    while (rn > tol)
    {
@@ -104,7 +105,12 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
             // update-and-product pass per Gram-Schmidt term, one read-back
             Mop(k, rho, phi[k], chi[k]);
             float2 *ck = F(chi[k]);
-            if (k > 0)
+            if (L.gcr_cgs)
+            {
+               for (int l = 0; l < k; l++) qv[l] = F(chi[l]);
+               gs_classical(ck, qv.data(), k, F(rho), dcoef, SB_, SC_, SR_, npts);
+            }
+            else if (k > 0)
             {
                gs_step<GS_DOT>(ck, nullptr, F(chi[0]), dcoef, 0, 0, npts);
                for (int l = 0; l < k; l++)
@@ -117,8 +123,11 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
             }
             else
                gs_step<GS_NORM>(ck, nullptr, nullptr, dcoef, 0, SB_, npts);
-            gs_step<GS_SCALE_DOT>(ck, nullptr, F(rho), dcoef, SB_, SC_, npts);
-            gs_step<GS_AXPY_NORM>(F(rho), ck, nullptr, dcoef, SC_, SR_, npts);
+            if (!L.gcr_cgs)
+            {
+               gs_step<GS_SCALE_DOT>(ck, nullptr, F(rho), dcoef, SB_, SC_, npts);
+               gs_step<GS_AXPY_NORM>(F(rho), ck, nullptr, dcoef, SC_, SR_, npts);
+            }
             HIP_OK(hipMemcpyAsync(hcoef, dcoef, sizeof(float2) * (nkv + 3), hipMemcpyDeviceToHost, L.stream));
             HIP_OK(hipStreamSynchronize(L.stream));
             for (int l = 0; l < k; l++)
@@ -212,6 +221,16 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
    assign_s2s(vt, 2, rho, chi);
    for (int n = 0; n < L.sap_ncy; n++) sap((float)mud, L.sap_isolv, L.sap_nmr, phi, chi);
    diff_s2s(vt, 2, rho, chi);
+}
+
+extern "C" void oqh_set_gcr_orth(int mode)
+{
+   if (mode != 0 && mode != 1)
+   {
+      fail("oqh_set_gcr_orth", "the mode must be 0 (per-term) or 1 (two-pass modified Gram-Schmidt)");
+      return;
+   }
+   lib().gcr_cgs = mode;
 }
 
 extern "C" double sap_gcr(int nkv, int nmx, int istop, double res, double mu, spinor_dble *eta,

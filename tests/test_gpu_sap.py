@@ -122,7 +122,11 @@ def test_sap_gcr_solves(setup):
     e, p = lat.alloc_sd(), lat.alloc_sd()
     lat.put_sd(e, eta)
     ifail, status = C.c_int(), C.c_int()
+    # modified Gram-Schmidt term by term, the order of fgcr.c, for the
+    # comparison of the Krylov vector count with the oracle
+    lat.lib.oqh_set_gcr_orth(0)
     rho = lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    lat.lib.oqh_set_gcr_orth(1)
     psi = lat.get_sd(p)
     o.sw_term(3)
     res = np.linalg.norm(eta - o.Dw_dble(c["mu"], psi.copy())) / np.linalg.norm(eta)
@@ -132,6 +136,15 @@ def test_sap_gcr_solves(setup):
     assert res <= 1.2e-10
     assert abs(status.value - status_o) <= 1
     assert np.array_equal(lat.get_sd(e), eta), "source must be preserved"
+    # two-pass form of modified Gram-Schmidt (the default): same tolerance,
+    # nearly the same count
+    p2 = lat.alloc_sd()
+    st2 = C.c_int()
+    lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p2, C.byref(ifail), C.byref(st2))
+    res2 = np.linalg.norm(eta - o.Dw_dble(c["mu"], lat.get_sd(p2).copy())) / np.linalg.norm(eta)
+    print("two-pass GS status", st2.value, "true residual", res2)
+    assert ifail.value == 0 and res2 <= 1.2e-10 and abs(st2.value - status_o) <= 2
+    lat.free(p2)
     # psi == eta (in place), as allowed by sap_gcr
     rho2 = lat.lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, e, C.byref(ifail), C.byref(status))
     assert ifail.value == 0
