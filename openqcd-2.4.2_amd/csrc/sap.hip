@@ -485,6 +485,29 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgs a)
 
 constexpr int RBV = 256;
 
+// complex multiply-accumulate as FMA chains: acc + a b, acc + conj(a) b
+__device__ inline float2 cmac(float2 acc, float2 a, float2 b)
+{
+   return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y)));
+}
+__device__ inline float2 cmacc(float2 acc, float2 a, float2 b)
+{
+   return make_float2(fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(-a.y, b.x, acc.y)));
+}
+
+// c = U h or U^dag h (U in registers) with FMA chains
+template <bool DAG> __device__ inline void su3_mul_f(const float2 *u, const float2 *h, float2 *c)
+{
+#pragma unroll
+   for (int i = 0; i < 3; i++)
+   {
+      if (!DAG)
+         c[i] = cmac(cmac(cmul(u[3 * i], h[0]), u[3 * i + 1], h[1]), u[3 * i + 2], h[2]);
+      else
+         c[i] = cmacc(cmacc(cmulc(u[i], h[0]), u[3 + i], h[1]), u[6 + i], h[2]);
+   }
+}
+
 // storage of links and SW matrices in the block solver: float2, or __half2
 // in the FP16 variant (arithmetic in FP32 either way)
 __device__ inline float2 to_f2(float2 v) { return v; }
@@ -525,8 +548,8 @@ __device__ inline void pauli_reg(const S *m, float mu, const float2 *v, float2 *
          for (int j = i + 1; j < 6; j++)
          {
             float2 hij = q[p];
-            o[i] = cadd(o[i], cmul(hij, w[j]));
-            o[j] = cadd(o[j], cmulc(hij, w[i]));
+            o[i] = cmac(o[i], hij, w[j]);
+            o[j] = cmacc(o[j], hij, w[i]);
             p++;
          }
    }
@@ -555,16 +578,16 @@ __device__ inline void rb_hop_mu(const RegBlock<S> &b, bool odd, const float2 *v
 #pragma unroll
          for (int c = 0; c < 12; c++) sp[c] = F[c][y];
          spin_project<float2, MU, true>(sp, h0, h1);
-         su3_mul<float2, false>(u, h0, c0);
-         su3_mul<float2, false>(u, h1, c1);
+         su3_mul_f<false>(u, h0, c0);
+         su3_mul_f<false>(u, h1, c1);
          spin_expand_add<float2, MU, true>(c0, c1, acc);
       }
       if (other)
       {
          // backward term of y from x through the same link
          spin_project<float2, MU, false>(v, h0, h1);
-         su3_mul<float2, true>(u, h0, c0);
-         su3_mul<float2, true>(u, h1, c1);
+         su3_mul_f<true>(u, h0, c0);
+         su3_mul_f<true>(u, h1, c1);
 #pragma unroll
          for (int c = 0; c < 3; c++)
          {
@@ -590,25 +613,112 @@ __device__ inline void rb_hop_mu(const RegBlock<S> &b, bool odd, const float2 *v
 
 // acc = sum of the hopping terms of the field v over in-block neighbours
 // (at the points of parity OUT); values at boundary points of open bc count
-// as zero (blk_hop skips them)
+// as zero (blk_hop skips them).  lb: LDS of 18 * RBV float2.
 template <int OUT, typename S>
-__device__ inline void rb_hop(const RegBlock<S> &b, bool odd, const float2 *vin, float2 *acc, float2 (*F)[RBV],
-                              float2 (*X)[RBV])
+__device__ inline void rb_hop(const RegBlock<S> &b, bool odd, const float2 *vin, float2 *acc, float2 *lb)
 {
    const int x = threadIdx.x;
    float2 v[12];
+   if (OUT == 2)
+   {
+      float2(*F)[RBV] = (float2(*)[RBV])lb;
+      float2(*X)[RBV] = (float2(*)[RBV])(lb + 12 * RBV);
+#pragma unroll
+      for (int c = 0; c < 12; c++)
+      {
+         v[c] = b.bp ? make_float2(0.0f, 0.0f) : vin[c];
+         F[c][x] = v[c];
+         acc[c] = make_float2(0.0f, 0.0f);
+      }
+      __syncthreads();
+      rb_hop_mu<0, OUT>(b, odd, v, acc, F, X);
+      rb_hop_mu<1, OUT>(b, odd, v, acc, F, X);
+      rb_hop_mu<2, OUT>(b, odd, v, acc, F, X);
+      rb_hop_mu<3, OUT>(b, odd, v, acc, F, X);
+      return;
+   }
+   // one parity: the source lanes (the other parity) store their values
+   // (F, indexed by the point within its parity), compute the backward
+   // terms of all four directions with their own links and pass them on
+   // (X[mu]), while the output lanes compute their forward terms from F;
+   // two barriers per pass.  The output lanes add the forward terms of
+   // mu = 0..3, then the backward ones.
+   constexpr int RBH = RBV / 2;
+   float2(*F)[RBH] = (float2(*)[RBH])lb;
+   float2(*X)[6][RBH] = (float2(*)[6][RBH])(lb + 12 * RBH);
+   const bool own = (OUT == (int)odd);
 #pragma unroll
    for (int c = 0; c < 12; c++)
    {
       v[c] = b.bp ? make_float2(0.0f, 0.0f) : vin[c];
-      if (OUT == 2 || OUT != (int)odd) F[c][x] = v[c];
+      if (!own) F[c][x & (RBH - 1)] = v[c];
       acc[c] = make_float2(0.0f, 0.0f);
    }
    __syncthreads();
-   rb_hop_mu<0, OUT>(b, odd, v, acc, F, X);
-   rb_hop_mu<1, OUT>(b, odd, v, acc, F, X);
-   rb_hop_mu<2, OUT>(b, odd, v, acc, F, X);
-   rb_hop_mu<3, OUT>(b, odd, v, acc, F, X);
+#pragma unroll
+   for (int mu = 0; mu < 4; mu++)
+   {
+      const int y = b.fwd[mu];
+      if (y < RBV)
+      {
+         float2 h0[3], h1[3], c0[3], c1[3], u[9];
+#pragma unroll
+         for (int k = 0; k < 9; k++) u[k] = to_f2(b.u[mu][k]);
+         if (own)
+         {
+            float2 sp[12];
+#pragma unroll
+            for (int c = 0; c < 12; c++) sp[c] = F[c][y & (RBH - 1)];
+            if (mu == 0) { spin_project<float2, 0, true>(sp, h0, h1); }
+            if (mu == 1) { spin_project<float2, 1, true>(sp, h0, h1); }
+            if (mu == 2) { spin_project<float2, 2, true>(sp, h0, h1); }
+            if (mu == 3) { spin_project<float2, 3, true>(sp, h0, h1); }
+            su3_mul_f<false>(u, h0, c0);
+            su3_mul_f<false>(u, h1, c1);
+            if (mu == 0) spin_expand_add<float2, 0, true>(c0, c1, acc);
+            if (mu == 1) spin_expand_add<float2, 1, true>(c0, c1, acc);
+            if (mu == 2) spin_expand_add<float2, 2, true>(c0, c1, acc);
+            if (mu == 3) spin_expand_add<float2, 3, true>(c0, c1, acc);
+         }
+         else
+         {
+            if (mu == 0) spin_project<float2, 0, false>(v, h0, h1);
+            if (mu == 1) spin_project<float2, 1, false>(v, h0, h1);
+            if (mu == 2) spin_project<float2, 2, false>(v, h0, h1);
+            if (mu == 3) spin_project<float2, 3, false>(v, h0, h1);
+            su3_mul_f<true>(u, h0, c0);
+            su3_mul_f<true>(u, h1, c1);
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               X[mu][c][y & (RBH - 1)] = c0[c];
+               X[mu][3 + c][y & (RBH - 1)] = c1[c];
+            }
+         }
+      }
+   }
+   __syncthreads();
+   if (own)
+   {
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++)
+      {
+         if (b.bwd[mu] < RBV)
+         {
+            float2 c0[3], c1[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               c0[c] = X[mu][c][x & (RBH - 1)];
+               c1[c] = X[mu][3 + c][x & (RBH - 1)];
+            }
+            if (mu == 0) spin_expand_add<float2, 0, false>(c0, c1, acc);
+            if (mu == 1) spin_expand_add<float2, 1, false>(c0, c1, acc);
+            if (mu == 2) spin_expand_add<float2, 2, false>(c0, c1, acc);
+            if (mu == 3) spin_expand_add<float2, 3, false>(c0, c1, acc);
+         }
+      }
+   }
 }
 
 // block sums of |s2|^2 and (s2, s1) as in blk_mres_dev, lanes with act
@@ -649,8 +759,7 @@ __global__ __launch_bounds__(RBV)
 __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? OQH_SAP_WAVES16 : OQH_SAP_WAVES32)))
 void k_sap_reg(SArgs a, const S *u16, const S *sw16)
 {
-   __shared__ float2 F[12][RBV];
-   __shared__ float2 X[6][RBV];
+   __shared__ float2 LB[18 * RBV];  // hopping exchange (rb_hop)
    // FP16 variant: the Pauli matrices of the block live in LDS, and psi,
    // rho, s0g are FP16 spinors with per-site scales
    constexpr bool MLDS = sizeof(S) == 4;
@@ -714,7 +823,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
 #pragma unroll
          for (int c = 0; c < 12; c++) s1[c] = b.bp ? make_float2(0.0f, 0.0f) : o[c];
       }
-      rb_hop<0>(b, !even, s1, acc, F, X);
+      rb_hop<0>(b, !even, s1, acc, LB);
       if (even)
       {
 #pragma unroll
@@ -729,7 +838,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
       if (ISOLV)
       {
          // s2 (odd) = A_oo^-1 (-1/2 hops of s1_e); s2 (even) = A_ee s1 + 1/2 hops of s2_o
-         rb_hop<1>(b, !even, s1, acc, F, X);
+         rb_hop<1>(b, !even, s1, acc, LB);
          if (!even)
          {
 #pragma unroll
@@ -741,7 +850,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
                for (int c = 0; c < 12; c++) s2[c] = make_float2(0.0f, 0.0f);
             }
          }
-         rb_hop<0>(b, !even, s2, acc, F, X);
+         rb_hop<0>(b, !even, s2, acc, LB);
          if (even)
          {
             pauli_reg(mp, a.mu, s1, s2, mst);
@@ -752,7 +861,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
       }
       else
       {
-         rb_hop<2>(b, !even, s1, acc, F, X);
+         rb_hop<2>(b, !even, s1, acc, LB);
          pauli_reg(mp, even ? a.mu : a.muo, s1, s2, mst);
 #pragma unroll
          for (int c = 0; c < 12; c++)
@@ -763,14 +872,14 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
       float r = (float)rr;
       if (r < 2.0f * FLT_MIN) break;
       r = 1.0f / r;
-      float2 z = make_float2((float)zr * r, (float)zi * r);
+      float2 z = make_float2((float)zr * r, (float)zi * r), mz = make_float2(-z.x, -z.y);
       if (!ISOLV || even)
       {
 #pragma unroll
          for (int c = 0; c < 12; c++)
          {
-            s0[c] = make_float2(s0[c].x + (z.x * s1[c].x - z.y * s1[c].y), s0[c].y + (z.x * s1[c].y + z.y * s1[c].x));
-            s1[c] = make_float2(s1[c].x - (z.x * s2[c].x - z.y * s2[c].y), s1[c].y - (z.x * s2[c].y + z.y * s2[c].x));
+            s0[c] = cmac(s0[c], z, s1[c]);
+            s1[c] = cmac(s1[c], mz, s2[c]);
          }
       }
    }
@@ -778,7 +887,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
    if (ISOLV)
    {
       // odd solution s0_o = s1_o - A_oo^-1 (-1/2 hops of s0_e) (Dwoe_blk, Dwoo_blk)
-      rb_hop<1>(b, !even, s0, acc, F, X);
+      rb_hop<1>(b, !even, s0, acc, LB);
       if (!even)
       {
          float2 o[12];
@@ -861,8 +970,8 @@ __global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, con
          load_spinor((const float2 *)a.s0g, x, sp);                                   \
       if (odd) OQH_LDU(y - a.h, 2 * MU) else OQH_LDU(x - a.h, 2 * MU + 1)               \
       spin_project<float2, MU, true>(sp, h0, h1);                                     \
-      su3_mul<float2, false>(u, h0, c0);                                              \
-      su3_mul<float2, false>(u, h1, c1);                                              \
+      su3_mul_f<false>(u, h0, c0);                                              \
+      su3_mul_f<false>(u, h1, c1);                                              \
       spin_expand_add<float2, MU, true>(c0, c1, acc);                                 \
    }                                                                                  \
    if (m & (2 << (2 * MU)))                                                           \
@@ -874,8 +983,8 @@ __global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, con
          load_spinor((const float2 *)a.s0g, x, sp);                                   \
       if (odd) OQH_LDU(y - a.h, 2 * MU + 1) else OQH_LDU(x - a.h, 2 * MU)               \
       spin_project<float2, MU, false>(sp, h0, h1);                                    \
-      su3_mul<float2, true>(u, h0, c0);                                               \
-      su3_mul<float2, true>(u, h1, c1);                                               \
+      su3_mul_f<true>(u, h0, c0);                                               \
+      su3_mul_f<true>(u, h1, c1);                                               \
       spin_expand_add<float2, MU, false>(c0, c1, acc);                                \
    }
    OQH_FACE(0)
