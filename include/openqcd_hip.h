@@ -87,6 +87,11 @@ void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
    fields and computes chi = Dw(phi) in FP32; the outer FGCR keeps its
    precision. */
 void oqh_set_sap_prec(int prec);
+/* the ncy Schwarz cycles of the FP16 preconditioner of sap_gcr are captured
+   once per solve as a HIP graph and replayed for every Krylov vector (1,
+   default), or launched kernel by kernel (0); no counterpart in the
+   reference, the results are the same */
+void oqh_set_sap_graph(int on);
 /* orthogonalisation of the Krylov vectors in fgcr on device fields (no
    counterpart in the reference): 1 = modified Gram-Schmidt in its inverse
    compact WY form, one pass for all scalar products (with the Gram entries

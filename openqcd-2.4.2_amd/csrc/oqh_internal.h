@@ -68,6 +68,7 @@ struct Lib
    int eoflg = 0;
    int sap_bs[4] = {0, 0, 0, 0}, sap_isolv = 0, sap_nmr = 0, sap_ncy = 0;
    int sap_prec = 0;  // SAP storage: 0 FP32, 1 FP16 links, SW and spinors (per-site scales)
+   int sap_graph = 1; // FP16 preconditioner of sap_gcr replayed as a HIP graph
    int gcr_cgs = 1;   // fgcr orthogonalisation on the device: 1 two-pass (compact WY) MGS, 0 per-term MGS
 
    // geometry tables on the device

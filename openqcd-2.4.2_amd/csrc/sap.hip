@@ -1065,8 +1065,11 @@ __global__ void k_blk_copy(float2 *glob, float2 *blk, const int *imb, int i0, in
 
 static int lexb(const int *bs, const int *y) { return y[3] + bs[3] * (y[2] + bs[2] * (y[1] + bs[1] * y[0])); }
 
+static void drop_sap_graph();
+
 static void free_grid()
 {
+   drop_sap_graph();
    dev_free(G.d_imb); dev_free(G.d_bnb);
    for (int c = 0; c < 2; c++) { dev_free(G.d_face[c]); dev_free(G.d_fmask[c]); G.d_face[c] = nullptr; G.d_fmask[c] = nullptr; }
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
@@ -1224,6 +1227,7 @@ static SArgs sargs(float mu, int nmr)
 {
    Lib &L = lib();
    SArgs a;
+   memset(&a, 0, sizeof(a));  // compared bytewise by the graph cache
    a.u = L.d_u;
    a.sw = G.d_swb;
    a.imb = G.d_imb;
@@ -1337,6 +1341,43 @@ void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
    }
 }
 
+#ifndef OQH_SAP_GRAPH
+#define OQH_SAP_GRAPH 1
+#endif
+
+// the Schwarz cycles of the FP16 preconditioner as one HIP graph: its kernel
+// arguments depend only on rho (fixed within fgcr), the SAP parameters and
+// the grid, so the graph is made once and replayed for every Krylov vector
+static struct
+{
+   hipGraphExec_t exec = nullptr;
+   SArgs a;
+   const float2 *rho = nullptr;
+   int isolv = -1, ncy = -1;
+} SG;
+
+static void drop_sap_graph()
+{
+   if (SG.exec) HIP_OK(hipGraphExecDestroy(SG.exec));
+   SG.exec = nullptr;
+   SG.rho = nullptr;
+}
+
+// rho -> FP16, then ncy cycles from psi = 0 (enqueued on the library stream)
+static void enqueue_mixed_cycles(const SArgs &a0, int isolv, int ncy, const float2 *rho)
+{
+   Lib &L = lib();
+   const int V = L.g.V;
+   SArgs a = a0;
+   k_s32_to_h16<<<(V + 255) / 256, 256, 0, L.stream>>>(rho, G.d_h16[1], G.d_hsc[1], V);
+   for (int n = 0; n < ncy; n++)
+   {
+      // the blocks of both colours see psi = 0 in the first cycle
+      a.pz = (n == 0);
+      for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+   }
+}
+
 // the SAP preconditioner of sap_gcr in the FP16 variant: ncy cycles on FP16
 // fields starting from psi = 0, rho = rho_in, then phi = psi (sap_gcr.c:99-117
 // with the fields kept in FP16).  The residual carried in FP16 no longer
@@ -1347,16 +1388,29 @@ bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, flo
    if (!half_variant()) return false;
    Lib &L = lib();
    const int V = L.g.V;
-   const unsigned nbk = (V + 255) / 256;
    SArgs a = sargs(mu, nmr);
-   k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(rho, G.d_h16[1], G.d_hsc[1], V);
-   for (int n = 0; n < ncy; n++)
+   if (OQH_SAP_GRAPH && L.sap_graph)
    {
-      // the blocks of both colours see psi = 0 in the first cycle
-      a.pz = (n == 0);
-      for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+      bool hit = SG.exec && SG.rho == rho && SG.isolv == isolv && SG.ncy == ncy && memcmp(&SG.a, &a, sizeof(a)) == 0;
+      if (!hit)
+      {
+         drop_sap_graph();
+         hipGraph_t g;
+         HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+         enqueue_mixed_cycles(a, isolv, ncy, rho);
+         HIP_OK(hipStreamEndCapture(L.stream, &g));
+         HIP_OK(hipGraphInstantiate(&SG.exec, g, nullptr, nullptr, 0));
+         HIP_OK(hipGraphDestroy(g));
+         SG.a = a;
+         SG.rho = rho;
+         SG.isolv = isolv;
+         SG.ncy = ncy;
+      }
+      HIP_OK(hipGraphLaunch(SG.exec, L.stream));
    }
-   k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], phi, V);
+   else
+      enqueue_mixed_cycles(a, isolv, ncy, rho);
+   k_h16_to_s32<<<(V + 255) / 256, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], phi, V);
    return true;
 }
 
@@ -1425,6 +1479,7 @@ int sap_grid_nblocks() { return G.ready ? G.nb : 0; }
 // the grid belongs to the geometry: released with it
 void sap_release()
 {
+   drop_sap_graph();
    if (G.ready || G.d_bs) free_grid();
 }
 
@@ -1448,6 +1503,16 @@ extern "C" void sap(float mu, int isolv, int nmr, spinor *psi, spinor *eta)
    sap_device(mu, isolv, nmr, (float2 *)p.dev, (float2 *)r.dev);
    finish(p, F32, V, true);
    finish(r, F32, V, true);
+}
+
+extern "C" void oqh_set_sap_graph(int on)
+{
+   if (on != 0 && on != 1)
+   {
+      fail("oqh_set_sap_graph", "the argument must be 0 or 1");
+      return;
+   }
+   lib().sap_graph = on;
 }
 
 extern "C" void oqh_set_sap_prec(int prec)

@@ -454,6 +454,7 @@ extern "C" void oqh_finalize(void)
    clear_ws_marks();
    L.sap_prec = 0;
    L.gcr_cgs = 1;
+   L.sap_graph = 1;
    for (auto &kv : L.reg) dev_free(kv.second.base);
    L.reg.clear();
    for (int k = 0; k < 4; k++)

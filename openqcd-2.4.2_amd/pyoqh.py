@@ -60,6 +60,7 @@ _SIG = {
     "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
     "oqh_set_sap_prec": (None, [_I]),
     "oqh_set_gcr_orth": (None, [_I]),
+    "oqh_set_sap_graph": (None, [_I]),
     "oqh_Dw_dble_multi": (None, [_D, _I, P, P]), "oqh_Dw_multi": (None, [_F, _I, P, P]),
     "oqh_time_dirac_multi": (_D, [_I, _D, _I, P, P, _I]),
 }

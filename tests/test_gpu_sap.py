@@ -235,3 +235,31 @@ def test_fp16_spinor_format(setup):
         print("field", k, "max deviation / site max", (dev / np.maximum(m, 1e-300)).max())
         assert np.all(dev <= (2.0 ** -12 + 2.0 ** -22) * m)
     assert np.array_equal(out[0][0][:n], psi0[:n])
+
+
+@pytest.mark.gpu
+def test_fp16_preconditioner_graph(setup):
+    """The FP16 Schwarz cycles of sap_gcr replayed as a HIP graph (once per
+    Krylov vector, with the solver's own kernels and host synchronisations
+    in between) give bitwise the solution of the kernel-by-kernel launches."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    lib.oqh_set_sap_prec(1)
+    e = lat.alloc_sd()
+    lat.put_sd(e, eta)
+    out = {}
+    for graph in (0, 1, 1):
+        lib.oqh_set_sap_graph(graph)
+        p = lat.alloc_sd()
+        ifail, status = C.c_int(), C.c_int()
+        rho = lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+        assert ifail.value == 0
+        out.setdefault(graph, []).append((lat.get_sd(p), status.value, rho))
+        lat.free(p)
+    lib.oqh_set_sap_graph(1)
+    lib.oqh_set_sap_prec(0)
+    ref = out[0][0]
+    for got in out[1]:
+        print("graph status", got[1], "eager status", ref[1])
+        assert got[1] == ref[1] and got[2] == ref[2]
+        assert np.array_equal(got[0], ref[0])
