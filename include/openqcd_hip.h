@@ -100,6 +100,34 @@ void oqh_set_sap_graph(int on);
    of the modified process in exact arithmetic. */
 void oqh_set_gcr_orth(int mode);
 
+/* learned preconditioner (no counterpart in the reference): a linear
+   gauge-equivariant network of parallel-transport layers,
+      out(x)[o] = sum_{p=0..8} sum_i W_l[p][o][i] (T_p in)(x)[i],
+   T_0 = 1, T_{2mu+1} in(x) = U(x,mu) in(x+mu), T_{2mu+2} in(x) =
+   U(x-mu,mu)^+ in(x-mu), with C_l colour-vector channels per point (a spinor
+   has its 4 spin components as channels).  Channel mixing runs as BF16
+   matrix-core GEMMs with FP32 accumulation.
+   oqh_nn_create: nlayer layers, widths[0..nlayer] in {4,8,12,16} with
+   widths[0] = widths[nlayer] = 4; returns 0 on success.  Weights are
+   complex, per layer [9][C_out][C_in], layers in order, as (re, im)
+   doubles (2 * oqh_nn_nweights() of them).  oqh_nn_apply: out = NN(in)
+   (single-precision full fields, in != out).  oqh_nn_grad: mean over the n
+   pairs of the loss |NN(r_k) - e_k|^2 / |e_k|^2, its gradient with respect
+   to the weights (dL/dRe w + i dL/dIm w) in dw.  oqh_nn_train_step: one Adam
+   step (beta 0.9, 0.999) of step size lr on that loss; returns the loss
+   before the step.  oqh_set_nn_prec(1): sap_gcr adds NN(residual of the
+   Schwarz cycles) to the preconditioner's output and takes chi = Dw(phi);
+   a solve that fails with it is repeated with plain SAP. */
+int oqh_nn_create(int nlayer, const int *widths);
+int oqh_nn_nweights(void);
+void oqh_nn_set_weights(const double *w);
+void oqh_nn_get_weights(double *w);
+void oqh_nn_apply(spinor *in, spinor *out);
+double oqh_nn_grad(int n, spinor **r, spinor **e, double *dw);
+double oqh_nn_train_step(int n, spinor **r, spinor **e, double lr);
+void oqh_set_nn_prec(int on);
+void oqh_nn_free(void);
+
 /* gauge field in the layout of udfld() (uflds/uflds.c:113-156; 8 links per
    odd point, README.global).  Import invalidates the SW term and the
    single-precision copy; the imported field is taken as given, with the

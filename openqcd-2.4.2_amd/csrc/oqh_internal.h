@@ -69,7 +69,8 @@ struct Lib
    int sap_bs[4] = {0, 0, 0, 0}, sap_isolv = 0, sap_nmr = 0, sap_ncy = 0;
    int sap_prec = 0;  // SAP storage: 0 FP32, 1 FP16 links, SW and spinors (per-site scales)
    int sap_graph = 1; // FP16 preconditioner of sap_gcr replayed as a HIP graph
-   int gcr_cgs = 1;   // fgcr orthogonalisation on the device: 1 two-pass (compact WY) MGS, 0 per-term MGS
+   int gcr_cgs = 1;
+   int nn_prec = 0;   // network correction in the preconditioner of sap_gcr   // fgcr orthogonalisation on the device: 1 two-pass (compact WY) MGS, 0 per-term MGS
 
    // geometry tables on the device
    int *d_nbr = nullptr;            // [8][V]: x+mu at 2mu, x-mu at 2mu+1; bit 31: boundary point
@@ -139,6 +140,9 @@ void build_geometry(Geometry &g, const int *L, const int *Ltrd);
 void upload_tables();
 void sap_release();  // frees the SAP block grid (made for the current geometry)
 void release_flow(); // frees the force field of the Wilson flow
+void release_nn();   // frees the network of the learned preconditioner
+bool nn_ready();     // network created and enabled in sap_gcr
+bool nn_correct(const float2 *r, float2 *phi);  // phi += NN(r)
 
 // layout conversions (reference AoS <-> site blocks), n sites from site 0
 void aos_to_blocks_f64(const double2 *aos, double2 *blk, int n, int nc, hipStream_t st);

@@ -1341,6 +1341,14 @@ void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
    }
 }
 
+// the FP16 residual of the last Schwarz cycles as an FP32 field
+void sap_mixed_residual(float2 *r)
+{
+   Lib &L = lib();
+   const int V = L.g.V;
+   k_h16_to_s32<<<(V + 255) / 256, 256, 0, L.stream>>>(G.d_h16[1], G.d_hsc[1], r, V);
+}
+
 #ifndef OQH_SAP_GRAPH
 #define OQH_SAP_GRAPH 1
 #endif

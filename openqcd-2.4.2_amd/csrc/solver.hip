@@ -20,6 +20,7 @@
 namespace oqh {
 bool sap_prepare(int isolv, const char *where, bool quiet);
 bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi);
+void sap_mixed_residual(float2 *r);
 }
 
 using namespace oqh;
@@ -197,6 +198,7 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
 // ---------------------------------------------------------------------------
 
 static double mud;
+static bool nn_use = false;  // network correction in Mop for the current solve
 
 static void Dop(spinor_dble *s, spinor_dble *r) { Dw_dble(mud, s, r); }
 
@@ -205,14 +207,18 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
    (void)k;
    Lib &L = lib();
    int vt = L.g.VT;
-   if (L.sap_prec == 1)
+   int o0 = 0, o1 = 0, o2 = 0;
+   float2 *r = (float2 *)device_field(rho, F32, &o0), *p = (float2 *)device_field(phi, F32, &o1),
+          *c = (float2 *)device_field(chi, F32, &o2);
+   const bool dev = r && p && c && !o0 && !o1 && !o2;
+   if (L.sap_prec == 1 && dev)
    {
-      // FP16 variant: the whole preconditioner on FP16 fields, chi = D phi
-      // with the FP32 operator
-      int o0 = 0, o1 = 0;
-      float2 *r = (float2 *)device_field(rho, F32, &o0), *p = (float2 *)device_field(phi, F32, &o1);
-      if (r && p && !o0 && !o1 && sap_mixed_mop((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p))
+      // FP16 variant: the whole preconditioner on FP16 fields; with the
+      // network, phi += NN(residual of the Schwarz cycles); chi = D phi with
+      // the FP32 operator
+      if (sap_mixed_mop((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p))
       {
+         if (nn_use) sap_mixed_residual(c), nn_correct(c, p);
          Dw((float)mud, phi, chi);
          return;
       }
@@ -220,6 +226,13 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
    set_s2zero(vt, 2, phi);
    assign_s2s(vt, 2, rho, chi);
    for (int n = 0; n < L.sap_ncy; n++) sap((float)mud, L.sap_isolv, L.sap_nmr, phi, chi);
+   if (nn_use && dev)
+   {
+      // learned correction from the residual the Schwarz cycles leave
+      nn_correct(c, p);
+      Dw((float)mud, phi, chi);
+      return;
+   }
    diff_s2s(vt, 2, rho, chi);
 }
 
@@ -263,7 +276,15 @@ extern "C" double sap_gcr(int nkv, int nmx, int istop, double res, double mu, sp
          mud = mu;
          assign_sd2sd(vt, 2, eta, rsd);
          scale_dble(vt, 2, 1.0 / rho0, rsd);
+         nn_use = nn_ready();
          rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+         if (nn_use && status[0] < 0)
+         {
+            // plain SAP as the fallback of the learned preconditioner
+            nn_use = false;
+            rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+         }
+         nn_use = false;
          scale_dble(vt, 2, rho0, psi);
          rho *= rho0;
          if (status[0] < 0)

@@ -454,6 +454,8 @@ extern "C" void oqh_finalize(void)
    clear_ws_marks();
    L.sap_prec = 0;
    L.gcr_cgs = 1;
+   release_nn();
+   L.nn_prec = 0;
    L.sap_graph = 1;
    for (auto &kv : L.reg) dev_free(kv.second.base);
    L.reg.clear();

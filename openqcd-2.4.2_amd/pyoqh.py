@@ -61,6 +61,11 @@ _SIG = {
     "oqh_set_sap_prec": (None, [_I]),
     "oqh_set_gcr_orth": (None, [_I]),
     "oqh_set_sap_graph": (None, [_I]),
+    "oqh_nn_create": (_I, [_I, P]), "oqh_nn_nweights": (_I, []),
+    "oqh_nn_set_weights": (None, [P]), "oqh_nn_get_weights": (None, [P]),
+    "oqh_nn_apply": (None, [P, P]), "oqh_nn_grad": (_D, [_I, P, P, P]),
+    "oqh_nn_train_step": (_D, [_I, P, P, _D]), "oqh_set_nn_prec": (None, [_I]),
+    "oqh_nn_free": (None, []),
     "oqh_Dw_dble_multi": (None, [_D, _I, P, P]), "oqh_Dw_multi": (None, [_F, _I, P, P]),
     "oqh_time_dirac_multi": (_D, [_I, _D, _I, P, P, _I]),
 }
