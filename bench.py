@@ -60,14 +60,98 @@ def measured_traffic():
     return None
 
 
+NN_WIDTHS = [4, 12, 16, 12, 4]
+
+
+def train_network():
+    """Trains the learned preconditioner on a 16^4 lattice with the physics
+    of the solver workload (periodic bc, Haar links of seed 1 flowed 50 x
+    0.01, csw=1.77, m0=-0.35, SAP 4^4 isolv=1 nmr=4 ncy=5): 32 pairs from
+    Gaussian sources r, e = D^-1 r by SAP+GCR to 1e-12; the network input is
+    the residual the 5 Schwarz cycles leave of r, its target the error they
+    leave of e.  28 pairs train (200 Adam steps, batches of 8), 4 are held
+    out.  Returns the weights and the training record."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    lat = pyoqh.Lattice((16, 16, 16, 16), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
+    rec = {"lattice": "16^4", "pairs": 32, "held_out": 4, "adam_steps": 200, "batch": 8,
+           "widths": NN_WIDTHS}
+    try:
+        lib = lat.lib
+        lib.sap_gcr.restype = C.c_double
+        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.sap.argtypes = [C.c_float, C.c_int, C.c_int, P, P]
+        rng = np.random.default_rng(1)
+        V = lat.V
+        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        lib.fwd_euler(50, 0.01)
+        lib.set_ud_phase()
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        t0 = time.perf_counter()
+        e_, p_ = lat.alloc_sd(), lat.alloc_sd()
+        psi, rho = lat.alloc_s(), lat.alloc_s()
+        fr, fe = [], []
+        for k in range(32):
+            r = rng.standard_normal((V, 24))
+            lat.put_sd(e_, r)
+            ifail, st = C.c_int(), C.c_int()
+            lib.sap_gcr(24, 1024, 0, 1e-12, 0.0, e_, p_, C.byref(ifail), C.byref(st))
+            e = lat.get_sd(p_)
+            lat.put_s(psi, np.zeros((V, 24), np.float32))
+            lat.put_s(rho, r.astype(np.float32))
+            for _ in range(5):
+                lib.sap(0.0, 1, 4, psi, rho)
+            a, b = lat.alloc_s(), lat.alloc_s()
+            lat.put_s(a, lat.get_s(rho))
+            lat.put_s(b, (e - lat.get_s(psi)).astype(np.float32))
+            fr.append(a)
+            fe.append(b)
+        rec["pair_generation_s"] = round(time.perf_counter() - t0, 2)
+        w = np.array(NN_WIDTHS, np.int32)
+        lib.oqh_nn_create(len(NN_WIDTHS) - 1, w.ctypes.data_as(P))
+        # small random complex initial weights [9][C_out][C_in] per layer
+        nw = lib.oqh_nn_nweights()
+        w0 = 0.01 * rng.standard_normal(2 * nw)
+        lib.oqh_nn_set_weights(w0.ctypes.data_as(P))
+
+        def ptrs(f):
+            return (C.c_void_p * len(f))(*f)
+        dw = np.empty(2 * lib.oqh_nn_nweights())
+        hold = (ptrs(fr[28:]), ptrs(fe[28:]))
+        rec["held_out_loss_before"] = round(lib.oqh_nn_grad(4, *hold, dw.ctypes.data_as(P)), 4)
+        lat.sync()
+        t0 = time.perf_counter()
+        for it in range(200):
+            idx = rng.choice(28, 8, replace=False)
+            lib.oqh_nn_train_step(8, ptrs([fr[i] for i in idx]), ptrs([fe[i] for i in idx]),
+                                  3e-3 if it < 100 else 1e-3)
+        lat.sync()
+        rec["training_s"] = round(time.perf_counter() - t0, 2)
+        rec["held_out_loss_after"] = round(lib.oqh_nn_grad(4, *hold, dw.ctypes.data_as(P)), 4)
+        wts = np.empty(2 * lib.oqh_nn_nweights())
+        lib.oqh_nn_get_weights(wts.ctypes.data_as(P))
+        log("bench: network trained, held-out loss %.3f -> %.3f" % (rec["held_out_loss_before"],
+                                                                   rec["held_out_loss_after"]))
+    finally:
+        lat.close()
+    return wts, rec
+
+
 def solve_times(args):
     """Time-to-solution of SAP+GCR on the 48^4 solver workload: periodic bc,
     Haar-random links (numpy, seed 3) smoothed by 50 Euler steps (eps 0.01) of
     the Wilson flow, csw=1.77, m0=-0.35, Gaussian source; FGCR with nkv=24 to
     relative residual 1e-10, SAP on 4^4 blocks (isolv=1, nmr=4, ncy=5).  The
-    plain solver (FP32 SAP, FP64 GCR) and the variant with FP16 storage of the
-    block solver's links and SW matrices are timed on the same field and
-    source; the true residual |eta - Dw_dble psi|/|eta| is recomputed."""
+    plain solver (FP32 SAP, FP64 GCR), the FP16-storage variant (links, SW
+    matrices and spinors in FP16 with per-site scales, FP32 arithmetic) and
+    the FP16 variant with the learned correction (trained at 16^4 by
+    train_network) are timed on the same field and source; the true residual
+    |eta - Dw_dble psi|/|eta| is recomputed."""
+    wts, nn_rec = train_network()
     import ctypes as C
     import numpy as np
     import pyoqh
@@ -100,8 +184,14 @@ def solve_times(args):
             q = lib.norm_square_dble(V, 0, f)
             return q.q[0] + q.q[1]
 
-        for name, prec in (("plain_fp32_sap", 0), ("fp16_storage_sap", 1)):
+        w = np.array(NN_WIDTHS, np.int32)
+        lib.oqh_nn_create(len(NN_WIDTHS) - 1, w.ctypes.data_as(P))
+        lib.oqh_nn_set_weights(wts.ctypes.data_as(P))
+        out["learned"] = nn_rec
+        for name, prec, nn in (("plain_fp32_sap", 0, 0), ("fp16_storage_sap", 1, 0),
+                               ("fp16_sap_learned", 1, 1)):
             lib.oqh_set_sap_prec(prec)
+            lib.oqh_set_nn_prec(nn)
             ifail, status = C.c_int(), C.c_int()
             lat.sync()
             t0 = time.perf_counter()
@@ -115,9 +205,11 @@ def solve_times(args):
                          "true_residual": float("%.3e" % res)}
             log("bench: %s %.3f s status %d residual %.2e" % (name, dt, status.value, res))
         lib.oqh_set_sap_prec(0)
-        if out["plain_fp32_sap"]["ifail"] == 0 and out["fp16_storage_sap"]["ifail"] == 0:
-            out["speedup_fp16_vs_plain"] = round(out["plain_fp32_sap"]["seconds"] /
-                                                 out["fp16_storage_sap"]["seconds"], 3)
+        lib.oqh_set_nn_prec(0)
+        for name in ("fp16_storage_sap", "fp16_sap_learned"):
+            if out["plain_fp32_sap"]["ifail"] == 0 and out[name]["ifail"] == 0:
+                out["speedup_%s_vs_plain" % name] = round(out["plain_fp32_sap"]["seconds"] /
+                                                          out[name]["seconds"], 3)
     finally:
         lat.close()
     return out

@@ -83,6 +83,7 @@ struct SArgs
    __half2 *psi16, *rho16, *s0g16;
    float *psc, *rsc, *s0sc;
    int pz;             // psi is zero on entry (not read)
+   int pull;           // add the face terms from the other colour's s0g first
 };
 
 // FP16 spinor with per-site scale: the 12 colour components of site x are
@@ -747,6 +748,58 @@ __device__ inline void rb_sums(bool act, const float2 *s2, const float2 *s1, dou
    wg_sum3(rr, zr, zi);
 }
 
+// rho(x) += 1/2 hopping terms into block point x from the corrections s0g
+// of the neighbouring blocks (the other colour), as k_sap_faces applies
+// them, taken here at the start of the block solve
+template <typename S>
+__device__ inline void pull_faces(const SArgs &a, const RegBlock<S> &b, const S *ub, int gx, float2 *s1)
+{
+   constexpr bool H = sizeof(S) == 4;
+   const bool odd = gx >= a.h;
+   float2 acc[12], sp[12], u[9], h0[3], h1[3], c0[3], c1[3];
+   zero12(acc);
+#define OQH_PULL_LDU(J, L)                                                             \
+   {                                                                                  \
+      const S *up = ub + ((J) >> 6) * (size_t)(72 * SB) + ((J) & (SB - 1)) + (size_t)(9 * (L)) * SB; \
+      _Pragma("unroll") for (int q = 0; q < 9; q++) u[q] = to_f2(up[q * SB]);          \
+   }
+#define OQH_PULL(MU)                                                                   \
+   if (b.fwd[MU] == RBV)                                                              \
+   {                                                                                  \
+      int y = a.nbr[(size_t)(2 * MU) * a.V + gx] & 0x7fffffff;                        \
+      if (H)                                                                          \
+         load_h16(a.s0g16, a.s0sc, y, sp);                                            \
+      else                                                                            \
+         load_spinor((const float2 *)a.s0g, y, sp);                                   \
+      if (odd) OQH_PULL_LDU(gx - a.h, 2 * MU) else OQH_PULL_LDU(y - a.h, 2 * MU + 1)    \
+      spin_project<float2, MU, true>(sp, h0, h1);                                     \
+      su3_mul_f<false>(u, h0, c0);                                                    \
+      su3_mul_f<false>(u, h1, c1);                                                    \
+      spin_expand_add<float2, MU, true>(c0, c1, acc);                                 \
+   }                                                                                  \
+   if (b.bwd[MU] == RBV)                                                              \
+   {                                                                                  \
+      int y = a.nbr[(size_t)(2 * MU + 1) * a.V + gx] & 0x7fffffff;                    \
+      if (H)                                                                          \
+         load_h16(a.s0g16, a.s0sc, y, sp);                                            \
+      else                                                                            \
+         load_spinor((const float2 *)a.s0g, y, sp);                                   \
+      if (odd) OQH_PULL_LDU(gx - a.h, 2 * MU + 1) else OQH_PULL_LDU(y - a.h, 2 * MU)    \
+      spin_project<float2, MU, false>(sp, h0, h1);                                    \
+      su3_mul_f<true>(u, h0, c0);                                                     \
+      su3_mul_f<true>(u, h1, c1);                                                     \
+      spin_expand_add<float2, MU, false>(c0, c1, acc);                                \
+   }
+   OQH_PULL(0)
+   OQH_PULL(1)
+   OQH_PULL(2)
+   OQH_PULL(3)
+#undef OQH_PULL
+#undef OQH_PULL_LDU
+#pragma unroll
+   for (int c = 0; c < 12; c++) s1[c] = cadd(s1[c], cscale(acc[c], 0.5f));
+}
+
 #ifndef OQH_SAP_WAVES16
 #define OQH_SAP_WAVES16 2
 #endif
@@ -804,6 +857,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
       load_h16(a.rho16, a.rsc, gx, s1);
    else
       load_spinor((const float2 *)a.rho, gx, s1);
+   if (a.pull) pull_faces<S>(a, b, ub, gx, s1);
 #pragma unroll
    for (int c = 0; c < 12; c++)
    {
@@ -1264,22 +1318,24 @@ static bool half_variant()
           G.d_swb16;
 }
 
-// one half-sweep over the blocks of colour ic followed by the face update of
-// the other colour's blocks
-static void sap_colour(const SArgs &a0, int ic, int isolv)
+static bool reg_kernel() { return OQH_SAP_REG && G.bvol == RBV; }
+
+// block solves of the blocks of colour ic; the register kernel first adds
+// the face terms from the other colour's corrections if pull is set
+static void sap_blocks(const SArgs &a0, int ic, int isolv, int pull)
 {
    Lib &L = lib();
    SArgs a = a0;
    a.n0 = ic ? G.nbh : 0;
-   const bool half = half_variant();
-   if (half)
+   a.pull = pull;
+   if (half_variant())
    {
       if (isolv)
          k_sap_reg<1, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
       else
          k_sap_reg<0, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
    }
-   else if (OQH_SAP_REG && G.bvol == RBV)
+   else if (reg_kernel())
    {
       if (isolv)
          k_sap_reg<1, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
@@ -1290,13 +1346,43 @@ static void sap_colour(const SArgs &a0, int ic, int isolv)
       k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
    else
       k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
-   int c = 1 - ic;
+}
+
+// face terms from the corrections of the colour-(1-c) blocks into rho on
+// the faces of the colour-c blocks
+static void sap_push_faces(const SArgs &a, int c)
+{
+   Lib &L = lib();
    if (!G.nface[c]) return;
    unsigned nbk = (G.nface[c] + 255) / 256;
-   if (half)
+   if (half_variant())
       k_sap_faces<__half2><<<nbk, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], G.d_u16);
    else
       k_sap_faces<float2><<<nbk, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], nullptr);
+}
+
+// one Schwarz cycle (sap.c:720-790): colour 0, faces of colour 1, colour 1,
+// faces of colour 0.  With the FP16 register kernel the face terms into a
+// colour are added by its block kernel (pull), so the cycle needs no face
+// kernel except the final one; first: rho is complete on entry (no pull for
+// colour 0); finish: complete rho at the end (faces of colour 0).
+static void sap_cycle(const SArgs &a, int isolv, bool first, bool finish)
+{
+   // the FP32 register kernel runs one wavefront per SIMD, too few to hide
+   // the latency of the scattered face loads: it keeps the face kernel
+   if (half_variant())
+   {
+      sap_blocks(a, 0, isolv, !first);
+      sap_blocks(a, 1, isolv, 1);
+      if (finish) sap_push_faces(a, 0);
+   }
+   else
+   {
+      sap_blocks(a, 0, isolv, 0);
+      sap_push_faces(a, 1);
+      sap_blocks(a, 1, isolv, 0);
+      sap_push_faces(a, 0);
+   }
 }
 
 // FP32 field <-> FP16 field with per-site scales
@@ -1333,7 +1419,7 @@ void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
       k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(psi, G.d_h16[0], G.d_hsc[0], V);
       k_s32_to_h16<<<nbk, 256, 0, L.stream>>>(rho, G.d_h16[1], G.d_hsc[1], V);
    }
-   for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+   sap_cycle(a, isolv, true, true);
    if (half_variant())
    {
       k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], psi, V);
@@ -1361,7 +1447,7 @@ static struct
    hipGraphExec_t exec = nullptr;
    SArgs a;
    const float2 *rho = nullptr;
-   int isolv = -1, ncy = -1;
+   int isolv = -1, ncy = -1, want_res = -1;
 } SG;
 
 static void drop_sap_graph()
@@ -1371,8 +1457,9 @@ static void drop_sap_graph()
    SG.rho = nullptr;
 }
 
-// rho -> FP16, then ncy cycles from psi = 0 (enqueued on the library stream)
-static void enqueue_mixed_cycles(const SArgs &a0, int isolv, int ncy, const float2 *rho)
+// rho -> FP16, then ncy cycles from psi = 0 (enqueued on the library stream);
+// the final face terms only if the residual is wanted
+static void enqueue_mixed_cycles(const SArgs &a0, int isolv, int ncy, const float2 *rho, bool want_res)
 {
    Lib &L = lib();
    const int V = L.g.V;
@@ -1382,7 +1469,7 @@ static void enqueue_mixed_cycles(const SArgs &a0, int isolv, int ncy, const floa
    {
       // the blocks of both colours see psi = 0 in the first cycle
       a.pz = (n == 0);
-      for (int ic = 0; ic < 2; ic++) sap_colour(a, ic, isolv);
+      sap_cycle(a, isolv, n == 0, n == ncy - 1 && want_res);
    }
 }
 
@@ -1391,7 +1478,7 @@ static void enqueue_mixed_cycles(const SArgs &a0, int isolv, int ncy, const floa
 // with the fields kept in FP16).  The residual carried in FP16 no longer
 // equals rho_in - D psi to FP32 accuracy, so the caller computes chi = D phi
 // with the FP32 operator instead of taking rho_in - rho.
-bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi)
+bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, bool want_res)
 {
    if (!half_variant()) return false;
    Lib &L = lib();
@@ -1399,13 +1486,14 @@ bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, flo
    SArgs a = sargs(mu, nmr);
    if (OQH_SAP_GRAPH && L.sap_graph)
    {
-      bool hit = SG.exec && SG.rho == rho && SG.isolv == isolv && SG.ncy == ncy && memcmp(&SG.a, &a, sizeof(a)) == 0;
+      bool hit = SG.exec && SG.rho == rho && SG.isolv == isolv && SG.ncy == ncy && SG.want_res == (int)want_res &&
+                 memcmp(&SG.a, &a, sizeof(a)) == 0;
       if (!hit)
       {
          drop_sap_graph();
          hipGraph_t g;
          HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
-         enqueue_mixed_cycles(a, isolv, ncy, rho);
+         enqueue_mixed_cycles(a, isolv, ncy, rho, want_res);
          HIP_OK(hipStreamEndCapture(L.stream, &g));
          HIP_OK(hipGraphInstantiate(&SG.exec, g, nullptr, nullptr, 0));
          HIP_OK(hipGraphDestroy(g));
@@ -1413,11 +1501,12 @@ bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, flo
          SG.rho = rho;
          SG.isolv = isolv;
          SG.ncy = ncy;
+         SG.want_res = want_res;
       }
       HIP_OK(hipGraphLaunch(SG.exec, L.stream));
    }
    else
-      enqueue_mixed_cycles(a, isolv, ncy, rho);
+      enqueue_mixed_cycles(a, isolv, ncy, rho, want_res);
    k_h16_to_s32<<<(V + 255) / 256, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], phi, V);
    return true;
 }

@@ -19,7 +19,7 @@
 
 namespace oqh {
 bool sap_prepare(int isolv, const char *where, bool quiet);
-bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi);
+bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, bool want_res);
 void sap_mixed_residual(float2 *r);
 }
 
@@ -216,7 +216,7 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
       // FP16 variant: the whole preconditioner on FP16 fields; with the
       // network, phi += NN(residual of the Schwarz cycles); chi = D phi with
       // the FP32 operator
-      if (sap_mixed_mop((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p))
+      if (sap_mixed_mop((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p, nn_use))
       {
          if (nn_use) sap_mixed_residual(c), nn_correct(c, p);
          Dw((float)mud, phi, chi);
