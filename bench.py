@@ -147,10 +147,12 @@ def solve_times(args):
     the Wilson flow, csw=1.77, m0=-0.35, Gaussian source; FGCR with nkv=24 to
     relative residual 1e-10, SAP on 4^4 blocks (isolv=1, nmr=4, ncy=5).  The
     plain solver (FP32 SAP, FP64 GCR), the FP16-storage variant (links, SW
-    matrices and spinors in FP16 with per-site scales, FP32 arithmetic) and
-    the FP16 variant with the learned correction (trained at 16^4 by
-    train_network) are timed on the same field and source; the true residual
-    |eta - Dw_dble psi|/|eta| is recomputed."""
+    matrices and spinors in FP16 with per-site scales, FP32 arithmetic), the
+    FP64 variant (FP64 Schwarz cycles) and the FP16 variant with the learned
+    correction (trained at 16^4 by train_network) are timed on the same field
+    and source; the true residual |eta - Dw_dble psi|/|eta| is recomputed and
+    the error |psi - psi_ref|/|psi_ref| against a solve to relative residual
+    1e-14 is reported."""
     wts, nn_rec = train_network()
     import ctypes as C
     import numpy as np
@@ -188,8 +190,19 @@ def solve_times(args):
         lib.oqh_nn_create(len(NN_WIDTHS) - 1, w.ctypes.data_as(P))
         lib.oqh_nn_set_weights(wts.ctypes.data_as(P))
         out["learned"] = nn_rec
+        # reference solution: relative residual 1e-14 (FP64 outer residual)
+        pref = lat.alloc_sd()
+        lib.oqh_set_sap_prec(0)
+        ifail, status = C.c_int(), C.c_int()
+        t0 = time.perf_counter()
+        lib.sap_gcr(24, 2048, 0, 1e-14, 0.0, e, pref, C.byref(ifail), C.byref(status))
+        lat.sync()
+        out["reference_solve"] = {"sap": "fp32", "residual": 1e-14, "status": status.value,
+                                  "ifail": ifail.value, "seconds": round(time.perf_counter() - t0, 3)}
+        log("bench: reference solve status %d ifail %d" % (status.value, ifail.value))
+        nref = nsq(pref)
         for name, prec, nn in (("plain_fp32_sap", 0, 0), ("fp16_storage_sap", 1, 0),
-                               ("fp16_sap_learned", 1, 1)):
+                               ("fp64_sap", 2, 0), ("fp16_sap_learned", 1, 1)):
             lib.oqh_set_sap_prec(prec)
             lib.oqh_set_nn_prec(nn)
             ifail, status = C.c_int(), C.c_int()
@@ -201,12 +214,14 @@ def solve_times(args):
             lib.Dw_dble(0.0, p, r)
             lib.mulr_spinor_add_dble(V, 0, r, e, -1.0)
             res = (nsq(r) / nsq(e)) ** 0.5
+            lib.mulr_spinor_add_dble(V, 0, p, pref, -1.0)
+            err = (nsq(p) / nref) ** 0.5
             out[name] = {"seconds": round(dt, 3), "status": status.value, "ifail": ifail.value,
-                         "true_residual": float("%.3e" % res)}
-            log("bench: %s %.3f s status %d residual %.2e" % (name, dt, status.value, res))
+                         "true_residual": float("%.3e" % res), "error_vs_reference": float("%.3e" % err)}
+            log("bench: %s %.3f s status %d residual %.2e error %.2e" % (name, dt, status.value, res, err))
         lib.oqh_set_sap_prec(0)
         lib.oqh_set_nn_prec(0)
-        for name in ("fp16_storage_sap", "fp16_sap_learned"):
+        for name in ("fp16_storage_sap", "fp64_sap", "fp16_sap_learned"):
             if out["plain_fp32_sap"]["ifail"] == 0 and out[name]["ifail"] == 0:
                 out["speedup_%s_vs_plain" % name] = round(out["plain_fp32_sap"]["seconds"] /
                                                           out[name]["seconds"], 3)

@@ -59,6 +59,10 @@ struct SapGrid
    float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
    __half2 *d_u16 = nullptr;             // FP16 links (FP16 variant of the solver)
    __half2 *d_swb16 = nullptr;           // FP16 block SW field
+   double2 *d_swb64 = nullptr;           // FP64 block SW field (FP64 variant)
+   int swb64_version = -1;
+   double2 *d_h64[3] = {nullptr, nullptr, nullptr};  // FP64 psi, rho, s0g
+   double2 *d_scr64 = nullptr;           // FP64 block fields [nbh][3][Ub]
    __half2 *d_h16[3] = {nullptr, nullptr, nullptr};  // FP16 psi, rho, s0g
    float *d_hsc[3] = {nullptr, nullptr, nullptr};    // their per-site scales
    int u16_epoch = -1, swb16_epoch = -1; // versions they were made from
@@ -69,15 +73,15 @@ struct SapGrid
 
 static SapGrid G;
 
-struct SArgs
+template <typename C> struct SArgsT
 {
-   const float2 *u, *sw;
+   const C *u, *sw;
    const int *imb, *bnb, *nbr;
    const unsigned char *bnd;
    int V, h, bvol, bh, masked, eoflg, nmr;
-   float mu, muo;
+   decltype(C::x) mu, muo;
    size_t Ub;
-   float2 *psi, *rho, *s0g, *scr;
+   C *psi, *rho, *s0g, *scr;
    int n0;             // first block of the colour
    // FP16 variant: psi, rho and s0g as FP16 spinors with per-site scales
    __half2 *psi16, *rho16, *s0g16;
@@ -85,6 +89,8 @@ struct SArgs
    int pz;             // psi is zero on entry (not read)
    int pull;           // add the face terms from the other colour's s0g first
 };
+typedef SArgsT<float2> SArgs;   // FP32 and FP16 variants
+typedef SArgsT<double2> SArgsD; // FP64 variant
 
 // FP16 spinor with per-site scale: the 12 colour components of site x are
 // stored as v / m in FP16 (same unit layout as the FP32 fields), m = the
@@ -115,7 +121,8 @@ __device__ inline void store_h16(__half2 *h, float *sc, int x, const float2 *v)
 
 // link of direction mu, forward (x->x+mu) or backward (x-mu->x), seen from
 // global point gx with global neighbour gy
-__device__ inline const float2 *blk_link(const SArgs &a, int gx, int gy, int mu, bool fwd)
+template <typename C>
+__device__ inline const C *blk_link(const SArgsT<C> &a, int gx, int gy, int mu, bool fwd)
 {
    if (gx >= a.h) return link_ptr(a.u, gx - a.h, fwd ? 2 * mu : 2 * mu + 1);
    return link_ptr(a.u, gy - a.h, fwd ? 2 * mu + 1 : 2 * mu);
@@ -123,10 +130,10 @@ __device__ inline const float2 *blk_link(const SArgs &a, int gx, int gy, int mu,
 
 // acc += hopping terms from the in-block neighbours of block point i;
 // neighbours on the global boundary (open bc) contribute nothing
-template <int MU>
-__device__ inline void blk_hop_mu(const SArgs &a, const int *imb, const float2 *S, int i, int gx, float2 *acc)
+template <int MU, typename C>
+__device__ inline void blk_hop_mu(const SArgsT<C> &a, const int *imb, const C *S, int i, int gx, C *acc)
 {
-   float2 sp[12];
+   C sp[12];
    int j = a.bnb[(2 * MU) * a.bvol + i];
    if (j < a.bvol)
    {
@@ -134,7 +141,7 @@ __device__ inline void blk_hop_mu(const SArgs &a, const int *imb, const float2 *
       if (!(a.masked && a.bnd[gy]))
       {
          load_spinor(S, j, sp);
-         hop_one<float2, MU, true>(sp, blk_link(a, gx, gy, MU, true), acc);
+         hop_one<C, MU, true>(sp, blk_link(a, gx, gy, MU, true), acc);
       }
    }
    j = a.bnb[(2 * MU + 1) * a.bvol + i];
@@ -144,12 +151,13 @@ __device__ inline void blk_hop_mu(const SArgs &a, const int *imb, const float2 *
       if (!(a.masked && a.bnd[gy]))
       {
          load_spinor(S, j, sp);
-         hop_one<float2, MU, false>(sp, blk_link(a, gx, gy, MU, false), acc);
+         hop_one<C, MU, false>(sp, blk_link(a, gx, gy, MU, false), acc);
       }
    }
 }
 
-__device__ inline void blk_hop(const SArgs &a, const int *imb, const float2 *S, int i, int gx, float2 *acc)
+template <typename C>
+__device__ inline void blk_hop(const SArgsT<C> &a, const int *imb, const C *S, int i, int gx, C *acc)
 {
    blk_hop_mu<0>(a, imb, S, i, gx, acc);
    blk_hop_mu<1>(a, imb, S, i, gx, acc);
@@ -157,10 +165,10 @@ __device__ inline void blk_hop(const SArgs &a, const int *imb, const float2 *S, 
    blk_hop_mu<3>(a, imb, S, i, gx, acc);
 }
 
-__device__ inline void zero12(float2 *v)
+template <typename C> __device__ inline void zero12(C *v)
 {
 #pragma unroll
-   for (int c = 0; c < 12; c++) v[c] = make_float2(0.0f, 0.0f);
+   for (int c = 0; c < 12; c++) v[c] = cmk<C>(0, 0);
 }
 
 // work-group sum of two doubles, broadcast to all threads
@@ -219,77 +227,83 @@ __device__ inline void wg_sum3(double &x, double &y, double &z)
 }
 
 // S2 = Dw_blk S1 on the points [i0, i1) of the block (Dw.c:1475)
-__device__ void blk_dw(const SArgs &a, const int *imb, float2 *S1, float2 *S2, int i0, int i1, float mu_e)
+template <typename C>
+__device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0, int i1, decltype(C::x) mu_e)
 {
+   typedef decltype(C::x) R;
    for (int i = i0 + threadIdx.x; i < i1; i += SWG)
    {
       int gx = imb[i];
-      float2 o[12];
+      C o[12];
       if (a.masked && a.bnd[gx])
       {
          zero12(o);
          store_spinor(S2, i, o);
          continue;
       }
-      float2 acc[12], v[12];
+      C acc[12], v[12];
       zero12(acc);
       blk_hop(a, imb, S1, i, gx, acc);
-      load_spinor((const float2 *)S1, i, v);
+      load_spinor((const C *)S1, i, v);
       pauli_site(a.sw, gx, i < a.bh ? mu_e : a.muo, v, o);
 #pragma unroll
-      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], -0.5f));
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], (R)-0.5));
       store_spinor(S2, i, o);
    }
 }
 
 // even points: S2_e = Dwhat_blk S1 (Dw.c:1828), odd half of S2 used for the
 // intermediate A_oo^-1 D_oe S1
-__device__ void blk_dwhat(const SArgs &a, const int *imb, float2 *S1, float2 *S2, float mu)
+template <typename C>
+__device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decltype(C::x) mu)
 {
+   typedef decltype(C::x) R;
    for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
    {
       int gx = imb[i];
-      float2 o[12];
+      C o[12];
       if (a.masked && a.bnd[gx])
       {
          zero12(o);
          store_spinor(S2, i, o);
          continue;
       }
-      float2 acc[12];
+      C acc[12];
       zero12(acc);
       blk_hop(a, imb, S1, i, gx, acc);
 #pragma unroll
-      for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
-      pauli_site(a.sw, gx, 0.0f, acc, o);
+      for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], (R)-0.5);
+      pauli_site(a.sw, gx, (R)0, acc, o);
       store_spinor(S2, i, o);
    }
    __syncthreads();
    for (int i = threadIdx.x; i < a.bh; i += SWG)
    {
       int gx = imb[i];
-      float2 o[12];
+      C o[12];
       if (a.masked && a.bnd[gx])
       {
          zero12(o);
          store_spinor(S2, i, o);
          continue;
       }
-      float2 acc[12], v[12];
+      C acc[12], v[12];
       zero12(acc);
       blk_hop(a, imb, S2, i, gx, acc);
-      load_spinor((const float2 *)S1, i, v);
+      load_spinor((const C *)S1, i, v);
       pauli_site(a.sw, gx, mu, v, o);
 #pragma unroll
-      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+      for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], (R)0.5));
       store_spinor(S2, i, o);
    }
 }
 
 // minimal residual steps on points [0, nv) (blk_solv.c:1117-1186)
-template <bool EO>
-__device__ void blk_mres_dev(const SArgs &a, const int *imb, float2 *S0, float2 *S1, float2 *S2, float mu)
+template <bool EO, typename C>
+__device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C *S2, decltype(C::x) mu)
 {
+   typedef decltype(C::x) R;
+   const R rmin = sizeof(R) == 4 ? (R)FLT_MIN : (R)DBL_MIN;
    int nv = EO ? a.bh : a.bvol;
    for (int i = threadIdx.x; i < nv; i += SWG) zero_spinor(S0, i);
    __syncthreads();
@@ -303,16 +317,16 @@ __device__ void blk_mres_dev(const SArgs &a, const int *imb, float2 *S0, float2 
       double rr = 0.0, zr = 0.0, zi = 0.0;
       for (int i = threadIdx.x; i < nv; i += SWG)
       {
-         const float2 *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
-         const float2 *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         const C *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         const C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
 #pragma unroll
          for (int k = 0; k < 4; k++)
          {
-            float nr = 0.0f, pr = 0.0f, pi = 0.0f;
+            R nr = 0, pr = 0, pi = 0;
 #pragma unroll
             for (int c = 0; c < 3; c++)
             {
-               float2 u = p2[(3 * k + c) * SB], v = p1[(3 * k + c) * SB];
+               C u = p2[(3 * k + c) * SB], v = p1[(3 * k + c) * SB];
                nr += u.x * u.x + u.y * u.y;
                pr += u.x * v.x + u.y * v.y;
                pi += u.x * v.y - u.y * v.x;
@@ -323,21 +337,21 @@ __device__ void blk_mres_dev(const SArgs &a, const int *imb, float2 *S0, float2 
          }
       }
       wg_sum3(rr, zr, zi);
-      float r = (float)rr;
-      if (r < 2.0f * FLT_MIN) break;
-      r = 1.0f / r;
-      float2 z = make_float2((float)zr * r, (float)zi * r);
+      R r = (R)rr;
+      if (r < (R)2 * rmin) break;
+      r = (R)1 / r;
+      C z = cmk<C>((R)zr * r, (R)zi * r);
       for (int i = threadIdx.x; i < nv; i += SWG)
       {
-         float2 *p0 = S0 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
-         float2 *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
-         const float2 *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         C *p0 = S0 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
+         const C *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
 #pragma unroll
          for (int c = 0; c < 12; c++)
          {
-            float2 s1 = p1[c * SB], s2 = p2[c * SB], s0 = p0[c * SB];
-            p0[c * SB] = make_float2(s0.x + (z.x * s1.x - z.y * s1.y), s0.y + (z.x * s1.y + z.y * s1.x));
-            p1[c * SB] = make_float2(s1.x - (z.x * s2.x - z.y * s2.y), s1.y - (z.x * s2.y + z.y * s2.x));
+            C s1 = p1[c * SB], s2 = p2[c * SB], s0 = p0[c * SB];
+            p0[c * SB] = cmk<C>(s0.x + (z.x * s1.x - z.y * s1.y), s0.y + (z.x * s1.y + z.y * s1.x));
+            p1[c * SB] = cmk<C>(s1.x - (z.x * s2.x - z.y * s2.y), s1.y - (z.x * s2.y + z.y * s2.x));
          }
       }
       __syncthreads();
@@ -346,18 +360,19 @@ __device__ void blk_mres_dev(const SArgs &a, const int *imb, float2 *S0, float2 
 
 // isolv=1 preparation: S1_o = A_oo^-1 S1_o, then S1_e -= D_eo S1_o
 // (Dwoo_blk, Dweo_blk of sap.c:759-761)
-__device__ void blk_eo_prepare(const SArgs &a, const int *imb, float2 *S1)
+template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const int *imb, C *S1)
 {
+   typedef decltype(C::x) R;
    for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
    {
       int gx = imb[i];
-      float2 v[12], o[12];
+      C v[12], o[12];
       if (a.masked && a.bnd[gx])
          zero12(o);
       else
       {
-         load_spinor((const float2 *)S1, i, v);
-         pauli_site(a.sw, gx, 0.0f, v, o);
+         load_spinor((const C *)S1, i, v);
+         pauli_site(a.sw, gx, (R)0, v, o);
       }
       store_spinor(S1, i, o);
    }
@@ -365,17 +380,17 @@ __device__ void blk_eo_prepare(const SArgs &a, const int *imb, float2 *S1)
    for (int i = threadIdx.x; i < a.bh; i += SWG)
    {
       int gx = imb[i];
-      float2 o[12];
+      C o[12];
       if (a.masked && a.bnd[gx])
          zero12(o);
       else
       {
-         float2 acc[12];
+         C acc[12];
          zero12(acc);
          blk_hop(a, imb, S1, i, gx, acc);
-         load_spinor((const float2 *)S1, i, o);
+         load_spinor((const C *)S1, i, o);
 #pragma unroll
-         for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+         for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], (R)0.5));
       }
       store_spinor(S1, i, o);
    }
@@ -383,26 +398,27 @@ __device__ void blk_eo_prepare(const SArgs &a, const int *imb, float2 *S1)
 }
 
 // isolv=1 completion: odd solution (Dwoe_blk, Dwoo_blk; update_flds1/2)
-__device__ void blk_eo_finish(const SArgs &a, const int *imb, float2 *S0, float2 *S1)
+template <typename C> __device__ void blk_eo_finish(const SArgsT<C> &a, const int *imb, C *S0, C *S1)
 {
+   typedef decltype(C::x) R;
    for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
    {
       int gx = imb[i];
-      float2 o[12];
+      C o[12];
       if (a.masked && a.bnd[gx])
          zero12(o);
       else
       {
-         float2 acc[12];
+         C acc[12];
          zero12(acc);
          blk_hop(a, imb, S0, i, gx, acc);
 #pragma unroll
-         for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
-         pauli_site(a.sw, gx, 0.0f, acc, o);
+         for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], (R)-0.5);
+         pauli_site(a.sw, gx, (R)0, acc, o);
       }
       // s0_o = s1_o - A_oo^-1 D_oe s0_e
-      float2 r1[12];
-      load_spinor((const float2 *)S1, i, r1);
+      C r1[12];
+      load_spinor((const C *)S1, i, r1);
 #pragma unroll
       for (int c = 0; c < 12; c++) o[c] = csub(r1[c], o[c]);
       store_spinor(S0, i, o);
@@ -413,22 +429,23 @@ __device__ void blk_eo_finish(const SArgs &a, const int *imb, float2 *S0, float2
 // the Schwarz sweep over the blocks of one colour
 // ---------------------------------------------------------------------------
 
-template <int ISOLV>
-__global__ __launch_bounds__(SWG) void k_sap_blocks(SArgs a)
+template <int ISOLV, typename C>
+__global__ __launch_bounds__(SWG) void k_sap_blocks(SArgsT<C> a)
 {
+   typedef decltype(C::x) R;
    int n = a.n0 + blockIdx.x;
    const int *imb = a.imb + (size_t)n * a.bvol;
-   float2 *S0 = a.scr + (size_t)blockIdx.x * 3 * a.Ub;
-   float2 *S1 = S0 + a.Ub, *S2 = S1 + a.Ub;
+   C *S0 = a.scr + (size_t)blockIdx.x * 3 * a.Ub;
+   C *S1 = S0 + a.Ub, *S2 = S1 + a.Ub;
 
    for (int i = threadIdx.x; i < a.bvol; i += SWG)
    {
       int gx = imb[i];
-      float2 v[12];
+      C v[12];
       if (!ISOLV && a.masked && a.bnd[gx])
          zero12(v);
       else
-         load_spinor((const float2 *)a.rho, gx, v);
+         load_spinor((const C *)a.rho, gx, v);
       store_spinor(S1, i, v);
    }
    __syncthreads();
@@ -446,22 +463,22 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgs a)
    for (int i = threadIdx.x; i < a.bvol; i += SWG)
    {
       int gx = imb[i];
-      float2 s0[12], s1[12], p[12];
-      load_spinor((const float2 *)S0, i, s0);
-      load_spinor((const float2 *)S1, i, s1);
-      load_spinor((const float2 *)a.psi, gx, p);
+      C s0[12], s1[12], p[12];
+      load_spinor((const C *)S0, i, s0);
+      load_spinor((const C *)S1, i, s1);
+      load_spinor((const C *)a.psi, gx, p);
 #pragma unroll
       for (int c = 0; c < 12; c++) p[c] = cadd(p[c], s0[c]);
       store_spinor(a.psi, gx, p);
       if (ISOLV && i >= a.bh)
       {
          // odd residual -i mu gamma5 s0 (update_flds1) or 0 (eoflg, update_flds2)
-         float m = a.eoflg ? 0.0f : a.mu;
+         R m = a.eoflg ? (R)0 : a.mu;
 #pragma unroll
          for (int c = 0; c < 12; c++)
          {
-            float cm = (c < 6) ? -m : m;
-            s1[c] = make_float2(-cm * s0[c].y, cm * s0[c].x);
+            R cm = (c < 6) ? -m : m;
+            s1[c] = cmk<C>(-cm * s0[c].y, cm * s0[c].x);
          }
       }
       store_spinor(a.rho, gx, s1);
@@ -1128,7 +1145,8 @@ static void free_grid()
    for (int c = 0; c < 2; c++) { dev_free(G.d_face[c]); dev_free(G.d_fmask[c]); G.d_face[c] = nullptr; G.d_fmask[c] = nullptr; }
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
    dev_free(G.d_u16); dev_free(G.d_swb16);
-   for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); }
+   for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); dev_free(G.d_h64[k]); }
+   dev_free(G.d_swb64); dev_free(G.d_scr64);
    G = SapGrid();
 }
 
@@ -1270,10 +1288,19 @@ static int assign_swb(int set)
    if (set == ALL_PTS || set == EVEN_PTS) ifail |= invert_pauli(tmp, tmp, 0, h, 0.0);
    if (set == ALL_PTS || set == ODD_PTS) ifail |= invert_pauli(tmp, tmp, h, h, 0.0);
    convert_f64_to_f32(tmp, G.d_swb, nun, L.stream);
+   G.swb_version++;
+   if (L.sap_prec == 2)
+   {
+      if (!G.d_swb64) G.d_swb64 = (double2 *)dev_alloc(nun * sizeof(double2));
+      if (G.d_swb64)
+      {
+         HIP_OK(hipMemcpyAsync(G.d_swb64, tmp, nun * sizeof(double2), hipMemcpyDeviceToDevice, L.stream));
+         G.swb64_version = G.swb_version;
+      }
+   }
    HIP_OK(hipStreamSynchronize(L.stream));
    G.swb_set = set;
    G.swb_epoch = L.swd_epoch;
-   G.swb_version++;
    return ifail;
 }
 
@@ -1404,11 +1431,143 @@ __global__ void k_h16_to_s32(const __half2 *h, const float *sc, float2 *__restri
    store_spinor(f, x, v);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 variant: the generic block solver on double fields, FP64 links and
+// the FP64 block SW field
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_sap_faces64(SArgsD a, const int *face, const unsigned char *fmask, int nf)
+{
+   int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= nf) return;
+   int y = face[k];
+   int m = fmask[k];
+   double2 acc[12], sp[12];
+   zero12(acc);
+   int odd = y >= a.h;
+#define OQH_FACE64(MU)                                                                 \
+   if (m & (1 << (2 * MU)))                                                           \
+   {                                                                                  \
+      int x = a.nbr[(size_t)(2 * MU) * a.V + y] & 0x7fffffff;                         \
+      load_spinor((const double2 *)a.s0g, x, sp);                                     \
+      const double2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU) : link_ptr(a.u, x - a.h, 2 * MU + 1); \
+      hop_one<double2, MU, true>(sp, up, acc);                                        \
+   }                                                                                  \
+   if (m & (2 << (2 * MU)))                                                           \
+   {                                                                                  \
+      int x = a.nbr[(size_t)(2 * MU + 1) * a.V + y] & 0x7fffffff;                     \
+      load_spinor((const double2 *)a.s0g, x, sp);                                     \
+      const double2 *up = odd ? link_ptr(a.u, y - a.h, 2 * MU + 1) : link_ptr(a.u, x - a.h, 2 * MU); \
+      hop_one<double2, MU, false>(sp, up, acc);                                       \
+   }
+   OQH_FACE64(0)
+   OQH_FACE64(1)
+   OQH_FACE64(2)
+   OQH_FACE64(3)
+#undef OQH_FACE64
+   double2 r[12];
+   load_spinor((const double2 *)a.rho, y, r);
+#pragma unroll
+   for (int c = 0; c < 12; c++) r[c] = cadd(r[c], cscale(acc[c], 0.5));
+   store_spinor(a.rho, y, r);
+}
+
+__global__ void k_s32_to_s64(const float2 *__restrict__ a, double2 *__restrict__ b, size_t n)
+{
+   size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (k < n) b[k] = make_double2(a[k].x, a[k].y);
+}
+
+// phi = psi, chi = rho_in - rho (FP64 fields rounded to FP32)
+__global__ void k_mop64_out(const double2 *__restrict__ psi, const double2 *__restrict__ rho,
+                            const float2 *__restrict__ rin, float2 *__restrict__ phi, float2 *__restrict__ chi, size_t n)
+{
+   size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (k >= n) return;
+   phi[k] = make_float2((float)psi[k].x, (float)psi[k].y);
+   chi[k] = make_float2((float)((double)rin[k].x - rho[k].x), (float)((double)rin[k].y - rho[k].y));
+}
+
+static SArgsD sargs64(double mu, int nmr)
+{
+   Lib &L = lib();
+   SArgsD a;
+   memset(&a, 0, sizeof(a));
+   a.u = L.d_ud;
+   a.sw = G.d_swb64;
+   a.imb = G.d_imb;
+   a.bnb = G.d_bnb;
+   a.nbr = L.d_nbr;
+   a.bnd = L.d_bnd;
+   a.V = L.g.V;
+   a.h = L.g.V / 2;
+   a.bvol = G.bvol;
+   a.bh = G.bh;
+   a.masked = (L.bc != 3);
+   a.eoflg = L.eoflg;
+   a.nmr = nmr;
+   a.mu = mu;
+   a.muo = L.eoflg ? 0.0 : mu;
+   a.Ub = G.Ub;
+   a.psi = G.d_h64[0];
+   a.rho = G.d_h64[1];
+   a.s0g = G.d_h64[2];
+   a.scr = G.d_scr64;
+   return a;
+}
+
+static bool f64_variant() { return lib().sap_prec == 2 && G.d_h64[0] && G.d_scr64 && G.d_swb64; }
+
+// one Schwarz cycle on the FP64 fields psi64, rho64
+static void sap_cycle64(const SArgsD &a0, int isolv)
+{
+   Lib &L = lib();
+   SArgsD a = a0;
+   for (int ic = 0; ic < 2; ic++)
+   {
+      a.n0 = ic ? G.nbh : 0;
+      if (isolv)
+         k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
+      else
+         k_sap_blocks<0><<<G.nbh, SWG, 0, L.stream>>>(a);
+      int c = 1 - ic;
+      if (G.nface[c])
+         k_sap_faces64<<<(G.nface[c] + 255) / 256, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c]);
+   }
+}
+
+// the SAP preconditioner of sap_gcr in the FP64 variant: ncy cycles on FP64
+// fields from psi = 0, rho = rho_in; phi = psi, chi = rho_in - rho
+bool sap_mop64(double mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, float2 *chi)
+{
+   if (!f64_variant()) return false;
+   Lib &L = lib();
+   const size_t nu = units_for(L.g.V, 12);
+   const unsigned nbk = (unsigned)((nu + 255) / 256);
+   SArgsD a = sargs64(mu, nmr);
+   HIP_OK(hipMemsetAsync(G.d_h64[0], 0, nu * sizeof(double2), L.stream));
+   k_s32_to_s64<<<nbk, 256, 0, L.stream>>>(rho, G.d_h64[1], nu);
+   for (int n = 0; n < ncy; n++) sap_cycle64(a, isolv);
+   k_mop64_out<<<nbk, 256, 0, L.stream>>>(G.d_h64[0], G.d_h64[1], rho, phi, chi, nu);
+   return true;
+}
+
 // one SAP cycle on device fields (no synchronisation); the FP16 variant
 // converts psi and rho to FP16 spinors and back
 void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
 {
    Lib &L = lib();
+   if (f64_variant())
+   {
+      const size_t nu = units_for(L.g.V, 12);
+      const unsigned nbk = (unsigned)((nu + 255) / 256);
+      k_s32_to_s64<<<nbk, 256, 0, L.stream>>>(psi, G.d_h64[0], nu);
+      k_s32_to_s64<<<nbk, 256, 0, L.stream>>>(rho, G.d_h64[1], nu);
+      sap_cycle64(sargs64(mu, nmr), isolv);
+      convert_f64_to_f32(G.d_h64[0], psi, nu, L.stream);
+      convert_f64_to_f32(G.d_h64[1], rho, nu, L.stream);
+      return;
+   }
    SArgs a = sargs(mu, nmr);
    a.psi = psi;
    a.rho = rho;
@@ -1558,7 +1717,7 @@ bool sap_prepare(int isolv, const char *where, bool quiet)
    ensure_u();
    int want = isolv ? ODD_PTS : NO_PTS;
    if (!L.swd_valid || L.swd_inv[0] || L.swd_inv[1]) ensure_swd(0, 0);
-   if (G.swb_set != want || G.swb_epoch != L.swd_epoch)
+   if (G.swb_set != want || G.swb_epoch != L.swd_epoch || (L.sap_prec == 2 && G.swb64_version != G.swb_version))
    {
       if (assign_swb(want))
       {
@@ -1568,6 +1727,13 @@ bool sap_prepare(int isolv, const char *where, bool quiet)
       }
    }
    if (L.sap_prec == 1 && G.bvol == RBV) ensure_half_fields();
+   if (L.sap_prec == 2)
+   {
+      for (int k = 0; k < 3; k++)
+         if (!G.d_h64[k]) G.d_h64[k] = (double2 *)dev_alloc(units_for(L.g.V, 12) * sizeof(double2));
+      if (!G.d_scr64) G.d_scr64 = (double2 *)dev_alloc((size_t)G.nbh * 3 * G.Ub * sizeof(double2));
+      if (!G.d_h64[0] || !G.d_h64[1] || !G.d_h64[2] || !G.d_scr64 || !G.d_swb64) return false;
+   }
    return true;
 }
 
@@ -1614,9 +1780,9 @@ extern "C" void oqh_set_sap_graph(int on)
 
 extern "C" void oqh_set_sap_prec(int prec)
 {
-   if (prec != 0 && prec != 1)
+   if (prec != 0 && prec != 1 && prec != 2)
    {
-      fail("oqh_set_sap_prec", "the precision must be 0 (FP32) or 1 (FP16 storage)");
+      fail("oqh_set_sap_prec", "the precision must be 0 (FP32), 1 (FP16 storage) or 2 (FP64)");
       return;
    }
    lib().sap_prec = prec;

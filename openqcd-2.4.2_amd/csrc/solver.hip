@@ -21,6 +21,7 @@ namespace oqh {
 bool sap_prepare(int isolv, const char *where, bool quiet);
 bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, bool want_res);
 void sap_mixed_residual(float2 *r);
+bool sap_mop64(double mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, float2 *chi);
 }
 
 using namespace oqh;
@@ -222,6 +223,17 @@ static void Mop(int k, spinor *rho, spinor *phi, spinor *chi)
          Dw((float)mud, phi, chi);
          return;
       }
+   }
+   if (L.sap_prec == 2 && dev && sap_mop64(mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, r, p, c))
+   {
+      // FP64 variant: the Schwarz cycles on FP64 fields
+      if (nn_use)
+      {
+         diff_s2s(vt, 2, rho, chi);  // the residual of the cycles
+         nn_correct(c, p);
+         Dw((float)mud, phi, chi);
+      }
+      return;
    }
    set_s2zero(vt, 2, phi);
    assign_s2s(vt, 2, rho, chi);

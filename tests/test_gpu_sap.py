@@ -263,3 +263,42 @@ def test_fp16_preconditioner_graph(setup):
         print("graph status", got[1], "eager status", ref[1])
         assert got[1] == ref[1] and got[2] == ref[2]
         assert np.array_equal(got[0], ref[0])
+
+
+@pytest.mark.gpu
+def test_fp64_variant(setup):
+    """The FP64 Schwarz cycle (generic block solver on double fields, FP64
+    links and SW matrices) agrees with the FP32 cycle to single-precision
+    rounding, and SAP+GCR with it reaches the requested residual."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    out = {}
+    for prec in (0, 2):
+        lib.oqh_set_sap_prec(prec)
+        psi, rho = lat.alloc_s(), lat.alloc_s()
+        lat.put_s(psi, np.zeros_like(s32))
+        lat.put_s(rho, s32)
+        lib.sap(c["mu"], c["isolv"], 4, psi, rho)
+        out[prec] = (lat.get_s(psi).astype(np.float64), lat.get_s(rho).astype(np.float64))
+        lat.free(psi)
+        lat.free(rho)
+    rel = np.linalg.norm(out[2][0] - out[0][0]) / np.linalg.norm(out[0][0])
+    print("fp64 vs fp32 SAP cycle", rel)
+    assert rel < 1e-4
+    # SAP invariant rho = eta - D psi, up to the FP32 rounding of the exported fields
+    o.assign_ud2u()
+    o.assign_swd2sw()
+    d = o.Dw(np.float32(c["mu"]), out[2][0].astype(np.float32))
+    e64 = s32.astype(np.float64)
+    inv = np.linalg.norm(e64 - d - out[2][1]) / np.linalg.norm(e64)
+    print("fp64 invariant", inv)
+    assert inv < 1e-5
+    e, p = lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(e, eta)
+    ifail, status = C.c_int(), C.c_int()
+    lib.sap_gcr(16, 256, 0, 1e-10, c["mu"], e, p, C.byref(ifail), C.byref(status))
+    o.sw_term(3)
+    res = np.linalg.norm(eta - o.Dw_dble(c["mu"], lat.get_sd(p).copy())) / np.linalg.norm(eta)
+    print("fp64 SAP+GCR status", status.value, "true residual", res)
+    assert ifail.value == 0 and res <= 1.2e-10
+    lib.oqh_set_sap_prec(0)

@@ -1,5 +1,6 @@
 """SAP+GCR on a 16^4 lattice (periodic bc) for both block solvers, FP32 and
-FP16 storage, both forms of modified Gram-Schmidt in fgcr, judged by
+FP16 storage and FP64, 4^4 and 8^4 blocks, both forms of modified Gram-Schmidt
+in fgcr, judged by
 the double-precision true residual |eta - Dw_dble psi| / |eta| computed on
 the device (devel/sap/check3.c:205-249).  The oracle is not run at this
 size."""
@@ -30,7 +31,8 @@ def test_sap_gcr_16_4():
         for isolv, bs, prec, orth in ((0, (4, 4, 4, 4), 0, 1), (1, (8, 4, 4, 4), 0, 1),
                                       (1, (4, 4, 4, 4), 0, 1), (1, (4, 4, 4, 4), 0, 0),
                                       (1, (4, 4, 4, 4), 1, 1), (1, (4, 4, 4, 4), 1, 0),
-                                      (0, (4, 4, 4, 4), 1, 1)):
+                                      (0, (4, 4, 4, 4), 1, 1), (1, (4, 4, 4, 4), 2, 1),
+                                      (1, (8, 8, 8, 8), 2, 1), (1, (8, 8, 8, 8), 0, 1)):
             b = np.array(bs, np.int32)
             lib.oqh_set_sap_parms(b.ctypes.data_as(P), isolv, 4, 5)
             lib.oqh_set_sap_prec(prec)
