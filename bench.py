@@ -263,6 +263,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only the solver section (training, reference solve, variants): "
+                         "prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -271,6 +274,10 @@ def main():
 
     import numpy as np
     import pyoqh
+
+    if args.solve_only:
+        print(json.dumps({"solve": solve_times(args)}))
+        return
 
     lat = pyoqh.Lattice(L, LTRD, bc=0, cF=(1.0, 1.0), m0=-0.3, csw=1.77)
     V = lat.V
