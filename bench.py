@@ -230,6 +230,53 @@ def solve_times(args):
     return out
 
 
+def uneven_batch():
+    """The uneven batch: 16 independent solves on lattices 8^4, 12^4, 16^4 and
+    24^4 (4 each; periodic bc with the quark phase, csw=1.77, m0=-0.35 as the
+    solver workload), Haar links from numpy seeds 10-25 flowed by 50 Euler
+    steps of 0.01 on the device, Gaussian sources; the SAP block size of each lattice is chosen by the autotuner
+    (isolv=1, nmr=4, ncy=5; 12^4 admits only 6^4), FGCR nkv=24 to 1e-10.  The
+    lattices run one after another (the library holds one geometry)."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    rec = {"lattices": [], "workload": "16 solves, 4 each of 8^4, 12^4, 16^4, 24^4, periodic bc, "
+                                       "csw=1.77, m0=-0.35, flow 50 x 0.01, autotuned SAP blocks, res 1e-10"}
+    t_all = time.perf_counter()
+    for k, n in enumerate([8] * 4 + [12] * 4 + [16] * 4 + [24] * 4):
+        lat = pyoqh.Lattice((n,) * 4, None, bc=3, m0=-0.35, csw=1.77)
+        try:
+            lib = lat.lib
+            lib.sap_gcr.restype = C.c_double
+            lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+            rng = np.random.default_rng(10 + k)
+            lat.import_ud(pyoqh.haar_su3(4 * lat.V, rng))
+            lib.fwd_euler(50, 0.01)
+            lib.set_ud_phase()
+            e, p = lat.alloc_sd(), lat.alloc_sd()
+            lat.put_sd(e, rng.standard_normal((lat.V, 24)))
+            lat.sync()
+            t0 = time.perf_counter()
+            bs = np.zeros(4, np.int32)
+            lib.oqh_sap_autotune(1, 4, 5, 0, bs.ctypes.data_as(P))
+            t1 = time.perf_counter()
+            ifail, status = C.c_int(), C.c_int()
+            lib.sap_gcr(24, 1024, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+            lat.sync()
+            t2 = time.perf_counter()
+            rec["lattices"].append({"L": n, "seed": 10 + k, "block": [int(b) for b in bs],
+                                    "autotune_s": round(t1 - t0, 3), "solve_s": round(t2 - t1, 3),
+                                    "status": status.value, "ifail": ifail.value})
+        finally:
+            lat.close()
+    rec["total_s"] = round(time.perf_counter() - t_all, 2)
+    rec["solve_s"] = round(sum(x["solve_s"] for x in rec["lattices"]), 3)
+    log("bench: uneven batch, 16 solves in %.2f s (solves %.2f s)" % (rec["total_s"], rec["solve_s"]))
+    return rec
+
+
 def cpu_baseline(V):
     """Reference CPU Dw_dble (oracle/_ref/ref_64x32x32x32_avx: the
     reference's modules built with -DAVX -DFMA3, OpenMP, 16 threads)."""
@@ -276,7 +323,7 @@ def main():
     import pyoqh
 
     if args.solve_only:
-        print(json.dumps({"solve": solve_times(args)}))
+        print(json.dumps({"solve": solve_times(args), "uneven_batch": uneven_batch()}))
         return
 
     lat = pyoqh.Lattice(L, LTRD, bc=0, cF=(1.0, 1.0), m0=-0.3, csw=1.77)
@@ -335,6 +382,7 @@ def main():
     copy_bw = copy_bandwidth(lib)
     lat.close()
     solve = None if args.no_solve else solve_times(args)
+    batch = None if args.no_solve else uneven_batch()
     cpu = None if args.no_cpu else cpu_baseline(V)
 
     out = {
@@ -360,6 +408,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,
         "solve": solve,
+        "uneven_batch": batch,
     }
     print(json.dumps(out))
 

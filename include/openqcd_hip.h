@@ -92,6 +92,14 @@ void oqh_set_sap_prec(int prec);
    default), or launched kernel by kernel (0); no counterpart in the
    reference, the results are the same */
 void oqh_set_sap_graph(int on);
+/* SAP block-size autotuner (no counterpart in the reference): among the
+   legal block sizes of sap_parms.c (even, >= 4, an even number of blocks per
+   axis; here with equal spatial sizes and at most maxvol points, 4096 if
+   maxvol <= 0) picks the one with the least time per e-fold of residual
+   reduction in ncy Schwarz cycles of the current precision variant, sets it
+   with the given isolv, nmr, ncy and returns that time in seconds (bs_out:
+   the block size); 0 on failure. */
+double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out);
 /* orthogonalisation of the Krylov vectors in fgcr on device fields (no
    counterpart in the reference): 1 = modified Gram-Schmidt in its inverse
    compact WY form, one pass for all scalar products (with the Gram entries

@@ -28,6 +28,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
+#include <vector>
 
 #include <hip/hip_fp16.h>
 
@@ -1768,6 +1770,68 @@ extern "C" void sap(float mu, int isolv, int nmr, spinor *psi, spinor *eta)
    finish(r, F32, V, true);
 }
 
+// ---------------------------------------------------------------------------
+// block-size autotuner (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+
+// reproducible test field: uniform values in [-1, 1) from a hash of the index
+__global__ void k_hash_fill(float2 *f, size_t n, unsigned long long seed)
+{
+   size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (k >= n) return;
+   unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (k + 1);
+   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+   z ^= z >> 31;
+   const float a = (float)(z & 0xffffff) * (1.0f / 8388608.0f) - 1.0f;
+   const float b = (float)((z >> 24) & 0xffffff) * (1.0f / 8388608.0f) - 1.0f;
+   f[k] = make_float2(a, b);
+}
+
+__global__ void k_sq_partial(const float2 *f, size_t n, double *part)
+{
+   __shared__ double sh[256];
+   double s = 0.0;
+   for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+      s += (double)f[k].x * f[k].x + (double)f[k].y * f[k].y;
+   sh[threadIdx.x] = s;
+   __syncthreads();
+   for (int m = 128; m > 0; m >>= 1)
+   {
+      if (threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+static double field_norm(const float2 *f, size_t n, double *d_part, double *h_part)
+{
+   Lib &L = lib();
+   k_sq_partial<<<256, 256, 0, L.stream>>>(f, n, d_part);
+   HIP_OK(hipMemcpyAsync(h_part, d_part, 256 * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   double s = 0.0;
+   for (int b = 0; b < 256; b++) s += h_part[b];
+   return sqrt(s);
+}
+
+// legal block sizes (sap_parms.c / README.block: even, >= 4, dividing the
+// lattice with an even number of blocks per axis), spatially isotropic, at
+// most maxvol points
+static std::vector<std::array<int, 4>> block_candidates(const int *Lx, int maxvol)
+{
+   std::vector<std::array<int, 4>> out;
+   auto legal = [](int l, int b) { return b >= 4 && b % 2 == 0 && l % b == 0 && (l / b) % 2 == 0; };
+   for (int b0 = 4; b0 <= Lx[0]; b0 += 2)
+      for (int bs = 4; bs <= std::min(Lx[1], std::min(Lx[2], Lx[3])); bs += 2)
+      {
+         if (!legal(Lx[0], b0) || !legal(Lx[1], bs) || !legal(Lx[2], bs) || !legal(Lx[3], bs)) continue;
+         if ((long)b0 * bs * bs * bs > maxvol) continue;
+         out.push_back({b0, bs, bs, bs});
+      }
+   return out;
+}
+
 extern "C" void oqh_set_sap_graph(int on)
 {
    if (on != 0 && on != 1)
@@ -1929,3 +1993,85 @@ extern "C" void oqh_get_imb(int n, int *imb)
    if (!build_grid("oqh_get_imb")) return;
    memcpy(imb, G.imb.data() + (size_t)n * G.bvol, sizeof(int) * G.bvol);
 }
+
+// Chooses the SAP block size with the least time per e-fold of residual
+// reduction: for every candidate, ncy Schwarz cycles (current precision
+// variant) on a reproducible test residual, timed with HIP events after one
+// warm-up cycle.  Sets the block size of the SAP parameters (isolv, nmr, ncy
+// as given) and returns the best time per e-fold in seconds (0 on failure).
+extern "C" double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out)
+{
+   const char *where = "oqh_sap_autotune";
+   Lib &L = lib();
+   if (!require_geometry(where)) return 0.0;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return 0.0;
+   }
+   if (maxvol <= 0) maxvol = 4096;
+   auto cand = block_candidates(L.g.L, maxvol);
+   if (cand.empty())
+   {
+      fail(where, "no legal block size");
+      return 0.0;
+   }
+   const size_t nu = units_for(L.g.V, 12);
+   float2 *psi = (float2 *)dev_alloc(nu * sizeof(float2));
+   float2 *rho = (float2 *)dev_alloc(nu * sizeof(float2));
+   double *d_part = (double *)dev_alloc(256 * sizeof(double));
+   double *h_part = nullptr;
+   HIP_OK(hipHostMalloc((void **)&h_part, 256 * sizeof(double), 0));
+   hipEvent_t e0, e1;
+   HIP_OK(hipEventCreate(&e0));
+   HIP_OK(hipEventCreate(&e1));
+   double best = 0.0;
+   std::array<int, 4> bbest = cand[0];
+   const int keep[4] = {L.sap_bs[0], L.sap_bs[1], L.sap_bs[2], L.sap_bs[3]};
+   for (const auto &bs : cand)
+   {
+      memcpy(L.sap_bs, bs.data(), sizeof(L.sap_bs));
+      L.sap_isolv = isolv;
+      L.sap_nmr = nmr;
+      L.sap_ncy = ncy;
+      if (!psi || !rho || !sap_prepare(isolv, where, true)) continue;
+      const unsigned nbk = (unsigned)((nu + 255) / 256);
+      // warm-up cycle
+      HIP_OK(hipMemsetAsync(psi, 0, nu * sizeof(float2), L.stream));
+      k_hash_fill<<<nbk, 256, 0, L.stream>>>(rho, nu, 12345);
+      sap_device((float)0.0f, isolv, nmr, psi, rho);
+      HIP_OK(hipMemsetAsync(psi, 0, nu * sizeof(float2), L.stream));
+      k_hash_fill<<<nbk, 256, 0, L.stream>>>(rho, nu, 12345);
+      const double r0 = field_norm(rho, nu, d_part, h_part);
+      HIP_OK(hipEventRecord(e0, L.stream));
+      for (int n = 0; n < ncy; n++) sap_device(0.0f, isolv, nmr, psi, rho);
+      HIP_OK(hipEventRecord(e1, L.stream));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      const double r1 = field_norm(rho, nu, d_part, h_part);
+      if (!(r1 > 0.0) || r1 >= r0) continue;
+      const double score = 1e-3 * ms / log(r0 / r1);
+      if (best == 0.0 || score < best)
+      {
+         best = score;
+         bbest = bs;
+      }
+   }
+   HIP_OK(hipEventDestroy(e0));
+   HIP_OK(hipEventDestroy(e1));
+   HIP_OK(hipHostFree(h_part));
+   dev_free(psi);
+   dev_free(rho);
+   dev_free(d_part);
+   if (best == 0.0)
+   {
+      memcpy(L.sap_bs, keep, sizeof(L.sap_bs));
+      fail(where, "no block size reduced the residual");
+      return 0.0;
+   }
+   memcpy(L.sap_bs, bbest.data(), sizeof(L.sap_bs));
+   for (int mu = 0; mu < 4; mu++) bs_out[mu] = bbest[mu];
+   return best;
+}
+
