@@ -15,8 +15,8 @@
 // approximation of D^-1 applied to the residual that the Schwarz cycles
 // leave.
 //
-// k_ptc: a work-group takes 128 points.  For each direction p its threads
-// transport the input channels of the 128 neighbours with the FP32 links
+// k_ptc: a work-group takes 128 points.  For two directions p at a time its
+// threads transport the input channels of the neighbours with the FP32 links
 // and store them as BF16 in LDS (a K x N matrix, K = real and imaginary
 // parts of the C_in channels, N = 3 colours x 128 points); the channel
 // mixing is then a real GEMM [2 C_out x K] x [K x N] on the matrix cores
@@ -87,21 +87,25 @@ __device__ inline void ptc_transport(const float2 *in, int y, int i, int p, cons
       su3_mul<float2, false>(um, v, w);
 }
 
-// out (+)= layer(in); wp: packed BF16 weights [9][32][kpad(CI)]
+// out (+)= layer(in); wp: packed BF16 weights [9][32][kpad(CI)].  Two
+// directions per phase (five phases): the two halves of the work-group
+// transport all input channels of their point for direction 2q and 2q+1,
+// so every link is loaded once, and one barrier pair serves two directions.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
 template <int CI, int CO>
 __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, float2 *__restrict__ out,
                                              const __bf16 *__restrict__ wp, const float2 *__restrict__ u,
                                              const int *__restrict__ nbr, int V, int accum)
 {
    constexpr int KP = kpad(CI);
-   constexpr int LD = KP + 8;  // LDS row stride (bf16)
-   constexpr int CH = CI / 2;  // channels gathered per thread
+   constexpr int LD = 2 * KP + 8;  // LDS row stride (bf16): two direction blocks
    static_assert(2 * CO <= 32 && CI % 2 == 0, "channel counts");
    __shared__ __attribute__((aligned(16))) __bf16 Bt[3 * PNS * LD];
    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
    const int x0 = blockIdx.x * PNS, h = V / 2;
    for (int q = t; q < 3 * PNS; q += PWG)
-      for (int k = 2 * CI; k < KP; k++) Bt[q * LD + k] = (__bf16)0.0f;
+      for (int k = 2 * CI; k < KP; k++) Bt[q * LD + k] = Bt[q * LD + KP + k] = (__bf16)0.0f;
    f32x16 acc[3];
 #pragma unroll
    for (int tt = 0; tt < 3; tt++)
@@ -109,9 +113,11 @@ __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, floa
       for (int j = 0; j < 16; j++) acc[tt][j] = 0.0f;
    const int s = t & (PNS - 1), hh = t >> 7;
    const int x = x0 + s;
-   for (int p = 0; p < NDIR; p++)
+   for (int ph = 0; ph < (NDIR + 1) / 2; ph++)
    {
-      if (x < V)
+      const int p = 2 * ph + hh;
+      __bf16 *brow = Bt + s * LD + hh * KP;
+      if (x < V && p < NDIR)
       {
          const int y = ptc_nbr(nbr, V, x, p);
          float2 um[9];
@@ -122,31 +128,43 @@ __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, floa
 #pragma unroll
             for (int k = 0; k < 9; k++) um[k] = up[k * SB];
          }
+         // all loads first (independent), then the products
+         float2 v[CI][3];
 #pragma unroll
-         for (int c = 0; c < CH; c++)
+         for (int i = 0; i < CI; i++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) v[i][a] = in[unit_index(y, 3 * i + a, 3 * CI)];
+#pragma unroll
+         for (int i = 0; i < CI; i++)
          {
-            const int i = hh * CH + c;
             float2 w[3];
-            ptc_transport<CI>(in, y, i, p, um, dag, w);
+            if (p == 0)
+            {
+#pragma unroll
+               for (int a = 0; a < 3; a++) w[a] = v[i][a];
+            }
+            else if (dag)
+               su3_mul<float2, true>(um, v[i], w);
+            else
+               su3_mul<float2, false>(um, v[i], w);
 #pragma unroll
             for (int a = 0; a < 3; a++)
             {
-               __bf16 *b = Bt + (a * PNS + s) * LD + 2 * i;
-               b[0] = (__bf16)w[a].x;
-               b[1] = (__bf16)w[a].y;
+               bf16x2 b;
+               b.x = (__bf16)w[a].x;
+               b.y = (__bf16)w[a].y;
+               *(bf16x2 *)(brow + a * PNS * LD + 2 * i) = b;
             }
          }
       }
       else
       {
+         bf16x2 z;
+         z.x = z.y = (__bf16)0.0f;
 #pragma unroll
-         for (int c = 0; c < CH; c++)
+         for (int i = 0; i < CI; i++)
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-            {
-               __bf16 *b = Bt + (a * PNS + s) * LD + 2 * (hh * CH + c);
-               b[0] = b[1] = (__bf16)0.0f;
-            }
+            for (int a = 0; a < 3; a++) *(bf16x2 *)(brow + a * PNS * LD + 2 * i) = z;
       }
       __syncthreads();
 #pragma unroll
@@ -154,11 +172,17 @@ __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, floa
       {
          const int n = (wv + 4 * tt) * 32 + (lane & 31);
 #pragma unroll
-         for (int k0 = 0; k0 < KP; k0 += 16)
+         for (int d = 0; d < 2; d++)
          {
-            const bf16x8 a = *(const bf16x8 *)(wp + (size_t)(p * 32 + (lane & 31)) * KP + k0 + 8 * (lane >> 5));
-            const bf16x8 b = *(const bf16x8 *)(Bt + n * LD + k0 + 8 * (lane >> 5));
-            acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tt], 0, 0, 0);
+            const int pd = 2 * ph + d;
+            if (pd >= NDIR) break;
+#pragma unroll
+            for (int k0 = 0; k0 < KP; k0 += 16)
+            {
+               const bf16x8 a = *(const bf16x8 *)(wp + (size_t)(pd * 32 + (lane & 31)) * KP + k0 + 8 * (lane >> 5));
+               const bf16x8 b = *(const bf16x8 *)(Bt + n * LD + d * KP + k0 + 8 * (lane >> 5));
+               acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[tt], 0, 0, 0);
+            }
          }
       }
       __syncthreads();
