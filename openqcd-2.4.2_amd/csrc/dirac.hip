@@ -21,6 +21,7 @@
 // in input and output of Dw and contribute no hopping terms to even points
 // (Dw_dble.c:1425-1456); the neighbour table carries that flag in bit 31.
 #include <cstdio>
+#include <cstring>
 
 #include "dirac_dev.h"
 
@@ -43,6 +44,7 @@ template <typename T> struct DArgs
    int p0, np;     // parity ops: first point and number of points
    T mu, muo, coef;
    int masked;     // 1 if boundary points exist (bc != 3)
+   const unsigned char *lsign;  // per odd point, bit l: link l is -SU(3) (rebuilt links)
 };
 
 template <typename T, int MU>
@@ -299,7 +301,7 @@ __device__ inline int tile_nbr(const DArgs<T> &a, const TileNbr<T> &nb, int e, i
 // global a.s) to acc_o; XCH: the odd lane passes the term of its even
 // neighbour through X, and even lanes gather the terms of neighbours outside
 // the tile from the odd-point field zsrc
-template <typename T, int MU, bool FWD, int TP, bool OWN = true, bool XCH = true>
+template <typename T, int MU, bool FWD, int TP, bool OWN = true, bool XCH = true, bool REC = false>
 __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, const TileNbr<T> &nb,
                                   typename cplx_of<T>::t (*S)[TP], typename cplx_of<T>::t (*X)[TP],
                                   const typename cplx_of<T>::t *so, typename cplx_of<T>::t *acc_o,
@@ -316,8 +318,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
       bool yin = (unsigned)ly < (unsigned)TP;
       const C *up = link_ptr(a.u, e, FWD ? 2 * MU : 2 * MU + 1);
       C u[9];
-#pragma unroll
-      for (int k = 0; k < 9; k++) u[k] = ld_stream<OQH_NT_LINK>(up + k * SB);
+      load_link<C, REC, OQH_NT_LINK>(up, REC && ((a.lsign[e] >> (FWD ? 2 * MU : 2 * MU + 1)) & 1), u);
       C h0[3], h1[3], c0[3], c1[3];
       if (OWN && !(a.masked && ny < 0))
       {
@@ -370,9 +371,14 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
          }
          else
          {
-            C sz[12];
+            C sz[12], u[9], h0[3], h1[3], c0[3], c1[3];
             load_spinor(zsrc, z, sz);
-            hop_one<C, MU, !FWD>(sz, link_ptr(a.u, z - a.h, FWD ? 2 * MU : 2 * MU + 1), acc_e);
+            const int l = FWD ? 2 * MU : 2 * MU + 1;
+            load_link<C, REC>(link_ptr(a.u, z - a.h, l), REC && ((a.lsign[z - a.h] >> l) & 1), u);
+            spin_project<C, MU, !FWD>(sz, h0, h1);
+            su3_mul<C, FWD>(u, h0, c0);
+            su3_mul<C, FWD>(u, h1, c1);
+            spin_expand_add<C, MU, !FWD>(c0, c1, acc_e);
          }
       }
    }
@@ -382,7 +388,7 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
    if (XBUF == 1) __syncthreads();
 }
 
-template <typename T, int TP = TileSize<T>::n>
+template <typename T, int TP = TileSize<T>::n, bool REC = false>
 __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
@@ -444,14 +450,14 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 #endif
    }
    __syncthreads();
-   tile_phase<T, 0, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
-   tile_phase<T, 0, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
-   tile_phase<T, 1, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
-   tile_phase<T, 1, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
-   tile_phase<T, 2, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
-   tile_phase<T, 2, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
-   tile_phase<T, 3, true, TP>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
-   tile_phase<T, 3, false, TP>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 0, true, TP, true, true, REC>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 0, false, TP, true, true, REC>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 1, true, TP, true, true, REC>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 1, false, TP, true, true, REC>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 2, true, TP, true, true, REC>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 2, false, TP, true, true, REC>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
+   tile_phase<T, 3, true, TP, true, true, REC>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
+   tile_phase<T, 3, false, TP, true, true, REC>(a, t, e0, valid, nb, S, X[XBUF - 1], so, acc_o, acc_e, a.s);
    if (!valid) return;
    C out[12];
    // boundary points of open bc are zeroed in output and input
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 //   pass 2 (even points): r_e = (A_ee + i mu g5) s_e + 1/2 sum of hops from
 //                         t_o, each link loaded once by its odd lane, which
 //                         passes the even neighbour's term through LDS.
-template <typename T, int TP = TileSize<T>::n>
+template <typename T, int TP = TileSize<T>::n, bool REC = false>
 __global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
@@ -533,14 +539,14 @@ __global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
       for (int c = 0; c < 12; c++) S[c][t] = se[c];
    }
    __syncthreads();
-   tile_phase<T, 0, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 0, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 1, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 1, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 2, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 2, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 3, true, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
-   tile_phase<T, 3, false, TP, true, false>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 0, true, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 0, false, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 1, true, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 1, false, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 2, true, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 2, false, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 3, true, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
+   tile_phase<T, 3, false, TP, true, false, REC>(a, t, e0, valid, nb, S, nullptr, nullptr, acc, nullptr, nullptr);
    if (!valid) return;
    if (ob)
    {
@@ -554,7 +560,7 @@ __global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
    store_spinor(a.t, o, out);
 }
 
-template <typename T, int TP = TileSize<T>::n>
+template <typename T, int TP = TileSize<T>::n, bool REC = false>
 __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
@@ -586,14 +592,14 @@ __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
       if (!(a.masked && a.bnd[o])) load_spinor((const C *)a.t, o, to);
    }
    const C *tz = a.t;
-   tile_phase<T, 0, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
-   tile_phase<T, 0, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
-   tile_phase<T, 1, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
-   tile_phase<T, 1, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
-   tile_phase<T, 2, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
-   tile_phase<T, 2, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
-   tile_phase<T, 3, true, TP, false, true>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
-   tile_phase<T, 3, false, TP, false, true>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 0, true, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 0, false, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 1, true, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 1, false, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 2, true, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 2, false, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
+   tile_phase<T, 3, true, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[0], to, nullptr, acc, tz);
+   tile_phase<T, 3, false, TP, false, true, REC>(a, t, e0, valid, nb, nullptr, X[XBUF - 1], to, nullptr, acc, tz);
    if (!valid) return;
    if (eb)
    {
@@ -800,6 +806,68 @@ __global__ __launch_bounds__(256) void k_zero_bnd(typename cplx_of<T>::t *s, con
 // host side
 // ---------------------------------------------------------------------------
 
+#ifndef OQH_LINK_REC
+#define OQH_LINK_REC 1
+#endif
+
+// per odd point j and link l: compares row 2 with +-conj(row0 x row1); sets
+// bit l of sign[j] for the minus sign and raises *dev to the largest
+// deviation (as the bits of a non-negative double)
+__global__ void k_link_classes(const double2 *__restrict__ ud, int h, unsigned char *sign, unsigned long long *dev)
+{
+   const int j = blockIdx.x * 256 + threadIdx.x;
+   if (j >= h) return;
+   unsigned char bits = 0;
+   double mx = 0.0;
+   for (int l = 0; l < 8; l++)
+   {
+      const double2 *up = link_ptr(ud, j, l);
+      double2 u[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[k] = up[k * SB];
+      double2 c[3] = {csub(cmul(u[1], u[5]), cmul(u[2], u[4])), csub(cmul(u[2], u[3]), cmul(u[0], u[5])),
+                      csub(cmul(u[0], u[4]), cmul(u[1], u[3]))};
+      double dp = 0.0, dm = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+      {
+         // row 2 against conj(c) and -conj(c)
+         dp = fmax(dp, fmax(fabs(u[6 + k].x - c[k].x), fabs(u[6 + k].y + c[k].y)));
+         dm = fmax(dm, fmax(fabs(u[6 + k].x + c[k].x), fabs(u[6 + k].y - c[k].y)));
+      }
+      if (dm < dp) bits |= (unsigned char)(1u << l);
+      mx = fmax(mx, fmin(dp, dm));
+   }
+   sign[j] = bits;
+   atomicMax(dev, (unsigned long long)__double_as_longlong(mx));
+}
+
+// link classes of the current FP64 field (once per field change): the
+// operators rebuild row 2 of every link if all deviations are below 1e-13
+static void ensure_link_classes()
+{
+   Lib &L = lib();
+   if (!OQH_LINK_REC || L.rec_version == L.ud_version) return;
+   const int h = L.g.V / 2;
+   static int nalloc = 0;
+   if (!L.d_lsign || nalloc != h)
+   {
+      dev_free(L.d_lsign);
+      L.d_lsign = (unsigned char *)dev_alloc((size_t)h + 1);
+      nalloc = h;
+   }
+   unsigned long long *d_dev = (unsigned long long *)tmp_buffer(sizeof(unsigned long long));
+   HIP_OK(hipMemsetAsync(d_dev, 0, sizeof(unsigned long long), L.stream));
+   k_link_classes<<<(h + 255) / 256, 256, 0, L.stream>>>(L.d_ud, h, L.d_lsign, d_dev);
+   unsigned long long bits = 0;
+   HIP_OK(hipMemcpyAsync(&bits, d_dev, sizeof(bits), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   double dev;
+   memcpy(&dev, &bits, sizeof(dev));
+   L.ud_rec = dev < 1e-13;
+   L.rec_version = L.ud_version;
+}
+
 template <typename T> static DArgs<T> base_args()
 {
    Lib &L = lib();
@@ -811,6 +879,8 @@ template <typename T> static DArgs<T> base_args()
    a.masked = (L.bc != 3);
    a.u = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_ud : (const void *)L.d_u);
    a.sw = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_swd : (const void *)L.d_sw);
+   // FP64 links rebuilt from two rows when the whole field allows it
+   a.lsign = (OQH_LINK_REC && sizeof(T) == 8 && L.ud_rec && L.rec_version == L.ud_version) ? L.d_lsign : nullptr;
    return a;
 }
 
@@ -822,7 +892,11 @@ template <typename T, int OP> static void launch(const DArgs<T> &a)
    {
       constexpr int TP = TileSize<T>::n;
       unsigned nt = (unsigned)((a.h + TP - 1) / TP);
-      if (nt) k_dw_tile<T><<<nt, TP, 0, L.stream>>>(a);
+      if (!nt) return;
+      if (a.lsign)
+         k_dw_tile<T, TP, true><<<nt, TP, 0, L.stream>>>(a);
+      else
+         k_dw_tile<T, TP, false><<<nt, TP, 0, L.stream>>>(a);
       return;
    }
 #endif
@@ -852,6 +926,7 @@ static bool prepare(const char *where, int prec)
       return false;
    }
    if (!L.swd_valid) ensure_swd(0, 0);
+   if (prec == F64) ensure_link_classes();
    if (prec == F32)
    {
       ensure_u();
@@ -931,10 +1006,16 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
          constexpr int TPn = TileSize<T>::n;
          unsigned nt = (unsigned)((h + TPn - 1) / TPn);
          a.coef = (T)-0.5;
-         k_hat1_tile<T><<<nt, TPn, 0, L.stream>>>(a);
+         if (a.lsign)
+            k_hat1_tile<T, TileSize<T>::n, true><<<nt, TPn, 0, L.stream>>>(a);
+         else
+            k_hat1_tile<T, TileSize<T>::n, false><<<nt, TPn, 0, L.stream>>>(a);
          a.coef = (T)0.5;
          a.mu = mu;
-         k_hat2_tile<T><<<nt, TPn, 0, L.stream>>>(a);
+         if (a.lsign)
+            k_hat2_tile<T, TileSize<T>::n, true><<<nt, TPn, 0, L.stream>>>(a);
+         else
+            k_hat2_tile<T, TileSize<T>::n, false><<<nt, TPn, 0, L.stream>>>(a);
       }
 #else
       a.coef = (T)-0.5;

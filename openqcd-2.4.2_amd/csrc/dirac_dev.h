@@ -142,6 +142,30 @@ __device__ inline void hop_one(const C *sp, const C *up, C *acc)
    spin_expand_add<C, MU, FWD>(c0, c1, acc);
 }
 
+// link U at up into registers; REC: only rows 0 and 1 are read and row 2 is
+// rebuilt as conj(row0 x row1), negated if neg (the links of an SU(3) field,
+// some multiplied by -1 through the quark phase)
+template <typename C, bool REC, bool NT = false>
+__device__ inline void load_link(const C *up, bool neg, C *u)
+{
+   if (!REC)
+   {
+#pragma unroll
+      for (int k = 0; k < 9; k++) u[k] = ld_stream<NT>(up + k * SB);
+      return;
+   }
+#pragma unroll
+   for (int k = 0; k < 6; k++) u[k] = ld_stream<NT>(up + k * SB);
+   C c0 = csub(cmul(u[1], u[5]), cmul(u[2], u[4]));
+   C c1 = csub(cmul(u[2], u[3]), cmul(u[0], u[5]));
+   C c2 = csub(cmul(u[0], u[4]), cmul(u[1], u[3]));
+   typedef decltype(C::x) R;
+   const R sg = neg ? (R)-1 : (R)1;
+   u[6] = cmk<C>(sg * c0.x, -sg * c0.y);
+   u[7] = cmk<C>(sg * c1.x, -sg * c1.y);
+   u[8] = cmk<C>(sg * c2.x, -sg * c2.y);
+}
+
 // address of link l (0..7, udfld order) stored with odd point j = y - V/2
 template <typename C> __device__ inline const C *link_ptr(const C *u, int j, int l)
 {

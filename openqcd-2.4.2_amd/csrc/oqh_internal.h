@@ -87,6 +87,10 @@ struct Lib
    bool swd_inv[2] = {false, false};
    int swd_epoch = 0;  // incremented whenever the contents of d_swd change
    int u_epoch = 0;    // incremented whenever d_u is remade
+   int ud_version = 0; // incremented whenever d_ud changes
+   int rec_version = -1;            // ud_version the link classes were made for
+   bool ud_rec = false;             // every link equals +-(rows 0,1, conj(row0 x row1))
+   unsigned char *d_lsign = nullptr; // per odd point: bit l set if link l is -SU(3)
 
    std::map<uintptr_t, Field> reg;
 

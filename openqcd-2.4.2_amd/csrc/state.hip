@@ -454,6 +454,10 @@ extern "C" void oqh_finalize(void)
    clear_ws_marks();
    L.sap_prec = 0;
    L.gcr_cgs = 1;
+   dev_free(L.d_lsign);
+   L.d_lsign = nullptr;
+   L.rec_version = -1;
+   L.ud_rec = false;
    release_nn();
    L.nn_prec = 0;
    L.sap_graph = 1;
@@ -596,6 +600,7 @@ extern "C" void oqh_import_ud(const su3_dble *ud)
    HIP_OK(hipStreamSynchronize(L.stream));
    L.ud_set = true;
    L.ud_phase = false;
+   L.ud_version++;
    L.u_valid = false;
    L.swd_valid = L.sw_valid = false;
 }

@@ -569,6 +569,7 @@ extern "C" void fwd_euler(int n, double eps)
       k_update_ud<<<nb, 256, 0, L.stream>>>(a, cc);
    }
    HIP_OK(hipStreamSynchronize(L.stream));
+   L.ud_version++;
    L.u_valid = false;
    L.swd_valid = L.sw_valid = false;
 }
@@ -582,6 +583,7 @@ extern "C" void set_ud_phase(void)
       int h = L.g.V / 2;
       k_ud_phase<<<(unsigned)((2 * (size_t)h + 255) / 256), 256, 0, L.stream>>>(L.d_ud, L.d_x0, h, L.g.L[0]);
       HIP_OK(hipStreamSynchronize(L.stream));
+      L.ud_version++;
       L.u_valid = false;
       L.swd_valid = L.sw_valid = false;
    }
@@ -598,6 +600,7 @@ extern "C" void unset_ud_phase(void)
       int h = L.g.V / 2;
       k_ud_phase<<<(unsigned)((2 * (size_t)h + 255) / 256), 256, 0, L.stream>>>(L.d_ud, L.d_x0, h, L.g.L[0]);
       HIP_OK(hipStreamSynchronize(L.stream));
+      L.ud_version++;
       L.u_valid = false;
       L.swd_valid = L.sw_valid = false;
    }
