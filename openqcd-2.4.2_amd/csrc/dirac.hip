@@ -879,8 +879,9 @@ template <typename T> static DArgs<T> base_args()
    a.masked = (L.bc != 3);
    a.u = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_ud : (const void *)L.d_u);
    a.sw = (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_swd : (const void *)L.d_sw);
-   // FP64 links rebuilt from two rows when the whole field allows it
-   a.lsign = (OQH_LINK_REC && sizeof(T) == 8 && L.ud_rec && L.rec_version == L.ud_version) ? L.d_lsign : nullptr;
+   // links rebuilt from two rows when the whole field allows it (the FP32
+   // links are the rounded FP64 ones: same signs)
+   a.lsign = (OQH_LINK_REC && L.ud_rec && L.rec_version == L.ud_version) ? L.d_lsign : nullptr;
    return a;
 }
 
@@ -926,7 +927,7 @@ static bool prepare(const char *where, int prec)
       return false;
    }
    if (!L.swd_valid) ensure_swd(0, 0);
-   if (prec == F64) ensure_link_classes();
+   ensure_link_classes();
    if (prec == F32)
    {
       ensure_u();

@@ -1,4 +1,4 @@
-"""The FP64 operators rebuild the third row of every link from the first two
+"""The operators rebuild the third row of every link from the first two
 (row 2 = +-conj(row0 x row1)) when the whole gauge field allows it, and read
 all nine entries otherwise.  Both paths must give the oracle's Dw_dble and
 Dwhat_dble to 1e-13 per point: on an SU(3) field, on the same field with the
