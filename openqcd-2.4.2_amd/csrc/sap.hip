@@ -826,7 +826,9 @@ __device__ inline void pull_faces(const SArgs &a, const RegBlock<S> &b, const S 
 #define OQH_SAP_WAVES32 1
 #endif
 // FP16 storage frees enough registers for two wavefronts per SIMD
-template <int ISOLV, typename S>
+// MASKED: open-bc boundary points exist (bc != 3); without them the masking
+// of boundary values is compiled out
+template <int ISOLV, typename S, bool MASKED>
 __global__ __launch_bounds__(RBV)
 __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? OQH_SAP_WAVES16 : OQH_SAP_WAVES32)))
 void k_sap_reg(SArgs a, const S *u16, const S *sw16)
@@ -842,7 +844,7 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
    const int gx = imb[x];
    const bool even = x < a.bh;
    RegBlock<S> b;
-   b.bp = a.masked && a.bnd[gx];
+   b.bp = MASKED && a.bnd[gx];
    // links and SW matrices: the FP32 fields, or their FP16 copies
    const S *ub = sizeof(S) == sizeof(float2) ? (const S *)a.u : u16;
    const S *swb = sizeof(S) == sizeof(float2) ? (const S *)a.sw : sw16;
@@ -1359,17 +1361,25 @@ static void sap_blocks(const SArgs &a0, int ic, int isolv, int pull)
    a.pull = pull;
    if (half_variant())
    {
-      if (isolv)
-         k_sap_reg<1, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+      if (isolv && a.masked)
+         k_sap_reg<1, __half2, true><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+      else if (isolv)
+         k_sap_reg<1, __half2, false><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+      else if (a.masked)
+         k_sap_reg<0, __half2, true><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
       else
-         k_sap_reg<0, __half2><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
+         k_sap_reg<0, __half2, false><<<G.nbh, RBV, 0, L.stream>>>(a, G.d_u16, G.d_swb16);
    }
    else if (reg_kernel())
    {
-      if (isolv)
-         k_sap_reg<1, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      if (isolv && a.masked)
+         k_sap_reg<1, float2, true><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else if (isolv)
+         k_sap_reg<1, float2, false><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else if (a.masked)
+         k_sap_reg<0, float2, true><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
       else
-         k_sap_reg<0, float2><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+         k_sap_reg<0, float2, false><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
    }
    else if (isolv)
       k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
