@@ -230,6 +230,72 @@ def solve_times(args):
     return out
 
 
+def multi_rhs():
+    """Several right-hand sides on one 32^3x64 field (periodic bc with the
+    quark phase, csw=1.77, m0=-0.35, Haar links of seed 2 flowed by 50 steps
+    of 0.01): the 12 spin-colour point sources at the origin and 4 Gaussian
+    sources, solved together by sap_gcr_multi (FP32 SAP on 4^4 blocks,
+    isolv=1, nmr=4, ncy=5; FGCR nkv=16 to 1e-10), against one sap_gcr call
+    per source for two of them."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    lat = pyoqh.Lattice((64, 32, 32, 32), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
+    rec = {"workload": "32^3x64 periodic, csw=1.77, m0=-0.35, flow 50 x 0.01, 12 point + 4 Gaussian "
+                       "sources, FP32 SAP 4^4 isolv=1 nmr=4 ncy=5, nkv=16, res 1e-10"}
+    try:
+        lib = lat.lib
+        lib.sap_gcr.restype = C.c_double
+        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        rng = np.random.default_rng(2)
+        V = lat.V
+        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        lib.fwd_euler(50, 0.01)
+        lib.set_ud_phase()
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        n = 16
+        es, ps = [lat.alloc_sd() for _ in range(n)], [lat.alloc_sd() for _ in range(n)]
+        for k in range(12):
+            e = np.zeros((V, 24))
+            e[0, 2 * k] = 1.0
+            lat.put_sd(es[k], e)
+        for k in range(12, 16):
+            lat.put_sd(es[k], rng.standard_normal((V, 24)))
+        rho = np.zeros(n)
+        ifail = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        lat.sync()
+        t0 = time.perf_counter()
+        lib.sap_gcr_multi(n, 16, 1024, 0, 1e-10, 0.0, (P * n)(*es), (P * n)(*ps), rho.ctypes.data_as(P),
+                          ifail.ctypes.data_as(P), status.ctypes.data_as(P))
+        lat.sync()
+        t1 = time.perf_counter()
+        rec["multi_s"] = round(t1 - t0, 3)
+        rec["status"] = [int(x) for x in status]
+        rec["ifail"] = [int(x) for x in ifail]
+        single = []
+        for k in (0, 12):
+            fi, st = C.c_int(), C.c_int()
+            lat.sync()
+            t0 = time.perf_counter()
+            lib.sap_gcr(16, 1024, 0, 1e-10, 0.0, es[k], ps[k], C.byref(fi), C.byref(st))
+            lat.sync()
+            single.append({"source": k, "seconds": round(time.perf_counter() - t0, 3), "status": st.value})
+        rec["single"] = single
+        per_vec_multi = rec["multi_s"] / max(1, int(status.sum()))
+        per_vec_single = sum(x["seconds"] for x in single) / max(1, sum(x["status"] for x in single))
+        rec["ms_per_krylov_vector_multi"] = round(1e3 * per_vec_multi, 3)
+        rec["ms_per_krylov_vector_single"] = round(1e3 * per_vec_single, 3)
+        log("bench: multi-RHS, 16 sources in %.2f s (%.2f ms per vector; single %.2f ms)" % (
+            rec["multi_s"], 1e3 * per_vec_multi, 1e3 * per_vec_single))
+    finally:
+        lat.close()
+    return rec
+
+
 def uneven_batch():
     """The uneven batch: 16 independent solves on lattices 8^4, 12^4, 16^4 and
     24^4 (4 each; periodic bc with the quark phase, csw=1.77, m0=-0.35 as the
@@ -323,7 +389,8 @@ def main():
     import pyoqh
 
     if args.solve_only:
-        print(json.dumps({"solve": solve_times(args), "uneven_batch": uneven_batch()}))
+        print(json.dumps({"solve": solve_times(args), "uneven_batch": uneven_batch(),
+                          "multi_rhs": multi_rhs()}))
         return
 
     lat = pyoqh.Lattice(L, LTRD, bc=0, cF=(1.0, 1.0), m0=-0.3, csw=1.77)
@@ -383,6 +450,7 @@ def main():
     lat.close()
     solve = None if args.no_solve else solve_times(args)
     batch = None if args.no_solve else uneven_batch()
+    mrhs = None if args.no_solve else multi_rhs()
     cpu = None if args.no_cpu else cpu_baseline(V)
 
     out = {
@@ -409,6 +477,7 @@ def main():
         "cpu_baseline": cpu,
         "solve": solve,
         "uneven_batch": batch,
+        "multi_rhs": mrhs,
     }
     print(json.dumps(out))
 

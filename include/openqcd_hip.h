@@ -100,6 +100,16 @@ void oqh_set_sap_graph(int on);
    with the given isolv, nmr, ncy and returns that time in seconds (bs_out:
    the block size); 0 on failure. */
 double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out);
+/* several right-hand sides (no counterpart in the reference): solves
+   D psi[s] = eta[s] for s < n as sap_gcr would (per source: rho_out[s],
+   ifail[s], status[s] as its return value and arguments); with FP32 SAP on
+   256-point blocks and istop = 0 the FGCR iterations of all sources advance
+   together and the Schwarz cycles of all active sources (up to 16 per
+   launch) run in the same kernel launches, consecutive work-groups taking
+   the same block for different sources; otherwise the sources are solved one
+   after the other.  Each source's result equals that of sap_gcr. */
+void sap_gcr_multi(int n, int nkv, int nmx, int istop, double res, double mu, spinor_dble **eta,
+                   spinor_dble **psi, double *rho_out, int *ifail, int *status);
 /* orthogonalisation of the Krylov vectors in fgcr on device fields (no
    counterpart in the reference): 1 = modified Gram-Schmidt in its inverse
    compact WY form, one pass for all scalar products (with the Gram entries

@@ -489,8 +489,6 @@ struct QList
 };
 
 static dd *d_mpart = nullptr;
-static float2 *d_gram = nullptr;
-static int gram_ld = 0;
 
 // partial sums of (q_j, s) and, if c != null, (q_j, c), j < nq <= NQD;
 // part[blockIdx][4 j + (0,1: with s; 2,3: with c)]
@@ -687,18 +685,13 @@ __global__ __launch_bounds__(RB) void k_scale_axpy_partial(float2 *s, float2 *r,
    }
 }
 
-void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n)
+void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n,
+                  float2 *d_gram, int gram_ld)
 {
    Lib &L = lib();
    if (!d_part) d_part = (dd *)dev_alloc(sizeof(dd) * 2 * (RMAXB + 8));
    if (!d_mpart) d_mpart = (dd *)dev_alloc(sizeof(dd) * 4 * NQD * (RMAXB + 8));
-   if (k + 1 > gram_ld)
-   {
-      dev_free(d_gram);
-      gram_ld = std::max(k + 1, 64);
-      d_gram = (float2 *)dev_alloc(sizeof(float2) * gram_ld * gram_ld);
-   }
-   if (!d_part || !d_mpart || !d_gram) return;
+   if (!d_part || !d_mpart || !d_gram || k > gram_ld) return;
    const int nb = red_blocks(n);
    // products with s and the Gram row of the newest vector q[k-1]
    for (int j0 = 0; j0 < k; j0 += NQD)

@@ -165,8 +165,10 @@ void gs_step(float2 *s, const float2 *r, const float2 *q, float2 *coef, int ic, 
 // coef[j] = d_j (the modified-GS coefficients) for j < k, s -= sum_j d_j q_j,
 // coef[ib] = (|s|, |s|^2), s /= |s|, coef[ic] = (s, rho), rho -= coef[ic] s,
 // coef[ir] = (|rho|, |rho|^2); the Gram entries of q[k-1] are formed on the
-// way (q[0..k-2] must be the vectors of the previous calls)
-void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n);
+// way into gram[(k-1) ldg + j] (q[0..k-2] must be the vectors of the
+// previous calls with the same gram; ldg >= k)
+void gs_classical(float2 *s, float2 *const *q, int k, float2 *rho, float2 *coef, int ib, int ic, int ir, int n,
+                  float2 *gram, int ldg);
 
 // SW field management (swterm.hip)
 void ensure_swd(int want_inv_e, int want_inv_o);

@@ -61,6 +61,7 @@ struct SapGrid
    float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
    __half2 *d_u16 = nullptr;             // FP16 links (FP16 variant of the solver)
    __half2 *d_swb16 = nullptr;           // FP16 block SW field
+   float2 *d_s0gm = nullptr;             // block corrections of 16 sources
    double2 *d_swb64 = nullptr;           // FP64 block SW field (FP64 variant)
    int swb64_version = -1;
    double2 *d_h64[3] = {nullptr, nullptr, nullptr};  // FP64 psi, rho, s0g
@@ -90,6 +91,11 @@ template <typename C> struct SArgsT
    float *psc, *rsc, *s0sc;
    int pz;             // psi is zero on entry (not read)
    int pull;           // add the face terms from the other colour's s0g first
+   // several right-hand sides in one launch (FP32): nsrc > 0 replaces psi,
+   // rho, s0g by the fields of source blockIdx % nsrc (block kernel) or
+   // blockIdx.y (face kernel)
+   int nsrc;
+   C *pl[16], *rl[16], *sl[16];
 };
 typedef SArgsT<float2> SArgs;   // FP32 and FP16 variants
 typedef SArgsT<double2> SArgsD; // FP64 variant
@@ -831,15 +837,26 @@ __device__ inline void pull_faces(const SArgs &a, const RegBlock<S> &b, const S 
 template <int ISOLV, typename S, bool MASKED>
 __global__ __launch_bounds__(RBV)
 __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? OQH_SAP_WAVES16 : OQH_SAP_WAVES32)))
-void k_sap_reg(SArgs a, const S *u16, const S *sw16)
+void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
 {
    __shared__ float2 LB[18 * RBV];  // hopping exchange (rb_hop)
    // FP16 variant: the Pauli matrices of the block live in LDS, and psi,
    // rho, s0g are FP16 spinors with per-site scales
    constexpr bool MLDS = sizeof(S) == 4;
    __shared__ S MS[MLDS ? 36 : 1][RBV];
+   // several sources: consecutive work-groups take the same block for
+   // different sources, so its links and SW matrices come from the cache
+   const int src = a0.nsrc ? (int)(blockIdx.x % a0.nsrc) : 0;
+   const int bi = a0.nsrc ? (int)(blockIdx.x / a0.nsrc) : (int)blockIdx.x;
+   SArgs a = a0;
+   if (a0.nsrc)
+   {
+      a.psi = a0.pl[src];
+      a.rho = a0.rl[src];
+      a.s0g = a0.sl[src];
+   }
    const int x = threadIdx.x;
-   const int n = a.n0 + blockIdx.x;
+   const int n = a.n0 + bi;
    const int *imb = a.imb + (size_t)n * RBV;
    const int gx = imb[x];
    const bool even = x < a.bh;
@@ -1018,10 +1035,16 @@ void k_sap_reg(SArgs a, const S *u16, const S *sw16)
 // neighbouring blocks, for the face points y of the blocks of one colour
 // (S = __half2: FP16 links, FP16 s0g and rho with per-site scales)
 template <typename S>
-__global__ __launch_bounds__(256) void k_sap_faces(SArgs a, const int *face, const unsigned char *fmask, int nf,
+__global__ __launch_bounds__(256) void k_sap_faces(SArgs a0, const int *face, const unsigned char *fmask, int nf,
                                                    const S *u16)
 {
    constexpr bool H = sizeof(S) == 4;
+   SArgs a = a0;
+   if (a0.nsrc)
+   {
+      a.rho = a0.rl[blockIdx.y];
+      a.s0g = a0.sl[blockIdx.y];
+   }
    int k = blockIdx.x * 256 + threadIdx.x;
    if (k >= nf) return;
    int y = face[k];
@@ -1150,7 +1173,7 @@ static void free_grid()
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
    dev_free(G.d_u16); dev_free(G.d_swb16);
    for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); dev_free(G.d_h64[k]); }
-   dev_free(G.d_swb64); dev_free(G.d_scr64);
+   dev_free(G.d_swb64); dev_free(G.d_scr64); dev_free(G.d_s0gm);
    G = SapGrid();
 }
 
@@ -1359,6 +1382,20 @@ static void sap_blocks(const SArgs &a0, int ic, int isolv, int pull)
    SArgs a = a0;
    a.n0 = ic ? G.nbh : 0;
    a.pull = pull;
+   if (a.nsrc)
+   {
+      // several sources: the FP32 register kernel only
+      const unsigned nb = (unsigned)(G.nbh * a.nsrc);
+      if (isolv && a.masked)
+         k_sap_reg<1, float2, true><<<nb, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else if (isolv)
+         k_sap_reg<1, float2, false><<<nb, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else if (a.masked)
+         k_sap_reg<0, float2, true><<<nb, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      else
+         k_sap_reg<0, float2, false><<<nb, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+      return;
+   }
    if (half_variant())
    {
       if (isolv && a.masked)
@@ -1394,6 +1431,11 @@ static void sap_push_faces(const SArgs &a, int c)
    Lib &L = lib();
    if (!G.nface[c]) return;
    unsigned nbk = (G.nface[c] + 255) / 256;
+   if (a.nsrc)
+   {
+      k_sap_faces<float2><<<dim3(nbk, a.nsrc), 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], nullptr);
+      return;
+   }
    if (half_variant())
       k_sap_faces<__half2><<<nbk, 256, 0, L.stream>>>(a, G.d_face[c], G.d_fmask[c], G.nface[c], G.d_u16);
    else
@@ -1409,7 +1451,7 @@ static void sap_cycle(const SArgs &a, int isolv, bool first, bool finish)
 {
    // the FP32 register kernel runs one wavefront per SIMD, too few to hide
    // the latency of the scattered face loads: it keeps the face kernel
-   if (half_variant())
+   if (half_variant() && !a.nsrc)
    {
       sap_blocks(a, 0, isolv, !first);
       sap_blocks(a, 1, isolv, 1);
@@ -1596,6 +1638,29 @@ void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho)
       k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], psi, V);
       k_h16_to_s32<<<nbk, 256, 0, L.stream>>>(G.d_h16[1], G.d_hsc[1], rho, V);
    }
+}
+
+// ncy Schwarz cycles (FP32) on n sources at once (n <= 16): psi[k] and
+// rho[k] are full FP32 device fields; each launch takes all sources
+bool sap_multi_device(float mu, int isolv, int nmr, int ncy, int n, float2 *const *psi, float2 *const *rho)
+{
+   Lib &L = lib();
+   if (n < 1 || n > 16 || !reg_kernel()) return false;
+   if (!G.d_s0gm)
+   {
+      G.d_s0gm = (float2 *)dev_alloc(16 * units_for(L.g.V, 12) * sizeof(float2));
+      if (!G.d_s0gm) return false;
+   }
+   SArgs a = sargs(mu, nmr);
+   a.nsrc = n;
+   for (int k = 0; k < n; k++)
+   {
+      a.pl[k] = psi[k];
+      a.rl[k] = rho[k];
+      a.sl[k] = G.d_s0gm + (size_t)k * units_for(L.g.V, 12);
+   }
+   for (int c = 0; c < ncy; c++) sap_cycle(a, isolv, true, true);
+   return true;
 }
 
 // the FP16 residual of the last Schwarz cycles as an FP32 field

@@ -22,6 +22,7 @@ bool sap_prepare(int isolv, const char *where, bool quiet);
 bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, bool want_res);
 void sap_mixed_residual(float2 *r);
 bool sap_mop64(double mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, float2 *chi);
+bool sap_multi_device(float mu, int isolv, int nmr, int ncy, int n, float2 *const *psi, float2 *const *rho);
 }
 
 using namespace oqh;
@@ -87,10 +88,19 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
    }
    const int SB_ = nkv, SC_ = nkv + 1, SR_ = nkv + 2;
    float2 *dcoef = nullptr, *hcoef = nullptr;
+   static float2 *d_gram = nullptr;
+   static int gram_ld = 0;
    if (fused)
    {
       dcoef = (float2 *)tmp_gs_coef(nkv + 3, &hcoef);
-      fused = dcoef != nullptr;
+      if (nkv > gram_ld)
+      {
+         dev_free(d_gram);
+         gram_ld = nkv;
+         d_gram = (float2 *)dev_alloc(sizeof(float2) * gram_ld * gram_ld);
+         if (!d_gram) gram_ld = 0;
+      }
+      fused = dcoef != nullptr && d_gram != nullptr;
    }
    auto F = [](spinor *p) { return (float2 *)device_field(p, F32, &dummy_off); };
    std::vector<float2 *> qv(nkv);
@@ -110,7 +120,7 @@ extern "C" double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
             if (L.gcr_cgs)
             {
                for (int l = 0; l < k; l++) qv[l] = F(chi[l]);
-               gs_classical(ck, qv.data(), k, F(rho), dcoef, SB_, SC_, SR_, npts);
+               gs_classical(ck, qv.data(), k, F(rho), dcoef, SB_, SC_, SR_, npts, d_gram, gram_ld);
             }
             else if (k > 0)
             {
@@ -324,3 +334,210 @@ extern "C" double sap_gcr(int nkv, int nmx, int istop, double res, double mu, sp
    }
    return rho;
 }
+
+// ---------------------------------------------------------------------------
+// several right-hand sides (no counterpart in the reference): the FGCR
+// iterations of all sources advance together, the Schwarz cycles of all
+// active sources run in the same launches (FP32 SAP, blocks of 256 points),
+// each source keeps its own Krylov space, coefficients and restarts
+// (fgcr.c:215-299 per source)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct MultiSrc
+{
+   spinor **phi = nullptr, *rho = nullptr, **chi = nullptr;
+   spinor_dble *wrk = nullptr, *rsd = nullptr, *psi = nullptr;
+   double rho0 = 0.0, rn = 0.0, tol = 0.0, rn_old = 0.0;
+   int k = 0, status = 0;
+   bool done = false, failed = false;
+   std::vector<complex> a, c;
+   std::vector<float> b;
+   float2 *dcoef = nullptr, *gram = nullptr;
+};
+
+}  // namespace
+
+extern "C" void sap_gcr_multi(int n, int nkv, int nmx, int istop, double res, double mu, spinor_dble **eta,
+                              spinor_dble **psi, double *rho_out, int *ifail, int *status)
+{
+   const char *where = "sap_gcr_multi";
+   Lib &L = lib();
+   if (!require_geometry(where)) return;
+   if (n < 1 || nkv < 1 || nmx < 1 || res <= DBL_EPSILON || istop < 0 || istop > 1)
+   {
+      fail(where, "Parameters are out of range");
+      return;
+   }
+   if (L.sap_ncy == 0)
+   {
+      fail(where, "SAP parameters are not set");
+      return;
+   }
+   const int vt = L.g.VT, V = L.g.V;
+   // the joint path: FP32 SAP with the register kernel, 2-norm stopping
+   // criterion, no network; otherwise one solve after the other
+   bool joint = (L.sap_prec == 0 && istop == 0 && !nn_ready());
+   if (joint && !sap_prepare(L.sap_isolv, where, true)) joint = false;
+   int bs_vol = L.sap_bs[0] * L.sap_bs[1] * L.sap_bs[2] * L.sap_bs[3];
+   if (bs_vol != 256) joint = false;
+   if (!joint)
+   {
+      for (int s = 0; s < n; s++) rho_out[s] = sap_gcr(nkv, nmx, istop, res, mu, eta[s], psi[s], ifail + s, status + s);
+      return;
+   }
+   const int SB_ = nkv, SC_ = nkv + 1, SR_ = nkv + 2, NC = nkv + 3;
+   const double PRECISION_LIMIT = (double)(100.0f * FLT_EPSILON);
+   alloc_ws(L.ws_used + n * (2 * nkv + 1));
+   alloc_wsd(L.wsd_used + 2 * n);
+   std::vector<MultiSrc> S(n);
+   float2 *dcoef = (float2 *)dev_alloc(sizeof(float2) * NC * n);
+   float2 *dgram = (float2 *)dev_alloc(sizeof(float2) * nkv * nkv * n);
+   float2 *hcoef = nullptr;
+   HIP_OK(hipHostMalloc((void **)&hcoef, sizeof(float2) * NC * n, 0));
+   auto F = [](spinor *p) { return (float2 *)device_field(p, F32, &dummy_off); };
+   mud = mu;
+   for (int s = 0; s < n; s++)
+   {
+      MultiSrc &m = S[s];
+      spinor **ws = reserve_ws(2 * nkv + 1);
+      spinor_dble **wsd = reserve_wsd(2);
+      m.phi = ws;
+      m.rho = ws[nkv];
+      m.chi = ws + nkv + 1;
+      m.wrk = wsd[0];
+      m.rsd = wsd[1];
+      m.psi = psi[s];
+      m.a.assign((size_t)nkv * nkv, complex{0.0f, 0.0f});
+      m.c.assign(nkv, complex{0.0f, 0.0f});
+      m.b.assign(nkv, 0.0f);
+      m.dcoef = dcoef + (size_t)s * NC;
+      m.gram = dgram + (size_t)s * nkv * nkv;
+      ifail[s] = 0;
+      status[s] = 0;
+      rho_out[s] = 0.0;
+      m.rho0 = unorm_dble(vt, 3, eta[s]);
+      if (m.rho0 == 0.0)
+      {
+         set_sd2zero(vt, 2, psi[s]);
+         m.done = true;
+         continue;
+      }
+      assign_sd2sd(vt, 2, eta[s], m.rsd);
+      scale_dble(vt, 2, 1.0 / m.rho0, m.rsd);
+      set_sd2zero(vt, 2, m.psi);
+      assign_sd2s(vt, 2, m.rsd, m.rho);
+      m.rn = get_res(vt, 0, m.rho);
+      m.tol = res * m.rn;
+      m.rn_old = m.rn;
+      if (m.rn <= m.tol) m.done = true;
+   }
+   std::vector<int> act;
+   std::vector<float2 *> pl, rl, qv(nkv);
+   while (true)
+   {
+      act.clear();
+      for (int s = 0; s < n; s++)
+         if (!S[s].done) act.push_back(s);
+      if (act.empty()) break;
+      // Mop of every active source: phi = SAP(rho), chi = rho - residual
+      for (int s : act)
+      {
+         set_s2zero(vt, 2, S[s].phi[S[s].k]);
+         assign_s2s(vt, 2, S[s].rho, S[s].chi[S[s].k]);
+      }
+      for (size_t i0 = 0; i0 < act.size(); i0 += 16)
+      {
+         pl.clear();
+         rl.clear();
+         for (size_t i = i0; i < act.size() && i < i0 + 16; i++)
+         {
+            pl.push_back(F(S[act[i]].phi[S[act[i]].k]));
+            rl.push_back(F(S[act[i]].chi[S[act[i]].k]));
+         }
+         sap_multi_device((float)mud, L.sap_isolv, L.sap_nmr, L.sap_ncy, (int)pl.size(), pl.data(), rl.data());
+      }
+      for (int s : act) diff_s2s(vt, 2, S[s].rho, S[s].chi[S[s].k]);
+      // Gram-Schmidt step of every active source, one read-back for all
+      for (int s : act)
+      {
+         MultiSrc &m = S[s];
+         for (int l = 0; l < m.k; l++) qv[l] = F(m.chi[l]);
+         gs_classical(F(m.chi[m.k]), qv.data(), m.k, F(m.rho), m.dcoef, SB_, SC_, SR_, V, m.gram, nkv);
+      }
+      HIP_OK(hipMemcpyAsync(hcoef, dcoef, sizeof(float2) * NC * n, hipMemcpyDeviceToHost, L.stream));
+      HIP_OK(hipStreamSynchronize(L.stream));
+      for (int s : act)
+      {
+         MultiSrc &m = S[s];
+         const float2 *hc = hcoef + (size_t)s * NC;
+         const int k = m.k;
+         for (int l = 0; l < k; l++)
+         {
+            m.a[(size_t)nkv * l + k].re = hc[l].x;
+            m.a[(size_t)nkv * l + k].im = hc[l].y;
+         }
+         m.b[k] = hc[SB_].x;
+         if (m.b[k] == 0.0f) fail("normalize [salg.c]", "Vector has vanishing norm");
+         m.c[k].re = hc[SC_].x;
+         m.c[k].im = hc[SC_].y;
+         m.rn = sqrt((double)hc[SR_].y);
+         status[s] += 1;
+         if (!(m.rn <= m.tol || m.rn < PRECISION_LIMIT * m.rn_old || k == nkv - 1 || status[s] >= nmx))
+         {
+            m.k++;
+            continue;
+         }
+         // restart: update_psi (fgcr.c:181-212)
+         for (int l = k; l >= 0; l--)
+         {
+            float zr = m.c[l].re, zi = m.c[l].im;
+            for (int i = l + 1; i <= k; i++)
+            {
+               zr -= (m.a[(size_t)l * nkv + i].re * m.c[i].re - m.a[(size_t)l * nkv + i].im * m.c[i].im);
+               zi -= (m.a[(size_t)l * nkv + i].re * m.c[i].im + m.a[(size_t)l * nkv + i].im * m.c[i].re);
+            }
+            float r = 1.0f / m.b[l];
+            m.c[l].re = zr * r;
+            m.c[l].im = zi * r;
+         }
+         set_s2zero(vt, 2, m.rho);
+         for (int l = k; l >= 0; l--) mulc_spinor_add(vt, 2, m.rho, m.phi[l], m.c[l]);
+         add_s2sd(vt, 2, m.rho, m.psi);
+         Dw_dble(mud, m.psi, m.wrk);
+         diff_sd2s(vt, 2, m.rsd, m.wrk, m.rho);
+         m.rn = get_res(vt, 0, m.rho);
+         m.k = 0;
+         m.rn_old = m.rn;
+         if (status[s] >= nmx && m.rn > m.tol)
+         {
+            m.failed = true;
+            m.done = true;
+         }
+         else if (m.rn <= m.tol)
+            m.done = true;
+      }
+   }
+   for (int s = 0; s < n; s++)
+   {
+      MultiSrc &m = S[s];
+      if (m.rho0 == 0.0) continue;
+      scale_dble(vt, 2, m.rho0, m.psi);
+      rho_out[s] = m.rn * m.rho0;
+      if (m.failed)
+      {
+         ifail[s] = -1;
+         status[s] = nmx;
+      }
+   }
+   HIP_OK(hipHostFree(hcoef));
+   dev_free(dcoef);
+   dev_free(dgram);
+   for (int s = 0; s < n; s++)
+   {
+      release_wsd();
+      release_ws();
+   }
+}
+
