@@ -3,8 +3,10 @@
 
 Workload (one "step" = one application of Dw_dble): the 32^3x64 operator
 throughput configuration -- open boundary conditions (cF=1), csw=1.77,
-m0=-0.3, Gaussian spinor source, Haar-random SU(3) links smoothed by 100
-forward-Euler steps (eps 0.01) of the Wilson flow.  The value is the algorithmic
+m0=-0.3, the reference's random field (start_ranlux(0, 2), random_ud: Haar
+SU(3) links from the ranlxd streams, generated on the device) smoothed by 100
+forward-Euler steps (eps 0.01) of the Wilson flow, Gaussian source
+(random_sd).  The value is the algorithmic
 HBM bandwidth of Dw_dble, 1536 bytes per lattice site (links 576, SW term
 576, spinor in 192, out 192), over the timed steps.  Inputs are resident in
 HBM before the timed region.
@@ -65,7 +67,7 @@ NN_WIDTHS = [4, 12, 16, 12, 4]
 
 def train_network():
     """Trains the learned preconditioner on a 16^4 lattice with the physics
-    of the solver workload (periodic bc, Haar links of seed 1 flowed 50 x
+    of the solver workload (periodic bc, random_ud of seed 1 flowed 50 x
     0.01, csw=1.77, m0=-0.35, SAP 4^4 isolv=1 nmr=4 ncy=5): 32 pairs from
     Gaussian sources r, e = D^-1 r by SAP+GCR to 1e-12; the network input is
     the residual the 5 Schwarz cycles leave of r, its target the error they
@@ -86,7 +88,8 @@ def train_network():
         lib.sap.argtypes = [C.c_float, C.c_int, C.c_int, P, P]
         rng = np.random.default_rng(1)
         V = lat.V
-        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        lib.start_ranlux(0, 1)
+        lib.random_ud()
         lib.fwd_euler(50, 0.01)
         lib.set_ud_phase()
         bs = np.array((4, 4, 4, 4), np.int32)
@@ -96,8 +99,8 @@ def train_network():
         psi, rho = lat.alloc_s(), lat.alloc_s()
         fr, fe = [], []
         for k in range(32):
-            r = rng.standard_normal((V, 24))
-            lat.put_sd(e_, r)
+            lib.random_sd(V, e_, 1.0)
+            r = lat.get_sd(e_)
             ifail, st = C.c_int(), C.c_int()
             lib.sap_gcr(24, 1024, 0, 1e-12, 0.0, e_, p_, C.byref(ifail), C.byref(st))
             e = lat.get_sd(p_)
@@ -143,7 +146,7 @@ def train_network():
 
 def solve_times(args):
     """Time-to-solution of SAP+GCR on the 48^4 solver workload: periodic bc,
-    Haar-random links (numpy, seed 3) smoothed by 50 Euler steps (eps 0.01) of
+    random_ud links (start_ranlux(0, 3)) smoothed by 50 Euler steps (eps 0.01) of
     the Wilson flow, csw=1.77, m0=-0.35, Gaussian source; FGCR with nkv=24 to
     relative residual 1e-10, SAP on 4^4 blocks (isolv=1, nmr=4, ncy=5).  The
     plain solver (FP32 SAP, FP64 GCR), the FP16-storage variant (links, SW
@@ -160,14 +163,14 @@ def solve_times(args):
     P = C.c_void_p
     L4 = (48, 48, 48, 48)
     lat = pyoqh.Lattice(L4, (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
-    out = {"workload": "SAP+GCR 48^4 periodic, Wilson-flowed (50 x 0.01) Haar links, csw=1.77, "
+    out = {"workload": "SAP+GCR 48^4 periodic, Wilson-flowed (50 x 0.01) random_ud links (seed 3), csw=1.77, "
                        "m0=-0.35, nkv=24, res=1e-10, SAP 4^4 blocks isolv=1 nmr=4 ncy=5"}
     try:
         V = lat.V
-        rng = np.random.default_rng(3)
-        log("bench: solver workload, generating %d SU(3) links" % (4 * V))
-        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        log("bench: solver workload, random_ud of seed 3 (%d links)" % (4 * V))
         lib = lat.lib
+        lib.start_ranlux(0, 3)
+        lib.random_ud()
         lib.fwd_euler(50, 0.01)
         lib.set_ud_phase()
         lib.oqh_set_sap_parms.argtypes = [P, C.c_int, C.c_int, C.c_int]
@@ -180,7 +183,7 @@ def solve_times(args):
         bs = np.array((4, 4, 4, 4), np.int32)
         lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
         e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
-        lat.put_sd(e, rng.standard_normal((V, 24)))
+        lib.random_sd(V, e, 1.0)
 
         def nsq(f):
             q = lib.norm_square_dble(V, 0, f)
@@ -232,7 +235,7 @@ def solve_times(args):
 
 def multi_rhs():
     """Several right-hand sides on one 32^3x64 field (periodic bc with the
-    quark phase, csw=1.77, m0=-0.35, Haar links of seed 2 flowed by 50 steps
+    quark phase, csw=1.77, m0=-0.35, random_ud links of seed 2 flowed by 50 steps
     of 0.01): the 12 spin-colour point sources at the origin and 4 Gaussian
     sources, solved together by sap_gcr_multi (FP32 SAP on 4^4 blocks,
     isolv=1, nmr=4, ncy=5; FGCR nkv=16 to 1e-10), against one sap_gcr call
@@ -249,9 +252,9 @@ def multi_rhs():
         lib.sap_gcr.restype = C.c_double
         lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
-        rng = np.random.default_rng(2)
         V = lat.V
-        lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+        lib.start_ranlux(0, 2)
+        lib.random_ud()
         lib.fwd_euler(50, 0.01)
         lib.set_ud_phase()
         bs = np.array((4, 4, 4, 4), np.int32)
@@ -263,7 +266,7 @@ def multi_rhs():
             e[0, 2 * k] = 1.0
             lat.put_sd(es[k], e)
         for k in range(12, 16):
-            lat.put_sd(es[k], rng.standard_normal((V, 24)))
+            lib.random_sd(V, es[k], 1.0)
         rho = np.zeros(n)
         ifail = np.zeros(n, np.int32)
         status = np.zeros(n, np.int32)
@@ -299,7 +302,7 @@ def multi_rhs():
 def uneven_batch():
     """The uneven batch: 16 independent solves on lattices 8^4, 12^4, 16^4 and
     24^4 (4 each; periodic bc with the quark phase, csw=1.77, m0=-0.35 as the
-    solver workload), Haar links from numpy seeds 10-25 flowed by 50 Euler
+    solver workload), random_ud links of seeds 10-25 flowed by 50 Euler
     steps of 0.01 on the device, Gaussian sources; the SAP block size of each lattice is chosen by the autotuner
     (isolv=1, nmr=4, ncy=5; 12^4 admits only 6^4), FGCR nkv=24 to 1e-10.  The
     lattices run one after another (the library holds one geometry)."""
@@ -311,18 +314,20 @@ def uneven_batch():
                                        "csw=1.77, m0=-0.35, flow 50 x 0.01, autotuned SAP blocks, res 1e-10"}
     t_all = time.perf_counter()
     for k, n in enumerate([8] * 4 + [12] * 4 + [16] * 4 + [24] * 4):
-        lat = pyoqh.Lattice((n,) * 4, None, bc=3, m0=-0.35, csw=1.77)
+        # thread layout of the reference run: one ranlxd stream per thread
+        ltrd = {8: (4, 4, 4, 4), 12: None, 16: (8, 8, 8, 8), 24: (8, 8, 8, 8)}[n]
+        lat = pyoqh.Lattice((n,) * 4, ltrd, bc=3, m0=-0.35, csw=1.77)
         try:
             lib = lat.lib
             lib.sap_gcr.restype = C.c_double
             lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
-            rng = np.random.default_rng(10 + k)
-            lat.import_ud(pyoqh.haar_su3(4 * lat.V, rng))
+            lib.start_ranlux(0, 10 + k)
+            lib.random_ud()
             lib.fwd_euler(50, 0.01)
             lib.set_ud_phase()
             e, p = lat.alloc_sd(), lat.alloc_sd()
-            lat.put_sd(e, rng.standard_normal((lat.V, 24)))
+            lib.random_sd(lat.V, e, 1.0)
             lat.sync()
             t0 = time.perf_counter()
             bs = np.zeros(4, np.int32)
@@ -396,13 +401,14 @@ def main():
     lat = pyoqh.Lattice(L, LTRD, bc=0, cF=(1.0, 1.0), m0=-0.3, csw=1.77)
     V = lat.V
     rng = np.random.default_rng(2)
-    log("bench: generating %d SU(3) links" % (4 * V))
-    lat.import_ud(pyoqh.haar_su3(4 * V, rng))
+    log("bench: random_ud of seed 2 (%d links)" % (4 * V))
+    lat.lib.start_ranlux(0, 2)
+    lat.lib.random_ud()
     log("bench: Wilson flow, 100 Euler steps of 0.01 on the device")
     lat.lib.fwd_euler(100, 0.01)
     assert lat.sw_term(3) == 0
     s, r = lat.alloc_sd(), lat.alloc_sd()
-    lat.put_sd(s, rng.standard_normal((V, 24)))
+    lat.lib.random_sd(V, s, 1.0)
     lib = lat.lib
 
     # warmup, then K timed steps bracketed by device synchronisation
@@ -465,8 +471,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: Haar-random SU(3) links (numpy, seed 2) smoothed by the Wilson flow "
-                "(100 forward-Euler steps, eps 0.01, on the device), Gaussian FP64 source",
+        "data": "synthetic: the reference's random_ud field (start_ranlux(0, 2), ranlxd on the device) "
+                "smoothed by the Wilson flow (100 forward-Euler steps, eps 0.01, on the device), "
+                "Gaussian FP64 source (random_sd)",
         "config": {"workload": "Dw_dble 32^3x64 open bc (type 0, cF=1), csw=1.77, m0=-0.3",
                    "lattice": "64x32x32x32", "bytes_per_site": BYTES["Dw_dble"],
                    "pct_of_8TBps": round(100 * value / PEAK_HBM, 1),

@@ -118,6 +118,18 @@ void sap_gcr_multi(int n, int nkv, int nmx, int istop, double res, double mu, sp
    of the modified process in exact arithmetic. */
 void oqh_set_gcr_orth(int mode);
 
+/* random numbers and random fields on the device: the streams of
+   random/ranlux.c:136-171 (ranlxd, one per reference thread of the
+   geometry's thread layout), random_ud (uflds/uflds.c:159-222, uflds.h),
+   random_sd (sflds/sflds.c:1161-1200, sflds.h; vol a multiple of the number
+   of threads, each thread filling vol/NTHREAD points), ranlxd
+   (random/ranlxd.c; the master thread's stream, n numbers into the host
+   array r).  The fields are the reference's for the same seed and layout. */
+void start_ranlux(int level, int seed);
+void random_ud(void);
+void random_sd(int vol, spinor_dble *sd, double sigma);
+void ranlxd(double *r, int n);
+
 /* learned preconditioner (no counterpart in the reference): a linear
    gauge-equivariant network of parallel-transport layers,
       out(x)[o] = sum_{p=0..8} sum_i W_l[p][o][i] (T_p in)(x)[i],

@@ -62,6 +62,8 @@ _SIG = {
     "oqh_set_gcr_orth": (None, [_I]),
     "oqh_set_sap_graph": (None, [_I]),
     "oqh_sap_autotune": (_D, [_I, _I, _I, _I, P]),
+    "start_ranlux": (None, [_I, _I]), "random_ud": (None, []),
+    "random_sd": (None, [_I, P, _D]), "ranlxd": (None, [P, _I]),
     "sap_gcr_multi": (None, [_I, _I, _I, _I, _D, _D, P, P, P, P, P]),
     "oqh_nn_create": (_I, [_I, P]), "oqh_nn_nweights": (_I, []),
     "oqh_nn_set_weights": (None, [P]), "oqh_nn_get_weights": (None, [P]),

@@ -1,0 +1,71 @@
+"""Device random numbers and random fields (rng.hip) against the known-answer
+vectors of devel/nompi/random/check1.c (the ranlxd stream, bit-exact) and
+against the oracle's restatement of random_ud and random_sd on the same seed
+and thread layout.  The streams are integer arithmetic and agree exactly;
+the fields pass through the device's sin, cos and log, so they agree to the
+last bits (1e-14)."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pyoqh
+import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.gpu
+def test_ranlxd_known_answers():
+    k = json.load(open(os.path.join(GOLD, "ranlux_kat.json")))["ranlxd"]
+    assert k["level"] == 1
+    lat = pyoqh.Lattice((8, 8, 8, 8), None, bc=3)
+    try:
+        lat.lib.start_ranlux(0, k["seed"])  # ranlxd runs at level + 1
+        n = k["calls"] * k["per_call"]
+        r = np.empty(n)
+        lat.lib.ranlxd(r.ctypes.data_as(ctypes.c_void_p), n)
+        last = r[(k["calls"] - 1) * k["per_call"]:]
+        got = last[k["offset"]:k["offset"] + 48] * 2.0 ** 48
+        assert np.array_equal(got, np.array(k["values_times_2p48"]))
+    finally:
+        lat.close()
+
+
+CFGS = [dict(name="8^4-periodic", L=(8, 8, 8, 8), T=None, bc=3, seed=1),
+        dict(name="8^4-open-nthread16", L=(8, 8, 8, 8), T=(4, 4, 4, 4), bc=0, seed=2),
+        dict(name="16x8^3-periodic", L=(16, 8, 8, 8), T=None, bc=3, seed=5),
+        dict(name="16^4-open-nthread16", L=(16, 16, 16, 16), T=(8, 8, 8, 8), bc=0, seed=7)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", CFGS, ids=lambda c: c["name"])
+def test_random_fields(c):
+    print("config", c, flush=True)
+    o = po.Oracle(c["L"], c["T"], bc=c["bc"])
+    o.start_ranlux(0, c["seed"])
+    o.random_ud()
+    ud_o = o.ud()
+    sd_o = o.random_sd(1.0)
+    lat = pyoqh.Lattice(c["L"], c["T"], bc=c["bc"])
+    try:
+        lat.lib.start_ranlux(0, c["seed"])
+        lat.lib.random_ud()
+        ud_g = lat.export_ud()
+        dev = np.abs(ud_g - ud_o).max()
+        print("random_ud max deviation", dev)
+        assert dev < 1e-14
+        s = lat.alloc_sd()
+        lat.lib.random_sd(lat.V, s, 1.0)
+        dev = np.abs(lat.get_sd(s) - sd_o).max()
+        print("random_sd max deviation", dev)
+        assert dev < 1e-13
+        # the operators accept the generated field (links SU(3))
+        z = (ud_g[:, 0::2] + 1j * ud_g[:, 1::2]).reshape(-1, 3, 3)
+        nz = np.abs(z).max(axis=(1, 2)) > 0
+        m = z[nz]
+        assert np.abs(m @ np.conj(np.transpose(m, (0, 2, 1))) - np.eye(3)).max() < 1e-14
+    finally:
+        lat.close()
