@@ -45,7 +45,12 @@ template <typename T> struct DArgs
    T mu, muo, coef;
    int masked;     // 1 if boundary points exist (bc != 3)
    const unsigned char *lsign;  // per odd point, bit l: link l is -SU(3) (rebuilt links)
+   const C *sws;   // SW term in the S-form (27 units per point) or null
+   const int *x0;  // time coordinates (diagonal constant of the S-form)
+   int N0, bc;
+   T cx1, cx3a, cx3b;  // 4 + m0 in the bulk, + cF - 1 next to open boundaries
 };
+
 
 template <typename T, int MU>
 __device__ inline void hop_mu(const DArgs<T> &a, const typename cplx_of<T>::t *in, int x, int odd,
@@ -388,7 +393,29 @@ __device__ inline void tile_phase(const DArgs<T> &a, int t, int e0, bool valid, 
    if (XBUF == 1) __syncthreads();
 }
 
-template <typename T, int TP = TileSize<T>::n, bool REC = false>
+// diagonal constant of the SW term at point x (sw_term.c:293-309)
+template <typename T> __device__ inline T sw_const(const DArgs<T> &a, int x)
+{
+   if (a.bc == 3) return a.cx1;
+   const int t = a.x0[x];
+   if (t == 0 || (t == a.N0 - 1 && a.bc == 0)) return (T)1;
+   if (t == 1) return a.cx3a;
+   if (t == a.N0 - 2 && a.bc == 0) return a.cx3b;
+   return a.cx1;
+}
+
+// (A + i mu) v at point x from the Pauli field or from the S-form
+template <typename T, bool SWS>
+__device__ inline void site_term(const DArgs<T> &a, int x, T mu, const typename cplx_of<T>::t *v,
+                                 typename cplx_of<T>::t *out)
+{
+   if (SWS)
+      sws_site<typename cplx_of<T>::t, OQH_NT_SW>(a.sws, x, sw_const(a, x), mu, v, out);
+   else
+      pauli_site<typename cplx_of<T>::t, OQH_NT_SW>(a.sw, x, mu, v, out);
+}
+
+template <typename T, int TP = TileSize<T>::n, bool REC = false, bool SWS = false>
 __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
@@ -472,7 +499,7 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 #pragma unroll
       for (int c = 0; c < 12; c++) out[c] = cscale(acc_o[c], a.coef);
 #else
-      pauli_site<C, OQH_NT_SW>(a.sw, o, a.muo, so, out);
+      site_term<T, SWS>(a, o, a.muo, so, out);
 #pragma unroll
       for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_o[c], a.coef));
 #endif
@@ -492,7 +519,7 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
       C se[12];
 #pragma unroll
       for (int c = 0; c < 12; c++) se[c] = S[c][t];
-      pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, out);
+      site_term<T, SWS>(a, e, a.mu, se, out);
 #pragma unroll
       for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_e[c], a.coef));
 #endif
@@ -809,6 +836,9 @@ __global__ __launch_bounds__(256) void k_zero_bnd(typename cplx_of<T>::t *s, con
 #ifndef OQH_LINK_REC
 #define OQH_LINK_REC 1
 #endif
+#ifndef OQH_SW_SFORM
+#define OQH_SW_SFORM 1
+#endif
 
 // per odd point j and link l: compares row 2 with +-conj(row0 x row1); sets
 // bit l of sign[j] for the minus sign and raises *dev to the largest
@@ -882,6 +912,17 @@ template <typename T> static DArgs<T> base_args()
    // links rebuilt from two rows when the whole field allows it (the FP32
    // links are the rounded FP64 ones: same signs)
    a.lsign = (OQH_LINK_REC && L.ud_rec && L.rec_version == L.ud_version) ? L.d_lsign : nullptr;
+   // FP64 SW term in the S-form when the field holds the uninverted term
+   // computed from the gauge field
+   a.sws = (OQH_SW_SFORM && sizeof(T) == 8 && L.sws_valid && !L.swd_inv[0] && !L.swd_inv[1])
+              ? (const typename cplx_of<T>::t *)(const void *)L.d_sws
+              : nullptr;
+   a.x0 = L.d_x0;
+   a.N0 = L.g.L[0];
+   a.bc = L.bc;
+   a.cx1 = (T)(4.0 + L.m0);
+   a.cx3a = (T)(4.0 + L.m0 + L.cF[0] - 1.0);
+   a.cx3b = (T)(4.0 + L.m0 + L.cF[1] - 1.0);
    return a;
 }
 
@@ -894,10 +935,14 @@ template <typename T, int OP> static void launch(const DArgs<T> &a)
       constexpr int TP = TileSize<T>::n;
       unsigned nt = (unsigned)((a.h + TP - 1) / TP);
       if (!nt) return;
-      if (a.lsign)
-         k_dw_tile<T, TP, true><<<nt, TP, 0, L.stream>>>(a);
+      if (a.lsign && a.sws)
+         k_dw_tile<T, TP, true, true><<<nt, TP, 0, L.stream>>>(a);
+      else if (a.lsign)
+         k_dw_tile<T, TP, true, false><<<nt, TP, 0, L.stream>>>(a);
+      else if (a.sws)
+         k_dw_tile<T, TP, false, true><<<nt, TP, 0, L.stream>>>(a);
       else
-         k_dw_tile<T, TP, false><<<nt, TP, 0, L.stream>>>(a);
+         k_dw_tile<T, TP, false, false><<<nt, TP, 0, L.stream>>>(a);
       return;
    }
 #endif

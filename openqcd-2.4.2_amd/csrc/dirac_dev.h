@@ -203,4 +203,76 @@ __device__ inline void pauli_site(const C *sw, int x, decltype(C::x) mu, const C
    pauli_half<C, NT>(m + 18 * SB, -mu, v + 6, out + 6);
 }
 
+// out = (A + i mu gamma5-sign) v with A of one point given in the S-form:
+// per Weyl half A = [[S3 + c, S1 - i S2], [S1 + i S2, -S3 + c]] with
+// Hermitian 3x3 S_k (units 0-8: real diagonals of S1..S3 of both halves,
+// units 9 + 9 half + 3 k + q: entries (0,1), (0,2), (1,2) of S_{k+1}); c is
+// the diagonal constant of the point; the upper half uses +mu, the lower -mu
+template <typename C>
+__device__ inline void herm3_mul(const decltype(C::x) *d, const C *o, const C *v, C *r)
+{
+   // r = S v, S = [[d0, o0, o1], [o0*, d1, o2], [o1*, o2*, d2]]
+   r[0] = cadd(cadd(cscale(v[0], d[0]), cmul(o[0], v[1])), cmul(o[1], v[2]));
+   r[1] = cadd(cadd(cmulc(o[0], v[0]), cscale(v[1], d[1])), cmul(o[2], v[2]));
+   r[2] = cadd(cadd(cmulc(o[1], v[0]), cmulc(o[2], v[1])), cscale(v[2], d[2]));
+}
+
+template <typename C, bool NT = false>
+__device__ inline void sws_site(const C *sws, int x, decltype(C::x) c, decltype(C::x) mu, const C *v, C *out)
+{
+   typedef decltype(C::x) R;
+   const C *m = sws + (size_t)(x >> 6) * (27 * SB) + (x & (SB - 1));
+#pragma unroll
+   for (int h = 0; h < 2; h++)
+   {
+      const R mm = h ? -mu : mu;
+      const C *vu = v + 6 * h, *vd = v + 6 * h + 3;
+      C *ou = out + 6 * h, *odn = out + 6 * h + 3;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+      {
+         ou[i] = cmk<C>(c * vu[i].x - mm * vu[i].y, c * vu[i].y + mm * vu[i].x);
+         odn[i] = cmk<C>(c * vd[i].x - mm * vd[i].y, c * vd[i].y + mm * vd[i].x);
+      }
+      // S1, S2, S3 of this half one after the other, each used twice
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+      {
+         R d[3];
+#pragma unroll
+         for (int i = 0; i < 3; i++)
+         {
+            const int g = 9 * h + 3 * k + i;  // among the 18 diagonal reals
+            C w = ld_stream<NT>(m + (g >> 1) * SB);
+            d[i] = (g & 1) ? w.y : w.x;
+         }
+         C o[3];
+#pragma unroll
+         for (int q = 0; q < 3; q++) o[q] = ld_stream<NT>(m + (9 + 9 * h + 3 * k + q) * SB);
+         C a[3], b[3];
+         herm3_mul(d, o, vd, a);  // S_k vd
+         herm3_mul(d, o, vu, b);  // S_k vu
+#pragma unroll
+         for (int i = 0; i < 3; i++)
+         {
+            if (k == 0)
+            {
+               ou[i] = cadd(ou[i], a[i]);   // S1 vd
+               odn[i] = cadd(odn[i], b[i]);  // S1 vu
+            }
+            else if (k == 1)
+            {
+               ou[i] = csubi(ou[i], a[i]);   // -i S2 vd
+               odn[i] = caddi(odn[i], b[i]);  // +i S2 vu
+            }
+            else
+            {
+               ou[i] = cadd(ou[i], b[i]);   // S3 vu
+               odn[i] = csub(odn[i], a[i]);  // -S3 vd
+            }
+         }
+      }
+   }
+}
+
 }  // namespace oqh

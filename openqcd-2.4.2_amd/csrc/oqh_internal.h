@@ -85,6 +85,8 @@ struct Lib
    bool ud_set = false, u_valid = false, swd_valid = false, sw_valid = false;
    bool ud_phase = false;  // quark phase of set_ud_phase() applied to d_ud
    bool swd_inv[2] = {false, false};
+   double2 *d_sws = nullptr;  // SW term as S1, S2, S3 per half (27 units per point)
+   bool sws_valid = false;    // d_sws holds the term that compute_swd last wrote
    int swd_epoch = 0;  // incremented whenever the contents of d_swd change
    int u_epoch = 0;    // incremented whenever d_u is remade
    int ud_version = 0; // incremented whenever d_ud changes

@@ -140,6 +140,7 @@ struct SwArgs
 {
    const double2 *u;
    double2 *sw;
+   double2 *sws;  // the same term as three Hermitian 3x3 matrices per half (or null)
    const int *nbr;
    const int *x0;
    int V, N0, bc;
@@ -152,11 +153,14 @@ __global__ __launch_bounds__(128) void k_sw_term(SwArgs a)
    if (x >= a.V) return;
    double2 *out = a.sw + (size_t)(x >> 6) * (36 * SB) + (x & (SB - 1));
    int t = a.x0[x];
+   double2 *so = a.sws ? a.sws + (size_t)(x >> 6) * (27 * SB) + (x & (SB - 1)) : nullptr;
    if ((t == 0 && a.bc != 3) || (t == a.N0 - 1 && a.bc == 0))
    {
 #pragma unroll
       for (int k = 0; k < 36; k++)
          out[k * SB] = (k < 3 || (k >= 18 && k < 21)) ? make_double2(1.0, 1.0) : make_double2(0.0, 0.0);
+      if (so)
+         for (int k = 0; k < 27; k++) so[k * SB] = make_double2(0.0, 0.0);
       return;
    }
    double cx = a.c1;
@@ -180,6 +184,7 @@ __global__ __launch_bounds__(128) void k_sw_term(SwArgs a)
             M[n][3 * i + j] = make_double2(-a.c2 * f.y, a.c2 * f.x);
          }
    }
+   double dgs[18];
    for (int half = 0; half < 2; half++)
    {
       double sg = half ? 1.0 : -1.0;
@@ -201,6 +206,23 @@ __global__ __launch_bounds__(128) void k_sw_term(SwArgs a)
             H[6 * i + j + 3] = csubi(s1, s2);
             H[6 * (i + 3) + j] = caddi(s1, s2);
          }
+      if (so)
+      {
+         // S-form: units 0-8 the real diagonals of S1, S2, S3 of both halves
+         // (18 reals, written after the loop), units 9 + 9 half + 3 k + q the
+         // entries (0,1), (0,2), (1,2) of S_{k+1}
+#pragma unroll
+         for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) dgs[9 * half + 3 * k + i] = S[k][4 * i].x;
+#pragma unroll
+         for (int k = 0; k < 3; k++)
+         {
+            so[(9 + 9 * half + 3 * k + 0) * SB] = S[k][1];
+            so[(9 + 9 * half + 3 * k + 1) * SB] = S[k][2];
+            so[(9 + 9 * half + 3 * k + 2) * SB] = S[k][5];
+         }
+      }
       double2 *o = out + (size_t)(18 * half) * SB;
       o[0] = make_double2(H[0].x + cx, H[7].x + cx);
       o[SB] = make_double2(H[14].x + cx, H[21].x + cx);
@@ -211,6 +233,9 @@ __global__ __launch_bounds__(128) void k_sw_term(SwArgs a)
 #pragma unroll
          for (int j = i + 1; j < 6; j++) o[(p++) * SB] = H[6 * i + j];
    }
+   if (so)
+#pragma unroll
+      for (int r = 0; r < 9; r++) so[r * SB] = make_double2(dgs[2 * r], dgs[2 * r + 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,6 +376,8 @@ static void compute_swd()
    SwArgs a;
    a.u = L.d_ud;
    a.sw = L.d_swd;
+   if (!L.d_sws) L.d_sws = (double2 *)dev_alloc(units_for(L.g.V, 27) * sizeof(double2));
+   a.sws = L.d_sws;
    a.nbr = L.d_nbr;
    a.x0 = L.d_x0;
    a.V = L.g.V;
@@ -361,6 +388,7 @@ static void compute_swd()
    a.c3a = a.c1 + L.cF[0] - 1.0;
    a.c3b = a.c1 + L.cF[1] - 1.0;
    k_sw_term<<<(unsigned)((L.g.V + 127) / 128), 128, 0, L.stream>>>(a);
+   L.sws_valid = L.d_sws != nullptr;
    L.swd_valid = true;
    L.swd_inv[0] = L.swd_inv[1] = false;
    L.sw_valid = false;
