@@ -46,6 +46,7 @@ template <typename T> struct DArgs
    int masked;     // 1 if boundary points exist (bc != 3)
    const unsigned char *lsign;  // per odd point, bit l: link l is -SU(3) (rebuilt links)
    const C *sws;   // SW term in the S-form (27 units per point) or null
+   int sws_par;    // parities whose S-form term is usable (bit 0 even, bit 1 odd)
    const int *x0;  // time coordinates (diagonal constant of the S-form)
    int N0, bc;
    T cx1, cx3a, cx3b;  // 4 + m0 in the bulk, + cF - 1 next to open boundaries
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
    store_spinor(a.t, o, out);
 }
 
-template <typename T, int TP = TileSize<T>::n, bool REC = false>
+template <typename T, int TP = TileSize<T>::n, bool REC = false, bool SWS = false>
 __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
@@ -635,7 +636,7 @@ __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
    }
    C se[12], out[12];
    load_spinor(a.s, e, se);
-   pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, out);
+   site_term<T, SWS>(a, e, a.mu, se, out);
 #pragma unroll
    for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc[c], a.coef));
    store_spinor(a.r, e, out);
@@ -912,11 +913,13 @@ template <typename T> static DArgs<T> base_args()
    // links rebuilt from two rows when the whole field allows it (the FP32
    // links are the rounded FP64 ones: same signs)
    a.lsign = (OQH_LINK_REC && L.ud_rec && L.rec_version == L.ud_version) ? L.d_lsign : nullptr;
-   // FP64 SW term in the S-form when the field holds the uninverted term
-   // computed from the gauge field
-   a.sws = (OQH_SW_SFORM && sizeof(T) == 8 && L.sws_valid && !L.swd_inv[0] && !L.swd_inv[1])
-              ? (const typename cplx_of<T>::t *)(const void *)L.d_sws
-              : nullptr;
+   // SW term in the S-form (FP64, or its FP32 rounding) for the parities
+   // where the field holds the uninverted term computed from the gauge field
+   const bool swsok = OQH_SW_SFORM && L.sws_valid && (sizeof(T) == 8 || L.sws32_valid);
+   a.sws_par = swsok ? ((L.swd_inv[0] ? 0 : 1) | (L.swd_inv[1] ? 0 : 2)) : 0;
+   a.sws = a.sws_par ? (const typename cplx_of<T>::t *)(sizeof(T) == 8 ? (const void *)L.d_sws
+                                                                         : (const void *)L.d_sws32)
+                     : nullptr;
    a.x0 = L.d_x0;
    a.N0 = L.g.L[0];
    a.bc = L.bc;
@@ -935,11 +938,12 @@ template <typename T, int OP> static void launch(const DArgs<T> &a)
       constexpr int TP = TileSize<T>::n;
       unsigned nt = (unsigned)((a.h + TP - 1) / TP);
       if (!nt) return;
-      if (a.lsign && a.sws)
+      const bool sws = a.sws_par == 3;
+      if (a.lsign && sws)
          k_dw_tile<T, TP, true, true><<<nt, TP, 0, L.stream>>>(a);
       else if (a.lsign)
          k_dw_tile<T, TP, true, false><<<nt, TP, 0, L.stream>>>(a);
-      else if (a.sws)
+      else if (sws)
          k_dw_tile<T, TP, false, true><<<nt, TP, 0, L.stream>>>(a);
       else
          k_dw_tile<T, TP, false, false><<<nt, TP, 0, L.stream>>>(a);
@@ -1058,10 +1062,15 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
             k_hat1_tile<T, TileSize<T>::n, false><<<nt, TPn, 0, L.stream>>>(a);
          a.coef = (T)0.5;
          a.mu = mu;
-         if (a.lsign)
-            k_hat2_tile<T, TileSize<T>::n, true><<<nt, TPn, 0, L.stream>>>(a);
+         const bool sws = a.sws_par & 1;  // A_ee uninverted
+         if (a.lsign && sws)
+            k_hat2_tile<T, TileSize<T>::n, true, true><<<nt, TPn, 0, L.stream>>>(a);
+         else if (a.lsign)
+            k_hat2_tile<T, TileSize<T>::n, true, false><<<nt, TPn, 0, L.stream>>>(a);
+         else if (sws)
+            k_hat2_tile<T, TileSize<T>::n, false, true><<<nt, TPn, 0, L.stream>>>(a);
          else
-            k_hat2_tile<T, TileSize<T>::n, false><<<nt, TPn, 0, L.stream>>>(a);
+            k_hat2_tile<T, TileSize<T>::n, false, false><<<nt, TPn, 0, L.stream>>>(a);
       }
 #else
       a.coef = (T)-0.5;

@@ -87,6 +87,8 @@ struct Lib
    bool swd_inv[2] = {false, false};
    double2 *d_sws = nullptr;  // SW term as S1, S2, S3 per half (27 units per point)
    bool sws_valid = false;    // d_sws holds the term that compute_swd last wrote
+   float2 *d_sws32 = nullptr; // its FP32 rounding
+   bool sws32_valid = false;
    int swd_epoch = 0;  // incremented whenever the contents of d_swd change
    int u_epoch = 0;    // incremented whenever d_u is remade
    int ud_version = 0; // incremented whenever d_ud changes

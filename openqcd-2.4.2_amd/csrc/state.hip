@@ -474,9 +474,10 @@ extern "C" void oqh_finalize(void)
    L.d_tmp = nullptr;
    L.tmp_bytes = 0;
    dev_free(L.d_nbr); dev_free(L.d_bnd); dev_free(L.d_x0);
-   dev_free(L.d_ud); dev_free(L.d_u); dev_free(L.d_swd); dev_free(L.d_sw); dev_free(L.d_sws);
+   dev_free(L.d_ud); dev_free(L.d_u); dev_free(L.d_swd); dev_free(L.d_sw); dev_free(L.d_sws); dev_free(L.d_sws32);
    L.d_sws = nullptr;
-   L.sws_valid = false;
+   L.d_sws32 = nullptr;
+   L.sws_valid = L.sws32_valid = false;
    L.d_nbr = nullptr; L.d_bnd = nullptr; L.d_x0 = nullptr;
    L.d_ud = nullptr; L.d_u = nullptr; L.d_swd = nullptr; L.d_sw = nullptr;
    L.ws.clear(); L.wsd.clear(); L.ws_ptrs.clear(); L.wsd_ptrs.clear();
@@ -640,7 +641,7 @@ extern "C" void oqh_import_swd(const pauli_dble *swd)
    HIP_OK(hipMemcpyAsync(aos, swd, bytes, hipMemcpyHostToDevice, L.stream));
    aos_to_blocks_f64((const double2 *)aos, L.d_swd, L.g.V, 36, L.stream);
    HIP_OK(hipStreamSynchronize(L.stream));
-   L.sws_valid = false;  // an imported term has no S-form
+   L.sws_valid = L.sws32_valid = false;  // an imported term has no S-form
    L.swd_valid = true;
    L.swd_inv[0] = L.swd_inv[1] = false;
    L.sw_valid = false;

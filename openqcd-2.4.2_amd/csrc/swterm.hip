@@ -389,6 +389,7 @@ static void compute_swd()
    a.c3b = a.c1 + L.cF[1] - 1.0;
    k_sw_term<<<(unsigned)((L.g.V + 127) / 128), 128, 0, L.stream>>>(a);
    L.sws_valid = L.d_sws != nullptr;
+   L.sws32_valid = false;
    L.swd_valid = true;
    L.swd_inv[0] = L.swd_inv[1] = false;
    L.sw_valid = false;
@@ -432,6 +433,15 @@ void ensure_sw()
    if (!L.swd_valid) ensure_swd(0, 0);
    if (L.sw_valid) return;
    convert_f64_to_f32(L.d_swd, L.d_sw, units_for(L.g.V, 36), L.stream);
+   if (L.sws_valid && !L.sws32_valid)
+   {
+      if (!L.d_sws32) L.d_sws32 = (float2 *)dev_alloc(units_for(L.g.V, 27) * sizeof(float2));
+      if (L.d_sws32)
+      {
+         convert_f64_to_f32(L.d_sws, L.d_sws32, units_for(L.g.V, 27), L.stream);
+         L.sws32_valid = true;
+      }
+   }
    L.sw_valid = true;
 }
 
