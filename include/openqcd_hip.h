@@ -239,6 +239,25 @@ void plaq_frc(void);
    Wilson flow with step eps on the double-precision gauge field */
 void fwd_euler(int n, double eps);
 
+/* include/uflds.h:33-34 (uflds/plaq_sum.c:155-191): sum of Re tr U(p) over
+   all unoriented plaquettes of the double-precision field; plaq_wsum_dble
+   gives the space-like plaquettes on the boundary slices of open bc weight
+   1/2.  Double-double sums on the device; icom has no effect (one process
+   holds the whole lattice). */
+double plaq_sum_dble(int icom);
+double plaq_wsum_dble(int icom);
+
+/* include/archive.h:34-36 (archive/archive.c:769-977, main/README.io):
+   gauge configurations in the universal little-endian format (lattice
+   sizes, average plaquette, then the 8 links of each odd point in
+   lexicographic order).  export_cnfg refuses a phase-set field; import_cnfg
+   requires the sizes on the file to be the current ones (the whole lattice
+   is local), checks the boundary links and the average plaquette, and
+   leaves the field without phase. */
+void export_cnfg(char *out);
+void import_cnfg(char *in, int mask);
+void lat_sizes(char *in, int *nl);
+
 /* include/linalg.h:70-79 (linalg/salg.c:1223-1429) */
 complex spinor_prod(int vol, int icom, spinor *s, spinor *r);
 float spinor_prod_re(int vol, int icom, spinor *s, spinor *r);

@@ -30,6 +30,7 @@
 #include <cmath>
 
 #include "dirac_dev.h"
+#include "ddsum.h"
 
 namespace oqh {
 
@@ -291,6 +292,60 @@ __global__ __launch_bounds__(256) void k_plaq_frc(FArgs a)
    for (int c = 0; c < 4; c++) f[c * SB] = make_double2(F[2 * c], F[2 * c + 1]);
 }
 
+// ---- plaquette sums (uflds/plaq_sum.c:69-175) ----
+
+// Re tr U(p) of the plaquette in plane n at point b: U(p) = U0 U1 U3^+ U2^+
+// with the links of plaq_uidx (lattice/uidx.c:29-40), evaluated as
+// Re tr{(U0 U1) (U2 U3)^+}
+__device__ inline double plaq_re(const FArgs &a, int b, int n)
+{
+   int p, q;
+   plane_pq(n, p, q);
+   Su3 U0 = link_at(a, b, p), U1 = link_at(a, up(a, b, p), q);
+   Su3 U2 = link_at(a, b, q), U3 = link_at(a, up(a, b, q), p);
+   Su3 w1 = mul(U0, U1), w2 = mul(U2, U3);
+   double s = 0.0;
+#pragma unroll
+   for (int k = 0; k < 9; k++) s += w1.e[k].x * w2.e[k].x + w1.e[k].y * w2.e[k].y;
+   return s;
+}
+
+// per point: the time-like planes except at x0 = N0-1 with open bc, the
+// space-like planes with weight wp on the boundary slices of open bc
+// (plaq_sum.c:116-146 for boundary types 0 and 3); double-double sum per
+// work-group
+__global__ __launch_bounds__(256) void k_plaq_partial(FArgs a, double wp, dd *part)
+{
+   dd acc = {0.0, 0.0};
+   for (int b = blockIdx.x * 256 + threadIdx.x; b < a.V; b += gridDim.x * 256)
+   {
+      const int t = a.x0[b];
+      double pa = 0.0;
+      if (t < a.N0 - 1 || a.bc != 0)
+         for (int n = 0; n < 3; n++) pa += plaq_re(a, b, n);
+      if ((t > 0 && t < a.N0 - 1) || a.bc == 3)
+      {
+         for (int n = 3; n < 6; n++) pa += plaq_re(a, b, n);
+      }
+      else
+      {
+         for (int n = 3; n < 6; n++) pa += wp * plaq_re(a, b, n);
+      }
+      acc = dd_add(acc, dd{pa, 0.0});
+   }
+   dd v[1] = {acc};
+   block_reduce_dd<1>(v);
+   if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(256) void k_plaq_final(const dd *part, int nb, dd *out)
+{
+   dd v[1] = {{0.0, 0.0}};
+   for (int k = threadIdx.x; k < nb; k += 256) v[0] = dd_add(v[0], part[k]);
+   block_reduce_dd<1>(v);
+   if (threadIdx.x == 0) out[0] = v[0];
+}
+
 // ---- SU(3) exponential, Cayley-Hamilton form (chexp.c) ----
 
 struct ChexpCoef
@@ -484,7 +539,7 @@ static ChexpCoef chexp_coef()
 static double2 *d_frc = nullptr;
 static size_t frc_bytes = 0;
 
-static FArgs fargs()
+static FArgs fargs(bool need_frc = true)
 {
    Lib &L = lib();
    FArgs a;
@@ -497,6 +552,8 @@ static FArgs fargs()
    a.N0 = L.g.L[0];
    a.bc = L.bc;
    a.eps = 0.0;
+   a.frc = nullptr;
+   if (!need_frc) return a;
    size_t need = units_for(a.h, 32) * sizeof(double2);
    if (need > frc_bytes)
    {
@@ -604,4 +661,39 @@ extern "C" void unset_ud_phase(void)
       L.u_valid = false;
       L.swd_valid = L.sw_valid = false;
    }
+}
+
+// uflds/plaq_sum.c:155-191: sums of Re tr U(p) over all unoriented plaquettes
+// (plaq_wsum_dble: space-like plaquettes on the boundary slices of open bc
+// with weight 1/2).  One process holds the whole lattice, so icom has no
+// effect.
+static double plaq_sum(double wp, const char *where)
+{
+   Lib &L = lib();
+   if (!flow_ready(where)) return 0.0;
+   static dd *d_part = nullptr, *h_out = nullptr;
+   const int nbmax = 1024;
+   if (!d_part) d_part = (dd *)dev_alloc(sizeof(dd) * (nbmax + 1));
+   if (!h_out) HIP_OK(hipHostMalloc((void **)&h_out, sizeof(dd), 0));
+   if (!d_part || !h_out) return 0.0;
+   FArgs a = oqh::fargs(false);
+   int nb = (a.V + 255) / 256;
+   nb = nb > nbmax ? nbmax : nb;
+   oqh::k_plaq_partial<<<nb, 256, 0, L.stream>>>(a, wp, d_part);
+   oqh::k_plaq_final<<<1, 256, 0, L.stream>>>(d_part, nb, d_part + nbmax);
+   HIP_OK(hipMemcpyAsync(h_out, d_part + nbmax, sizeof(dd), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   return h_out->hi;
+}
+
+extern "C" double plaq_sum_dble(int icom)
+{
+   (void)icom;
+   return plaq_sum(1.0, "plaq_sum_dble [plaq_sum.c]");
+}
+
+extern "C" double plaq_wsum_dble(int icom)
+{
+   (void)icom;
+   return plaq_sum(0.5, "plaq_wsum_dble [plaq_sum.c]");
 }

@@ -57,6 +57,9 @@ _SIG = {
     "sw_term": (_I, [_I]), "assign_swd2sw": (None, []), "assign_ud2u": (None, []),
     "oqh_copy_bandwidth": (_D, [_D, _I]), "oqh_export_frc": (None, [P]),
     "plaq_frc": (None, []), "fwd_euler": (None, [_I, _D]),
+    "plaq_sum_dble": (_D, [_I]), "plaq_wsum_dble": (_D, [_I]),
+    "export_cnfg": (None, [C.c_char_p]), "import_cnfg": (None, [C.c_char_p, _I]),
+    "lat_sizes": (None, [C.c_char_p, P]),
     "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
     "oqh_set_sap_prec": (None, [_I]),
     "oqh_set_gcr_orth": (None, [_I]),

@@ -11,6 +11,7 @@
  *                    377-390 (eval_td), 697-729 (chexp_drv0), 878-929
  *                    (expXsu3)
  *   update           wflow/wflow.c:76-155 (update_ud), 217-249 (fwd_euler)
+ *   plaquette sums   uflds/plaq_sum.c:69-191 (plaq_sum_dble, plaq_wsum_dble)
  *
  * The arithmetic is written in the reference's operation order (left to right
  * sums, same association of products), so that with contraction of
@@ -218,6 +219,87 @@ void oc_plaq_frc(oc_ctx *c, double *frc)
          frc_part(c, frc, ofs, vol);
          frc_part(c, frc, ofs + c->V / 2, vol);
       }
+}
+
+/* ---- plaquette sums: uflds/plaq_sum.c:69-191 ---- */
+
+/* w = u^dag v^dag: column j is u^dag applied to the conjugate of row j of v
+   (su3dagxsu3dag, include/su3fcts.h:235-272) */
+static void su3dagxsu3dag(const double *u, const double *v, double *w)
+{
+   double p[6], q[6];
+   for (int j = 0; j < 3; j++)
+   {
+      for (int k = 0; k < 3; k++)
+      {
+         p[2 * k] = v[6 * j + 2 * k];
+         p[2 * k + 1] = -v[6 * j + 2 * k + 1];
+      }
+      mulv_dag(u, p, q);
+      put_col(w, j, q);
+   }
+}
+
+/* Re tr(u v), summed as cm3x3_retr (include/su3fcts.h:69-88) */
+static double retr(const double *u, const double *v)
+{
+   double r = 0.0;
+   for (int i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++)
+      {
+         const double *a = u + 6 * i + 2 * k, *b = v + 6 * k + 2 * i;
+         double t = a[0] * b[0] - a[1] * b[1];
+         if (i == 0 && k == 0)
+            r = t;
+         else
+            r += t;
+      }
+   return r;
+}
+
+/* plaq_dble (plaq_sum.c:69-81) with plaq_uidx (uidx.c:237-290, NPROC=1) */
+static double plaq_dble(const oc_ctx *c, int n, int ix)
+{
+   int mu = PLNS[n][0], nu = PLNS[n][1];
+   int ip0 = oc_link_fwd(c, ix, mu), ip1 = oc_link_fwd(c, c->iup[ix][mu], nu);
+   int ip2 = oc_link_fwd(c, ix, nu), ip3 = oc_link_fwd(c, c->iup[ix][nu], mu);
+   double wd1[18], wd2[18];
+   su3xsu3(c->ud + 18 * ip0, c->ud + 18 * ip1, wd1);
+   su3dagxsu3dag(c->ud + 18 * ip3, c->ud + 18 * ip2, wd2);
+   return retr(wd1, wd2);
+}
+
+/* local_plaq_sum_dble (plaq_sum.c:84-152) for boundary types 0 and 3: the
+   points of each thread range ix in [k VT, (k+1) VT) in order, one acc_qflt
+   per point, then the thread sums in thread order; iw = 1 gives the
+   space-like plaquettes on the boundary slices of open bc weight 1/2 */
+double oc_plaq_sum(const oc_ctx *c, int iw)
+{
+   const double wp = iw ? 0.5 : 1.0;
+   const int N0 = c->L[0], bc = c->bc;
+   double tot[2] = {0.0, 0.0};
+   for (int k = 0; k < c->nthread; k++)
+   {
+      double q[2] = {0.0, 0.0};
+      for (int ix = k * c->VT; ix < (k + 1) * c->VT; ix++)
+      {
+         int t = c->x0[ix];
+         double pa = 0.0;
+         if (t < N0 - 1 || bc != 0)
+            for (int n = 0; n < 3; n++) pa += plaq_dble(c, n, ix);
+         if ((t > 0 && t < N0 - 1) || bc == 3)
+         {
+            for (int n = 3; n < 6; n++) pa += plaq_dble(c, n, ix);
+         }
+         else
+         {
+            for (int n = 3; n < 6; n++) pa += wp * plaq_dble(c, n, ix);
+         }
+         oc_acc_qflt(pa, q);
+      }
+      oc_add_qflt(tot, q, tot);
+   }
+   return tot[0];
 }
 
 /* ---- SU(3) exponential (Cayley-Hamilton representation) ---- */

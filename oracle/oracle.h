@@ -66,6 +66,7 @@ float *oc_ufld(oc_ctx *c);
 void oc_plaq_frc(oc_ctx *c, double *frc);            /* 4V su3_alg_dble (8 doubles) */
 void oc_expXsu3(double eps, const double *X, double *u);
 void oc_fwd_euler(oc_ctx *c, int n, double eps);     /* acts on the FP64 field  */
+double oc_plaq_sum(const oc_ctx *c, int iw);        /* plaq_sum_dble (iw=0), plaq_wsum_dble (iw=1) */
 
 /* ---- spinor fields (sflds/sflds.c, sflds/unorm.c, lattice/bcnds.c) ---- */
 void oc_random_sd(oc_ctx *c, double *sd, double sigma);

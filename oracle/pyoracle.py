@@ -38,6 +38,7 @@ def lib():
             "oc_start_ranlux": (None, [P, i, i]),
             "oc_kat_ranlxs": (None, [i, i, P, i]), "oc_kat_ranlxd": (None, [i, i, P, i]),
             "oc_random_ud": (None, [P]), "oc_set_ud_phase": (None, [P]),
+            "oc_plaq_sum": (d, [P, i]),
             "oc_udfld": (P, [P]), "oc_set_ud": (None, [P, P]),
             "oc_assign_ud2u": (None, [P]), "oc_ufld": (P, [P]),
             "oc_random_sd": (None, [P, P, d]), "oc_random_s": (None, [P, P, f]),
@@ -147,6 +148,17 @@ class Oracle:
         f = np.empty((4 * self.V, 8), np.float64)
         lib().oc_plaq_frc(self.h, ptr(f))
         return f
+
+    def plaq_sum(self, iw=0):
+        """plaq_sum_dble (iw=0) / plaq_wsum_dble (iw=1) of the FP64 field."""
+        return lib().oc_plaq_sum(self.h, iw)
+
+    def cnfg_bytes(self):
+        """The field as export_cnfg writes it (archive/archive.c:804-870,
+        main/README.io): int32 N0..N3, the average plaquette, then the 8
+        links of every odd point in lexicographic order (x3 fastest)."""
+        return cnfg_encode(self.L, self.ipt(), self.ud(),
+                           self.plaq_sum(0) / (6.0 * np.prod(self.L, dtype=np.float64)))
 
     def fwd_euler(self, n, eps):
         lib().oc_fwd_euler(self.h, n, eps)
@@ -310,6 +322,34 @@ def add_qflt(qu, qv):
     qr = np.zeros(2)
     lib().oc_add_qflt(ptr(qu), ptr(qv), ptr(qr))
     return qr
+
+
+def odd_lex_order(L, ipt):
+    """ipt indices of the odd points in lexicographic order (x3 fastest);
+    ipt is indexed by x3 + L3*(x2 + L2*(x1 + L1*x0))."""
+    x = np.indices(tuple(L)).reshape(4, -1)
+    odd = (x.sum(axis=0) & 1) == 1
+    return np.asarray(ipt)[np.flatnonzero(odd)]
+
+
+def cnfg_encode(L, ipt, ud, plaq):
+    """Bytes of a configuration file in the universal format."""
+    L = [int(v) for v in L]
+    h = int(np.prod(L)) // 2
+    ix = odd_lex_order(L, ipt)
+    links = np.asarray(ud, "<f8").reshape(h, 8, 18)[ix - h]
+    return (np.array(L, "<i4").tobytes() + np.array([plaq], "<f8").tobytes() + links.tobytes())
+
+
+def cnfg_decode(data, ipt):
+    """(sizes, average plaquette, ud in the layout of udfld()) of a file."""
+    L = np.frombuffer(data[:16], "<i4").astype(int)
+    plaq = float(np.frombuffer(data[16:24], "<f8")[0])
+    h = int(np.prod(L)) // 2
+    links = np.frombuffer(data[24:], "<f8").reshape(h, 8, 18)
+    ud = np.empty((h, 8, 18))
+    ud[odd_lex_order(L, ipt) - h] = links
+    return L, plaq, ud.reshape(8 * h, 18)
 
 
 def ref_exe(tag):

@@ -8,6 +8,8 @@
  * Usage:
  *   ref_<tag> dump  OUTDIR [options]   write fields and operator outputs
  *   ref_<tag> bench [options]          time Dw_dble/Dw/Dwhat(_dble), print JSON
+ *   ref_<tag> cnfg IN OUT [options]    import_cnfg(IN), print the plaquette
+ *                                      sums, export_cnfg(OUT)
  * Options: -bc T -cF a b -csw c -m0 m -seed s -mu mu -eoflg e
  *          -bs b0 b1 b2 b3 -isolv i -nmr n -ncy n -nkv n -nmx n -res r
  *          -secs t (bench: target seconds per operator)
@@ -43,6 +45,11 @@
 #include "wflow.h"
 #include "global.h"
 
+/* include/archive.h:34-36 (the MPI include directory holds another
+   archive.h, so the prototypes are declared here) */
+extern void export_cnfg(char *out);
+extern void import_cnfg(char *in, int mask);
+
 static char outdir[512];
 
 static void wr(const char *name, const void *p, size_t nbytes)
@@ -67,11 +74,20 @@ int main(int argc, char *argv[])
 
    if (argc < 2) { fprintf(stderr, "usage: %s dump OUTDIR|bench [options]\n", argv[0]); return 1; }
    i = 2;
+   const char *cnfg_in = NULL, *cnfg_out = NULL;
    if (strcmp(argv[1], "dump") == 0)
    {
       if (argc < 3) return 1;
       snprintf(outdir, sizeof(outdir), "%s", argv[2]);
       i = 3;
+   }
+   else if (strcmp(argv[1], "cnfg") == 0)
+   {
+      if (argc < 4) return 1;
+      cnfg_in = argv[2];
+      cnfg_out = argv[3];
+      dump = 0;
+      i = 4;
    }
    else
       dump = 0;
@@ -110,6 +126,16 @@ int main(int argc, char *argv[])
    spinor_dble **wsd = reserve_wsd(8);
    spinor **ws = reserve_ws(8);
 
+   if (cnfg_in)
+   {
+      /* ---------------- configuration file round trip -------------------- */
+      import_cnfg((char *)cnfg_in, 0);
+      printf("{\"plaq_sum\": %.17e, \"plaq_wsum\": %.17e}\n", plaq_sum_dble(1), plaq_wsum_dble(1));
+      export_cnfg((char *)cnfg_out);
+      MPI_Finalize();
+      return 0;
+   }
+
    random_ud();
    if (dump && nflow > 0)
    {
@@ -117,6 +143,15 @@ int main(int argc, char *argv[])
       wr("frc0.f64", (*mdflds()).frc, 4 * VOLUME * sizeof(su3_alg_dble));
       fwd_euler(nflow, eflow);
       wr("ud_flow.f64", udfld(), 4 * VOLUME * sizeof(su3_dble));
+   }
+   if (dump)
+   {
+      /* the field before the phase: plaquette sums and the exported file */
+      double pq[2] = {plaq_sum_dble(1), plaq_wsum_dble(1)};
+      char path[700];
+      wr("plaq.f64", pq, sizeof(pq));
+      snprintf(path, sizeof(path), "%s/cnfg.dat", outdir);
+      export_cnfg(path);
    }
    set_ud_phase();
 
