@@ -257,6 +257,14 @@ double plaq_wsum_dble(int icom);
 void export_cnfg(char *out);
 void import_cnfg(char *in, int mask);
 void lat_sizes(char *in, int *nl);
+/* include/archive.h:64-65 (archive/sarchive.c:424-580): a double-precision
+   quark field in the universal format (sizes, eo, square norm, then the
+   spinors of all points (eo=0) or of the even points (eo=1) in
+   lexicographic order).  sd is a device field of the library or a host
+   array; import_sfld checks the sizes, eo and the square norm and, with
+   eo=1, leaves the odd points of sd unchanged. */
+void export_sfld(char *out, int eo, spinor_dble *sd);
+void import_sfld(char *in, int eo, spinor_dble *sd);
 
 /* include/linalg.h:70-79 (linalg/salg.c:1223-1429) */
 complex spinor_prod(int vol, int icom, spinor *s, spinor *r);

@@ -1,5 +1,7 @@
-// archive.hip -- exchange of gauge-field configurations in openQCD's
-// "universal" file format (archive/archive.c:769-977, main/README.io:39-85).
+// archive.hip -- exchange of gauge-field configurations and of
+// double-precision quark fields in openQCD's "universal" file format
+// (archive/archive.c:769-977, archive/sarchive.c:424-580,
+// main/README.io:39-85).
 //
 // File layout (little endian): the lattice sizes N0..N3 as 4-byte integers,
 // the average plaquette plaq_sum_dble()/(6 N0 N1 N2 N3) as a double, then,
@@ -8,8 +10,13 @@
 // whole lattice in one process the records of a point are the 8 consecutive
 // entries of udfld() at 8*(ipt[x]-V/2).
 //
-// The field moves through host memory in the layout of udfld(); the
-// plaquette is computed on the device.
+// Quark-field files: the sizes N0..N3 and eo (0: all points, 1: the even
+// points only) as 4-byte integers, the square norm norm_square_dble() of the
+// written points as a double, then the spinors (24 doubles) of the written
+// points in lexicographic order.
+//
+// The fields move through host memory in the reference's layouts; the
+// plaquette and the square norm are computed on the device.
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -20,6 +27,7 @@
 #include "oqh_internal.h"
 
 extern "C" double plaq_sum_dble(int icom);
+extern "C" qflt norm_square_dble(int vol, int icom, spinor_dble *sd);
 
 namespace oqh {
 namespace {
@@ -240,4 +248,145 @@ extern "C" void import_cnfg(char *in, int mask)
    double eps = sqrt(64.0 * nplaq) * DBL_EPSILON;
    eps += 64.0 * (fabs(plaq0) + fabs(plaq1)) * DBL_EPSILON;
    if (fabs(plaq1 - plaq0) > eps) fail(where, "Incorrect average plaquette");
+}
+
+// sarchive.c:424-503.  sd is a device field of the library or a host array
+// in the layout of the reference (ipt order, even points first).
+extern "C" void export_sfld(char *out, int eo, spinor_dble *sd)
+{
+   Lib &L = lib();
+   const char *where = "export_sfld [sarchive.c]";
+   if (!require_geometry(where)) return;
+   if (eo < 0 || eo > 1)
+   {
+      fail(where, "Parameter eo is out of range");
+      return;
+   }
+   if (sd == nullptr)
+   {
+      fail(where, "Attempt to access unallocated memory space");
+      return;
+   }
+   const int *N = L.g.L, V = L.g.V, n = eo ? V / 2 : V;
+   const int vt = L.g.VT;
+   double norm = norm_square_dble(eo ? vt / 2 : vt, 3, sd).q[0];
+   std::vector<spinor_dble> buf;
+   const spinor_dble *src = sd;
+   int off = 0;
+   if (device_field(sd, F64, &off))
+   {
+      buf.resize((size_t)n);
+      oqh_export_sd(sd, buf.data(), n);
+      src = buf.data();
+   }
+   FILE *f = fopen(out, "wb");
+   if (f == nullptr)
+   {
+      fail(where, "Unable to open output file");
+      return;
+   }
+   const bool swap = host_is_big_endian();
+   int32_t lsize[5] = {N[0], N[1], N[2], N[3], eo};
+   if (swap)
+   {
+      swap_bytes(lsize, 4, 5);
+      swap_bytes(&norm, 8, 1);
+   }
+   size_t iw = fwrite(lsize, sizeof(int32_t), 5, f);
+   iw += fwrite(&norm, sizeof(double), 1, f);
+   bool bad = iw != 6;
+   std::vector<spinor_dble> line((size_t)N[3]);
+   int x[4];
+   for (x[0] = 0; x[0] < N[0] && !bad; x[0]++)
+      for (x[1] = 0; x[1] < N[1]; x[1]++)
+         for (x[2] = 0; x[2] < N[2]; x[2]++)
+         {
+            int m = 0;
+            for (x[3] = eo ? (x[0] + x[1] + x[2]) & 1 : 0; x[3] < N[3]; x[3] += eo ? 2 : 1)
+               line[m++] = src[L.g.ipt[lex_index(N, x)]];
+            if (swap) swap_bytes(line.data(), 8, (size_t)m * 24);
+            bad |= fwrite(line.data(), sizeof(spinor_dble), (size_t)m, f) != (size_t)m;
+         }
+   fclose(f);
+   if (bad) fail(where, "Incorrect write count");
+}
+
+// sarchive.c:522-580
+extern "C" void import_sfld(char *in, int eo, spinor_dble *sd)
+{
+   Lib &L = lib();
+   const char *where = "import_sfld [sarchive.c]";
+   if (!require_geometry(where)) return;
+   if (eo < 0 || eo > 1)
+   {
+      fail(where, "Parameter eo is out of range");
+      return;
+   }
+   if (sd == nullptr)
+   {
+      fail(where, "Attempt to access unallocated memory space");
+      return;
+   }
+   const int *N = L.g.L, V = L.g.V, n = eo ? V / 2 : V;
+   const int vt = L.g.VT;
+   FILE *f = fopen(in, "rb");
+   if (f == nullptr)
+   {
+      fail(where, "Unable to open input file");
+      return;
+   }
+   const bool swap = host_is_big_endian();
+   int32_t lsize[5];
+   double norm0;
+   size_t ir = fread(lsize, sizeof(int32_t), 5, f);
+   ir += fread(&norm0, sizeof(double), 1, f);
+   if (ir != 6)
+   {
+      fclose(f);
+      fail(where, "Incorrect read count");
+      return;
+   }
+   if (swap)
+   {
+      swap_bytes(lsize, 4, 5);
+      swap_bytes(&norm0, 8, 1);
+   }
+   if (lsize[0] != N[0] || lsize[1] != N[1] || lsize[2] != N[2] || lsize[3] != N[3] || lsize[4] != eo)
+   {
+      fclose(f);
+      fail(where, "Lattice sizes do not match");
+      return;
+   }
+   int off = 0;
+   const bool dev = device_field(sd, F64, &off) != nullptr;
+   std::vector<spinor_dble> buf;
+   spinor_dble *dst = sd;
+   if (dev)
+   {
+      buf.resize((size_t)n);
+      dst = buf.data();
+   }
+   std::vector<spinor_dble> line((size_t)N[3]);
+   bool bad = false;
+   int x[4];
+   for (x[0] = 0; x[0] < N[0] && !bad; x[0]++)
+      for (x[1] = 0; x[1] < N[1]; x[1]++)
+         for (x[2] = 0; x[2] < N[2]; x[2]++)
+         {
+            const int x3a = eo ? (x[0] + x[1] + x[2]) & 1 : 0, m = eo ? N[3] / 2 : N[3];
+            bad |= fread(line.data(), sizeof(spinor_dble), (size_t)m, f) != (size_t)m;
+            if (swap) swap_bytes(line.data(), 8, (size_t)m * 24);
+            int k = 0;
+            for (x[3] = x3a; x[3] < N[3]; x[3] += eo ? 2 : 1) dst[L.g.ipt[lex_index(N, x)]] = line[k++];
+         }
+   fclose(f);
+   if (bad)
+   {
+      fail(where, "Incorrect read count");
+      return;
+   }
+   if (dev) oqh_import_sd(sd, buf.data(), n);
+   double norm1 = norm_square_dble(eo ? vt / 2 : vt, 3, sd).q[0];
+   double eps = 64.0 * (norm0 + norm1) * DBL_EPSILON;
+   if (fabs(norm1 - norm0) > eps) fail(where, "Incorrect square norm");
 }

@@ -60,6 +60,7 @@ _SIG = {
     "plaq_sum_dble": (_D, [_I]), "plaq_wsum_dble": (_D, [_I]),
     "export_cnfg": (None, [C.c_char_p]), "import_cnfg": (None, [C.c_char_p, _I]),
     "lat_sizes": (None, [C.c_char_p, P]),
+    "export_sfld": (None, [C.c_char_p, _I, P]), "import_sfld": (None, [C.c_char_p, _I, P]),
     "set_ud_phase": (None, []), "unset_ud_phase": (None, []),
     "oqh_set_sap_prec": (None, [_I]),
     "oqh_set_gcr_orth": (None, [_I]),

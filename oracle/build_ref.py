@@ -37,7 +37,7 @@ sflds/unorm su3fcts/chexp su3fcts/su3prod su3fcts/su3ren su3fcts/cm3x3 su3fcts/r
 sw_term/pauli sw_term/pauli_dble sw_term/swalg sw_term/swexp sw_term/swflds sw_term/sw_term
 tcharge/ftcom tcharge/ftensor uflds/plaq_sum uflds/shift uflds/udcom uflds/uflds
 uflds/uinit utils/endian utils/error utils/mutils utils/qsum utils/utils utils/wspace
-forces/force0 mdflds/mdflds mdflds/fcom wflow/wflow uflds/bstap archive/archive""".split()
+forces/force0 mdflds/mdflds mdflds/fcom wflow/wflow uflds/bstap archive/archive archive/sarchive""".split()
 
 # tag: (L[4], Ltrd[4], simd)   simd=False -> portable C branches (parity)
 #                              simd=True  -> -DAVX -DFMA3 -DPM (timing)
@@ -74,8 +74,10 @@ def build(tag, jobs=8):
     gh = global_h(tag)
     simd = CONFIGS[tag][2]
     objdir = os.path.dirname(gh)
+    # the reference headers first: the MPI include directory also holds an
+    # unrelated archive.h
     flags = ["-std=gnu11", "-O2", "-fopenmp", "-w", "-include", gh,
-             "-I" + MPI_INC, "-I" + os.path.join(REF, "include")]
+             "-I" + os.path.join(REF, "include"), "-I" + MPI_INC]
     if simd:
         flags += ["-DAVX", "-DFMA3", "-DPM", "-mavx2", "-mfma"]
     else:

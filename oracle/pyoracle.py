@@ -341,6 +341,36 @@ def cnfg_encode(L, ipt, ud, plaq):
     return (np.array(L, "<i4").tobytes() + np.array([plaq], "<f8").tobytes() + links.tobytes())
 
 
+def sfld_encode(L, ipt, sd, eo=0):
+    """Bytes of a quark-field file (archive/sarchive.c:424-503): int32
+    N0..N3 and eo, the square norm of the written points (norm_square_dble,
+    double-double, high part), then their spinors in lexicographic order
+    (eo=1: the even points only)."""
+    L = [int(v) for v in L]
+    x = np.indices(tuple(L)).reshape(4, -1)
+    sel = np.ones(x.shape[1], bool) if eo == 0 else (x.sum(axis=0) & 1) == 0
+    ix = np.asarray(ipt)[np.flatnonzero(sel)]
+    n = len(ix)
+    qn = norm_square_dble(np.ascontiguousarray(np.asarray(sd, np.float64)[:n]))
+    return (np.array(L + [eo], "<i4").tobytes() + np.array([qn[0]], "<f8").tobytes()
+            + np.asarray(sd, "<f8")[ix].tobytes())
+
+
+def sfld_decode(data, ipt, V):
+    """(sizes, eo, square norm, points written as a dict-free array): the
+    spinors placed at their ipt positions in a (V, 24) array (NaN where the
+    file holds no point)."""
+    hdr = np.frombuffer(data[:20], "<i4").astype(int)
+    L, eo = hdr[:4], int(hdr[4])
+    norm = float(np.frombuffer(data[20:28], "<f8")[0])
+    x = np.indices(tuple(L)).reshape(4, -1)
+    sel = np.ones(x.shape[1], bool) if eo == 0 else (x.sum(axis=0) & 1) == 0
+    ix = np.asarray(ipt)[np.flatnonzero(sel)]
+    sd = np.full((V, 24), np.nan)
+    sd[ix] = np.frombuffer(data[28:], "<f8").reshape(-1, 24)
+    return L, eo, norm, sd
+
+
 def cnfg_decode(data, ipt):
     """(sizes, average plaquette, ud in the layout of udfld()) of a file."""
     L = np.frombuffer(data[:16], "<i4").astype(int)

@@ -43,12 +43,8 @@
 #include "mdflds.h"
 #include "forces.h"
 #include "wflow.h"
+#include "archive.h"
 #include "global.h"
-
-/* include/archive.h:34-36 (the MPI include directory holds another
-   archive.h, so the prototypes are declared here) */
-extern void export_cnfg(char *out);
-extern void import_cnfg(char *in, int mask);
 
 static char outdir[512];
 
@@ -200,6 +196,13 @@ int main(int argc, char *argv[])
    random_sd(VOLUME_TRD, 2, eta, 1.0);
    bnd_sd2zero(ALL_PTS, eta);
    wr("eta.f64", eta, sizeof(spinor_dble) * V);
+   {
+      char path[700];
+      snprintf(path, sizeof(path), "%s/eta_sfld0.dat", outdir);
+      export_sfld(path, 0, eta);
+      snprintf(path, sizeof(path), "%s/eta_sfld1.dat", outdir);
+      export_sfld(path, 1, eta);
+   }
    random_s(VOLUME_TRD, 2, ws[0], 1.0f);
    bnd_s2zero(ALL_PTS, ws[0]);
    wr("s32.f32", ws[0], sizeof(spinor) * V);

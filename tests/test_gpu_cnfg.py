@@ -1,7 +1,8 @@
 """Plaquette sums (plaq_sum_dble, plaq_wsum_dble) and the exchange of gauge
-configurations (export_cnfg, import_cnfg, lat_sizes) through the C-ABI,
-against the oracle: its plaquette sums and its encoder of the file format
-(both bit-exact with the reference build, tests/test_oracle_cnfg.py).
+configurations (export_cnfg, import_cnfg, lat_sizes) and quark fields
+(export_sfld, import_sfld) through the C-ABI, against the oracle: its
+plaquette sums and its encoders of the file formats (bit-exact with the
+reference build, tests/test_oracle_cnfg.py).
 
 The sums are double-double reductions of per-point values on the device; the
 per-plaquette products round as the reference's only up to the order of the
@@ -137,6 +138,64 @@ def test_export_import(tmp_path, bc):
         lat.lib.import_cnfg(bpath.encode(), 0)
         n, msg = last_error(lat)
         assert n == 1 and "Incompatible boundary conditions" in msg
+    finally:
+        lat.lib.oqh_set_error_mode(1)
+        lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("eo", [0, 1])
+def test_sfld_export_import(tmp_path, eo):
+    print("quark-field files, eo", eo, flush=True)
+    L, T = LAT
+    o = po.Oracle(L, T, bc=3)
+    o.start_ranlux(0, 4)
+    eta = o.random_sd(1.0)
+    ipt = o.ipt()
+    ref = po.sfld_encode(L, ipt, eta, eo)
+    lat = pyoqh.Lattice(L, T, bc=3)
+    try:
+        V = lat.V
+        n = V if eo == 0 else V // 2
+        s, r = lat.alloc_sd(), lat.alloc_sd()
+        lat.put_sd(s, eta)
+        path = str(tmp_path / "dev.sfld")
+        # device field and host array give the same file
+        for arg in (s, eta.ctypes.data_as(C.c_void_p)):
+            lat.lib.export_sfld(path.encode(), eo, arg)
+            data = open(path, "rb").read()
+            assert len(data) == len(ref) and data[:20] == ref[:20] and data[28:] == ref[28:]
+            nd = np.frombuffer(data[20:28], "<f8")[0]
+            nr = np.frombuffer(ref[20:28], "<f8")[0]
+            assert abs(nd - nr) <= 1e-14 * nr
+        # the oracle's file into a device field and into a host array; with
+        # eo=1 the odd points keep their values
+        opath = str(tmp_path / "oracle.sfld")
+        open(opath, "wb").write(ref)
+        fill = np.random.default_rng(1).standard_normal((V, 24))
+        lat.put_sd(r, fill)
+        lat.lib.import_sfld(opath.encode(), eo, r)
+        got = lat.get_sd(r)
+        assert np.array_equal(got[:n], eta[:n]) and np.array_equal(got[n:], fill[n:])
+        host = fill.copy()
+        lat.lib.import_sfld(opath.encode(), eo, host.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(host[:n], eta[:n]) and np.array_equal(host[n:], fill[n:])
+
+        lat.lib.oqh_set_error_mode(0)
+        last_error(lat)
+        lat.lib.import_sfld(opath.encode(), 1 - eo, r)
+        n_err, msg = last_error(lat)
+        assert n_err == 1 and "Lattice sizes do not match" in msg
+        bad = bytearray(ref)
+        bad[20:28] = np.array([nr * (1.0 + 1e-10)], "<f8").tobytes()
+        bpath = str(tmp_path / "bad.sfld")
+        open(bpath, "wb").write(bytes(bad))
+        lat.lib.import_sfld(bpath.encode(), eo, r)
+        n_err, msg = last_error(lat)
+        assert n_err == 1 and "Incorrect square norm" in msg
+        lat.lib.export_sfld(path.encode(), 2, s)
+        n_err, msg = last_error(lat)
+        assert n_err == 1 and "out of range" in msg
     finally:
         lat.lib.oqh_set_error_mode(1)
         lat.close()

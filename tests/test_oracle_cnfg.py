@@ -1,5 +1,6 @@
-"""The oracle's plaquette sums (uflds/plaq_sum.c) and its encoder of
-configuration files (archive/archive.c:804-870, main/README.io) against the
+"""The oracle's plaquette sums (uflds/plaq_sum.c) and its encoders of
+configuration and quark-field files (archive/archive.c:804-870,
+archive/sarchive.c:424-503, main/README.io) against the
 reference build (bit-exact: file bytes and sums on the reference's own field)
 and against what the mathematics fixes: the brute-force plaquette sum of the
 test helpers, closed forms on the unit field, gauge invariance."""
@@ -121,3 +122,21 @@ def test_codec_round_trip_and_order():
     assert np.array_equal(first, o.ud()[8 * (ipt[1] - h):8 * (ipt[1] - h) + 8])
     second = np.frombuffer(data[24 + 8 * 144:24 + 16 * 144], "<f8").reshape(8, 18)
     assert np.array_equal(second, o.ud()[8 * (ipt[3] - h):8 * (ipt[3] - h) + 8])
+
+
+@pytest.mark.parametrize("tag,L,T,bc", CASES)
+def test_sfld_against_reference_export(tmp_path, tag, L, T, bc):
+    if po.ref_exe(tag) is None:
+        pytest.skip("reference build not present")
+    ld = po.ref_dump(tag, str(tmp_path), bc=bc, seed=6)
+    eta = ld("eta.f64", np.float64, (-1, 24))
+    ipt = ld("ipt.i32", np.int32)
+    V = int(np.prod(L))
+    for eo in (0, 1):
+        raw = open(os.path.join(str(tmp_path), "eta_sfld%d.dat" % eo), "rb").read()
+        assert po.sfld_encode(L, ipt, eta, eo) == raw
+        nl, eo_f, norm, sd = po.sfld_decode(raw, ipt, V)
+        assert tuple(nl) == L and eo_f == eo
+        n = V if eo == 0 else V // 2
+        assert np.array_equal(sd[:n], eta[:n]) and np.isnan(sd[n:]).all()
+        assert norm == po.norm_square_dble(np.ascontiguousarray(eta[:n]))[0]
