@@ -329,8 +329,13 @@ void assign_ud2ubgr(blk_grid_t grid);
 int assign_swd2swbgr(blk_grid_t grid, ptset_t set);
 void assign_s2sblk(blk_grid_t grid, int n, ptset_t set, spinor *s, int k);
 void assign_sblk2s(blk_grid_t grid, int n, ptset_t set, int k, spinor *s);
-/* include/dirac.h:34-39 (dirac/Dw.c:1475-1907) */
+/* include/dirac.h:34-39 (dirac/Dw.c:1475-1907); the even/odd pieces accept
+   k == l */
 void Dw_blk(blk_grid_t grid, int n, float mu, int k, int l);
+void Dwee_blk(blk_grid_t grid, int n, float mu, int k, int l);
+void Dwoo_blk(blk_grid_t grid, int n, float mu, int k, int l);
+void Dwoe_blk(blk_grid_t grid, int n, int k, int l);
+void Dweo_blk(blk_grid_t grid, int n, int k, int l);
 void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l);
 
 /* include/sap.h:21-35 (sap/blk_solv.c:1117-1186, sap/sap.c:720,

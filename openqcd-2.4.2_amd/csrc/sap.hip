@@ -1108,7 +1108,7 @@ __global__ __launch_bounds__(256) void k_sap_faces(SArgs a0, const int *face, co
 // block operators on the block fields of the API (one work-group per call)
 // ---------------------------------------------------------------------------
 
-enum BlkOp { BOP_DW = 0, BOP_DWHAT, BOP_MRES, BOP_EO_MRES };
+enum BlkOp { BOP_DW = 0, BOP_DWHAT, BOP_MRES, BOP_EO_MRES, BOP_DWEE, BOP_DWOO, BOP_DWOE, BOP_DWEO };
 
 __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, int l, float mu)
 {
@@ -1116,14 +1116,16 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
    float2 *F = a.scr + (size_t)n * 3 * a.Ub;  // the block's fields 0,1,2
    float2 *Sk = F + (size_t)k * a.Ub, *Sl = F + (size_t)l * a.Ub;
    // open bc: the operators zero their input on the boundary points (all of
-   // them for Dw_blk, the even ones for Dwhat_blk); in the MinRes solvers
-   // the input is field 1
+   // them for Dw_blk, the even ones for Dwhat_blk, Dwee_blk and Dwoe_blk,
+   // the odd ones for Dwoo_blk and Dweo_blk); in the MinRes solvers the
+   // input is field 1
    if (a.masked)
    {
-      bool eo = (op == BOP_DWHAT || op == BOP_EO_MRES);
+      const bool even = (op == BOP_DWHAT || op == BOP_EO_MRES || op == BOP_DWEE || op == BOP_DWOE);
+      const bool odd = (op == BOP_DWOO || op == BOP_DWEO);
       float2 *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
       for (int i = threadIdx.x; i < a.bvol; i += SWG)
-         if (a.bnd[imb[i]] && (!eo || i < a.bh)) zero_spinor(In, i);
+         if (a.bnd[imb[i]] && (!even || i < a.bh) && (!odd || i >= a.bh)) zero_spinor(In, i);
       __syncthreads();
    }
    if (op == BOP_DW)
@@ -1132,6 +1134,56 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
    {
       // Dwhat_blk keeps the odd half of the output: use field 2 only if l != 2
       blk_dwhat(a, imb, Sk, Sl, mu);
+   }
+   else if (op == BOP_DWEE || op == BOP_DWOO)
+   {
+      // (A + i mu gamma5) on one parity (Dw.c:1584-1694); the fields may coincide
+      const bool ee = op == BOP_DWEE;
+      const float m = ee ? mu : a.muo;
+      for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += SWG)
+      {
+         const int gx = imb[i];
+         float2 v[12], o[12];
+         if (a.masked && a.bnd[gx])
+            zero12(o);
+         else
+         {
+            load_spinor((const float2 *)Sk, i, v);
+            pauli_site(a.sw, gx, m, v, o);
+         }
+         store_spinor(Sl, i, o);
+      }
+   }
+   else if (op == BOP_DWOE || op == BOP_DWEO)
+   {
+      // hopping terms between the parities with zero exterior (Dw.c:1696-1826):
+      // Dwoe_blk assigns D_oe s to the odd points, Dweo_blk subtracts D_eo s
+      // from the even points; the output parity is never read
+      const bool oe = op == BOP_DWOE;
+      for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += SWG)
+      {
+         const int gx = imb[i];
+         float2 acc[12], o[12];
+         zero12(acc);
+         if (a.masked && a.bnd[gx])
+            zero12(o);
+         else
+         {
+            blk_hop(a, imb, (const float2 *)Sk, i, gx, acc);
+            if (oe)
+            {
+#pragma unroll
+               for (int c = 0; c < 12; c++) o[c] = cscale(acc[c], -0.5f);
+            }
+            else
+            {
+               load_spinor((const float2 *)Sl, i, o);
+#pragma unroll
+               for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+            }
+         }
+         store_spinor(Sl, i, o);
+      }
    }
    else if (op == BOP_MRES)
       blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
@@ -2040,6 +2092,30 @@ extern "C" void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l)
       return;
    }
    blk_op(n, BOP_DWHAT, k, l, mu, 0, "Dwhat_blk [Dw.c]");
+}
+
+extern "C" void Dwee_blk(blk_grid_t grid, int n, float mu, int k, int l)
+{
+   (void)grid;
+   blk_op(n, BOP_DWEE, k, l, mu, 0, "Dwee_blk [Dw.c]");
+}
+
+extern "C" void Dwoo_blk(blk_grid_t grid, int n, float mu, int k, int l)
+{
+   (void)grid;
+   blk_op(n, BOP_DWOO, k, l, mu, 0, "Dwoo_blk [Dw.c]");
+}
+
+extern "C" void Dwoe_blk(blk_grid_t grid, int n, int k, int l)
+{
+   (void)grid;
+   blk_op(n, BOP_DWOE, k, l, 0.0f, 0, "Dwoe_blk [Dw.c]");
+}
+
+extern "C" void Dweo_blk(blk_grid_t grid, int n, int k, int l)
+{
+   (void)grid;
+   blk_op(n, BOP_DWEO, k, l, 0.0f, 0, "Dweo_blk [Dw.c]");
 }
 
 extern "C" void blk_mres(int n, float mu, int nmr)

@@ -280,6 +280,39 @@ static void blk_Dweo(oc_ctx *c, const oc_block *b, float *s, float *r)
    free(bp);
 }
 
+static void blk_Dwee(oc_ctx *c, const oc_block *b, float mu, float *s, float *r)
+{
+   for (int x = 0; x < b->vol / 2; x++)
+   {
+      if (blk_bnd_pt(c, b, x)) { ZS(s, x); ZS(r, x); continue; }
+      float out[24];
+      blk_pauli(c, b, x, mu, s + 24 * (size_t)x, out);
+      memcpy(r + 24 * (size_t)x, out, sizeof(out));
+   }
+}
+
+/* Dwee_blk, Dwoo_blk, Dwoe_blk, Dweo_blk (dirac/Dw.c:1584-1826) on copies
+   of the block fields b.s[k] = s and b.s[l] = r (vol points each, block
+   order); s and r are written back, with the zeros the operators put on the
+   boundary points of open bc.  op: 0 Dwee, 1 Dwoo, 2 Dwoe, 3 Dweo. */
+void oc_blk_eo_op(oc_ctx *c, int n, int op, float mu, float *s, float *r)
+{
+   oc_block *b = c->blk + n;
+   size_t nb = sizeof(float) * 24 * (size_t)b->vol;
+   float *sc = calloc((size_t)24 * (b->vol + 1), sizeof(float));
+   float *rc = calloc((size_t)24 * (b->vol + 1), sizeof(float));
+   memcpy(sc, s, nb);
+   memcpy(rc, r, nb);
+   if (op == 0) blk_Dwee(c, b, mu, sc, rc);
+   else if (op == 1) blk_Dwoo(c, b, mu, sc, rc);
+   else if (op == 2) blk_Dwoe(c, b, sc, rc);
+   else blk_Dweo(c, b, sc, rc);
+   memcpy(s, sc, nb);
+   memcpy(r, rc, nb);
+   free(sc);
+   free(rc);
+}
+
 static void mres(oc_ctx *c, int n, float mu, int nmr, int eo, float *s0, float *s1)
 {
    int vol = c->blk[n].vol, nv = eo ? vol / 2 : vol;

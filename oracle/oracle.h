@@ -119,6 +119,7 @@ void oc_sap_block_imb(oc_ctx *c, int n, int *imb);   /* block index -> global in
 int oc_sap_prepare(oc_ctx *c);                       /* FP32 links/sw on block grid */
 void oc_Dw_blk(oc_ctx *c, int n, float mu, const float *s, float *r); /* block-local fields, vol+1 */
 void oc_Dwhat_blk(oc_ctx *c, int n, float mu, const float *s, float *r);
+void oc_blk_eo_op(oc_ctx *c, int n, int op, float mu, float *s, float *r); /* 0 Dwee 1 Dwoo 2 Dwoe 3 Dweo _blk */
 void oc_blk_mres(oc_ctx *c, int n, float mu, int nmr, float *s0, float *s1);
 void oc_sap(oc_ctx *c, float mu, int isolv, int nmr, float *psi, float *rho);
 double oc_sap_gcr(oc_ctx *c, int nkv, int nmx, int istop, double res, double mu,

@@ -56,6 +56,7 @@ def lib():
             "oc_sap_nblocks": (i, [P]), "oc_sap_block_info": (None, [P, i, P, P]),
             "oc_sap_block_imb": (None, [P, i, P]), "oc_sap_prepare": (i, [P]),
             "oc_Dw_blk": (None, [P, i, f, P, P]), "oc_Dwhat_blk": (None, [P, i, f, P, P]),
+            "oc_blk_eo_op": (None, [P, i, i, f, P, P]),
             "oc_blk_mres": (None, [P, i, f, i, P, P]),
             "oc_sap": (None, [P, f, i, i, P, P]),
             "oc_sap_gcr": (d, [P, i, i, i, d, d, P, P, P, P]),
@@ -251,6 +252,15 @@ class Oracle:
         r = np.zeros_like(s)
         lib().oc_Dw_blk(self.h, n, mu, ptr(s), ptr(r))
         return r
+
+    def blk_eo_op(self, n, op, mu, s, r):
+        """Dwee_blk (op 0), Dwoo_blk (1), Dwoe_blk (2), Dweo_blk (3) on
+        block fields s (input, b.s[k]) and r (output, b.s[l]); returns the
+        new (s, r)."""
+        s = np.array(s, np.float32)
+        r = np.array(r, np.float32)
+        lib().oc_blk_eo_op(self.h, n, op, mu, ptr(s), ptr(r))
+        return s, r
 
     def blk_mres(self, n, mu, nmr, s1):
         s0 = np.zeros_like(s1)

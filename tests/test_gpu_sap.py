@@ -35,6 +35,10 @@ def _sig(lib):
     lib.Dw_blk.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int]
     lib.blk_mres.argtypes = [C.c_int, C.c_float, C.c_int]
     lib.oqh_export_sblk.argtypes = [C.c_int, C.c_int, P]
+    for nm in ("Dwee_blk", "Dwoo_blk"):
+        getattr(lib, nm).argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int]
+    for nm in ("Dwoe_blk", "Dweo_blk"):
+        getattr(lib, nm).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.oqh_get_imb.argtypes = [C.c_int, P]
 
 
@@ -85,6 +89,49 @@ def test_block_operator_and_mres(setup):
         print("block", n, "Dw", rel, "mres", rel0, rel1)
         assert rel0 < 1e-4 and rel1 < 1e-3
     lat.free(s)
+
+
+@pytest.mark.gpu
+def test_block_eo_operators(setup):
+    """Dwee_blk, Dwoo_blk, Dwoe_blk, Dweo_blk (Dw.c:1584-1826) against the
+    oracle: output field and the input's boundary zeros, on the first and the
+    last block; with isolv=1 the odd block SW term is inverted, as SAP
+    leaves it.  Dwee_blk with k == l equals the out-of-place result."""
+    c, o, eta, s32, lat = setup
+    assert o.sap_prepare() == 0
+    rng = np.random.default_rng(7)
+    other = rng.standard_normal(s32.shape).astype(np.float32)
+    s, t = lat.alloc_s(), lat.alloc_s()
+    lat.put_s(s, s32)
+    lat.put_s(t, other)
+    mu = np.float32(c["mu"])
+
+    def block(n, k):
+        a = np.empty((256, 24), np.float32)
+        lat.lib.oqh_export_sblk(n, k, a.ctypes.data_as(P))
+        return a
+
+    calls = [lambda n: lat.lib.Dwee_blk(0, n, c["mu"], 1, 2), lambda n: lat.lib.Dwoo_blk(0, n, c["mu"], 1, 2),
+             lambda n: lat.lib.Dwoe_blk(0, n, 1, 2), lambda n: lat.lib.Dweo_blk(0, n, 1, 2)]
+    for n in (0, o.nblocks() - 1):
+        imb = o.block_imb(n)
+        for op, call in enumerate(calls):
+            lat.lib.assign_s2sblk(0, n, 0, s, 1)
+            lat.lib.assign_s2sblk(0, n, 0, t, 2)
+            call(n)
+            s_ref, r_ref = o.blk_eo_op(n, op, mu, s32[imb], other[imb])
+            rel, absz = site_rel(block(n, 2), r_ref)
+            rels, abss = site_rel(block(n, 1), s_ref)
+            print("block", n, "op", op, rel, rels)
+            assert rel < 1e-5 and absz == 0.0 and rels == 0.0 and abss == 0.0
+            if op == 0:
+                out = block(n, 2)
+                lat.lib.assign_s2sblk(0, n, 0, s, 1)
+                lat.lib.Dwee_blk(0, n, c["mu"], 1, 1)
+                inpl = block(n, 1)
+                assert np.array_equal(inpl[:128], out[:128]) and np.array_equal(inpl[128:], s_ref[128:])
+    lat.free(s)
+    lat.free(t)
 
 
 @pytest.mark.gpu
