@@ -225,6 +225,18 @@ void Dwhat_dble(double mu, spinor_dble *s, spinor_dble *r);
 /* include/sw_term.h:77-82 (sw_term/sw_term.c:399, swflds.c:129) */
 int sw_term(ptset_t set);
 void assign_swd2sw(void);
+/* include/sw_term.h:75-76 (sw_term/swflds.c:79-126): host copies of the
+   device SW fields (2V matrices, the reference's layout) taken at the call:
+   the current term (with the parts sw_term() inverted), or unit matrices
+   before any SW term; writes to them do not reach the device */
+pauli *swfld(void);
+pauli_dble *swdfld(void);
+/* include/sw_term.h:43,52 (sw_term/pauli.c:1107-1119,
+   pauli_dble.c:1840-1857): r = (A + i mu gamma5) s on vol points with the
+   matrices m[2k], m[2k+1] (any host array of that layout); s and r are
+   device fields or host arrays and may coincide */
+void apply_sw(int vol, float mu, pauli *m, spinor *s, spinor *r);
+void apply_sw_dble(int vol, double mu, pauli_dble *m, spinor_dble *s, spinor_dble *r);
 /* include/uflds.h:59 (uflds/uflds.c:479) */
 void assign_ud2u(void);
 

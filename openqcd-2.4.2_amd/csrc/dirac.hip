@@ -1240,3 +1240,67 @@ extern "C" double oqh_time_dirac(int op, double mu, void *s, void *r, int ncall)
    HIP_OK(hipEventDestroy(e1));
    return (double)ms / ncall;
 }
+
+// ---------------------------------------------------------------------------
+// apply_sw_dble, apply_sw (sw_term/pauli_dble.c:1840-1857, pauli.c:1107-1119):
+// r = (A + i mu gamma5) s on vol points with the matrices m[2k], m[2k+1] of
+// point k, for any array m of the reference's layout (for instance a copy
+// from swdfld()/swfld() at an offset).  m is moved to the device in the
+// site-block layout; s and r are device fields or host arrays and may
+// coincide.
+// ---------------------------------------------------------------------------
+
+namespace oqh {
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_apply_sw(const C *m, const C *s, C *r, int soff, int roff, int vol,
+                                                  decltype(C::x) mu)
+{
+   const int k = blockIdx.x * 256 + threadIdx.x;
+   if (k >= vol) return;
+   C v[12], o[12];
+   load_spinor(s, soff + k, v);
+   pauli_site<C, false>(m, k, mu, v, o);
+   store_spinor(r, roff + k, o);
+}
+
+template <typename C>
+static void apply_sw_any(int vol, double mu, const void *m, void *s, void *r, int prec, const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where) || vol <= 0) return;
+   if (m == nullptr)
+   {
+      fail(where, "Attempt to access unallocated memory space");
+      return;
+   }
+   const size_t eb = sizeof(C);
+   const size_t aos_bytes = (size_t)vol * 36 * eb, blk_bytes = units_for(vol, 36) * eb;
+   char *buf = (char *)tmp_buffer(aos_bytes + blk_bytes);
+   if (!buf) return;
+   C *aos = (C *)buf, *blk = (C *)(buf + aos_bytes);
+   HIP_OK(hipMemcpyAsync(aos, m, aos_bytes, hipMemcpyHostToDevice, L.stream));
+   if (prec == F64)
+      aos_to_blocks_f64((const double2 *)aos, (double2 *)blk, vol, 36, L.stream);
+   else
+      aos_to_blocks_f32((const float2 *)aos, (float2 *)blk, vol, 36, L.stream);
+   Res rs = resolve(s, prec, vol, 2, true);
+   Res rr = resolve(r, prec, vol, 3, false);
+   if (!rs.dev || !rr.dev) return;
+   k_apply_sw<C><<<(unsigned)((vol + 255) / 256), 256, 0, L.stream>>>(blk, (const C *)rs.dev, (C *)rr.dev, rs.off,
+                                                                        rr.off, vol, (decltype(C::x))mu);
+   finish(rr, prec, vol, true);
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+}  // namespace oqh
+
+extern "C" void apply_sw_dble(int vol, double mu, pauli_dble *m, spinor_dble *s, spinor_dble *r)
+{
+   apply_sw_any<double2>(vol, mu, m, s, r, F64, "apply_sw_dble [pauli_dble.c]");
+}
+
+extern "C" void apply_sw(int vol, float mu, pauli *m, spinor *s, spinor *r)
+{
+   apply_sw_any<float2>(vol, mu, m, s, r, F32, "apply_sw [pauli.c]");
+}

@@ -480,3 +480,46 @@ extern "C" void assign_ud2u(void)
    L.u_valid = false;
    ensure_u();
 }
+
+// swflds.c:79-146: the SW fields.  The fields live on the device; swdfld()
+// and swfld() return host copies (2V matrices in the reference's layout)
+// taken at the time of the call: the current SW term as last computed
+// (uninverted or with the inverted parts sw_term() left), or unit matrices
+// when no SW term has been computed.  Changes made through the returned
+// pointers do not reach the device (oqh_import_swd does).
+static std::vector<pauli_dble> h_swd;
+static std::vector<pauli> h_sw;
+
+extern "C" pauli_dble *swdfld(void)
+{
+   Lib &L = lib();
+   if (!require_geometry("swdfld [swflds.c]")) return nullptr;
+   h_swd.assign(2 * (size_t)L.g.V, pauli_dble{});
+   if (L.swd_valid)
+      oqh_export_swd(h_swd.data());
+   else
+      for (auto &m : h_swd)
+         for (int k = 0; k < 6; k++) m.u[k] = 1.0;
+   return h_swd.data();
+}
+
+extern "C" pauli *swfld(void)
+{
+   Lib &L = lib();
+   if (!require_geometry("swfld [swflds.c]")) return nullptr;
+   h_sw.assign(2 * (size_t)L.g.V, pauli{});
+   if (L.swd_valid)
+   {
+      ensure_sw();
+      const size_t n = (size_t)L.g.V;
+      float2 *aos = (float2 *)tmp_buffer(n * sizeof(pauli) * 2);
+      if (!aos) return nullptr;
+      blocks_to_aos_f32(L.d_sw, aos, L.g.V, 36, L.stream);
+      HIP_OK(hipMemcpyAsync(h_sw.data(), aos, n * 2 * sizeof(pauli), hipMemcpyDeviceToHost, L.stream));
+      HIP_OK(hipStreamSynchronize(L.stream));
+   }
+   else
+      for (auto &m : h_sw)
+         for (int k = 0; k < 6; k++) m.u[k] = 1.0f;
+   return h_sw.data();
+}

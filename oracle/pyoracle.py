@@ -195,6 +195,11 @@ class Oracle:
     def assign_swd2sw(self):
         lib().oc_assign_swd2sw(self.h)
 
+    def sw(self):
+        """The single-precision SW field (after assign_swd2sw)."""
+        p = lib().oc_swfld(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), (2 * self.V, 36)).copy()
+
     # operators (inputs may be modified, as in the reference)
     def _op_d(self, name, mu, s, r=None):
         if r is None:
@@ -304,6 +309,16 @@ def mul_pauli_dble(mu, m, s):
     s = np.ascontiguousarray(s, np.float64)
     r = np.empty(12, np.float64)
     lib().oc_mul_pauli_dble(mu, ptr(m), ptr(s), ptr(r))
+    return r
+
+
+def apply_sw_dble(mu, m, s):
+    """r = (A + i mu gamma5) s on len(s) points with the matrices m[2k],
+    m[2k+1] (sw_term/pauli_dble.c:1840-1857)."""
+    m = np.ascontiguousarray(m, np.float64)
+    s = np.ascontiguousarray(s, np.float64)
+    r = np.empty_like(s)
+    lib().oc_apply_sw_dble(s.shape[0], mu, ptr(m), ptr(s), ptr(r))
     return r
 
 
