@@ -184,6 +184,15 @@ def solve_times(args):
         lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
         e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
         lib.random_sd(V, e, 1.0)
+        # the operators on this lattice (HIP events, 50 calls each)
+        assert lat.sw_term(3) == 0
+        ms = lib.oqh_time_dirac(0, 0.0, e, r, 50)
+        out["Dw_dble_GBps"] = round(BYTES["Dw_dble"] * V / ms / 1e6, 1)
+        out["Dw_dble_pct_of_8TBps"] = round(100 * BYTES["Dw_dble"] * V / ms / 1e6 / PEAK_HBM, 1)
+        assert lat.sw_term(2) == 0
+        ms = lib.oqh_time_dirac(2, 0.0, e, r, 50)
+        out["Dwhat_dble_GBps"] = round(BYTES["Dwhat_dble"] * V / ms / 1e6, 1)
+        log("bench: 48^4 Dw_dble %.1f GB/s, Dwhat_dble %.1f GB/s" % (out["Dw_dble_GBps"], out["Dwhat_dble_GBps"]))
 
         def nsq(f):
             q = lib.norm_square_dble(V, 0, f)
