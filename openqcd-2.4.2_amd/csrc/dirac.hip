@@ -281,9 +281,6 @@ template <> struct TileSize<float> { static constexpr int n = OQH_DW_TPF; };
 #define OQH_X_DOUBLE 0
 #endif
 constexpr int XBUF = OQH_X_DOUBLE ? 2 : 1;
-#ifndef OQH_SW_FIRST
-#define OQH_SW_FIRST 0
-#endif
 
 // neighbour-table entry k (0..7 for the odd point, 8..15 for the even point)
 template <typename T> struct TileNbr
@@ -459,23 +456,6 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
 #pragma unroll
       for (int c = 0; c < 12; c++) S[c][t] = se[c];
       if (!ob) load_spinor(a.s, o, so);
-#if OQH_SW_FIRST
-      // diagonal terms first, scaled by 1/coef (a power of 2, so that
-      // coef * (A s / coef + hops) rounds as A s + coef * hops)
-      T ic = (T)1 / a.coef;
-      if (!ob)
-      {
-         pauli_site<C, OQH_NT_SW>(a.sw, o, a.muo, so, acc_o);
-#pragma unroll
-         for (int c = 0; c < 12; c++) acc_o[c] = cscale(acc_o[c], ic);
-      }
-      if (!eb)
-      {
-         pauli_site<C, OQH_NT_SW>(a.sw, e, a.mu, se, acc_e);
-#pragma unroll
-         for (int c = 0; c < 12; c++) acc_e[c] = cscale(acc_e[c], ic);
-      }
-#endif
    }
    __syncthreads();
    tile_phase<T, 0, true, TP, true, true, REC>(a, t, e0, valid, nb, S, X[0], so, acc_o, acc_e, a.s);
@@ -496,14 +476,9 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
    }
    else
    {
-#if OQH_SW_FIRST
-#pragma unroll
-      for (int c = 0; c < 12; c++) out[c] = cscale(acc_o[c], a.coef);
-#else
       site_term<T, SWS>(a, o, a.muo, so, out);
 #pragma unroll
       for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_o[c], a.coef));
-#endif
       store_spinor(a.r, o, out);
    }
    if (eb)
@@ -513,17 +488,12 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
    }
    else
    {
-#if OQH_SW_FIRST
-#pragma unroll
-      for (int c = 0; c < 12; c++) out[c] = cscale(acc_e[c], a.coef);
-#else
       C se[12];
 #pragma unroll
       for (int c = 0; c < 12; c++) se[c] = S[c][t];
       site_term<T, SWS>(a, e, a.mu, se, out);
 #pragma unroll
       for (int c = 0; c < 12; c++) out[c] = cadd(out[c], cscale(acc_e[c], a.coef));
-#endif
       store_spinor(a.r, e, out);
    }
 }
