@@ -164,6 +164,10 @@ void oqh_nn_free(void);
    library's phase flag of set_ud_phase() cleared. */
 void oqh_import_ud(const su3_dble *ud);
 void oqh_export_ud(su3_dble *ud);
+/* no counterpart: the boundary field l of face ifc of block n after Dw_bnd
+   (bb[ifc].w[l] of the reference, block/block.c; vol/bs[ifc/2] Weyl
+   spinors in the point order of lattice/blk_geometry.c:241-313) */
+void oqh_export_bnd(int n, int ifc, int l, weyl *w);
 /* SW field in the layout of swdfld() (2 pauli_dble per point) */
 void oqh_export_swd(pauli_dble *swd);
 void oqh_import_swd(const pauli_dble *swd);
@@ -349,6 +353,11 @@ void Dwoo_blk(blk_grid_t grid, int n, float mu, int k, int l);
 void Dwoe_blk(blk_grid_t grid, int n, int k, int l);
 void Dweo_blk(blk_grid_t grid, int n, int k, int l);
 void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l);
+/* include/dirac.h:25 (dirac/Dw_bnd.c:801-1011): the Weyl fields of the
+   hopping terms from block n to its exterior, face by face, into the
+   block's boundary field l (SAP_BLOCKS has one, l = 0); read them with
+   oqh_export_bnd */
+void Dw_bnd(blk_grid_t grid, int n, int k, int l);
 
 /* include/sap.h:21-35 (sap/blk_solv.c:1117-1186, sap/sap.c:720,
    sap/sap_gcr.c:140) */

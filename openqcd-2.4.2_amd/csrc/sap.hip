@@ -72,6 +72,9 @@ struct SapGrid
    int swb_version = 0;                  // incremented by every assign_swb
    std::vector<int> imb;                 // host copy
    std::vector<int> bo;                  // [nb][4]
+   int *d_ipp = nullptr;                 // face points of a block, faces 0..7 in turn
+   int fofs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // offsets of the faces in d_ipp
+   float2 *d_bw = nullptr;               // Weyl fields of Dw_bnd, [nb][fofs[8]][6]
 };
 
 static SapGrid G;
@@ -1191,6 +1194,89 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
       blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
 }
 
+// Dw_bnd (dirac/Dw_bnd.c:801-1011): for the face points x of block n the
+// Weyl field w = -1/2 U chi of the hopping term into the exterior neighbour
+// y, chi the upper components of (1 -+ gamma_mu) s(x): faces -mu (y = x - mu)
+// use U(y,mu), faces +mu (y = x + mu) U(x,mu)^+.  Faces on the open time
+// boundary give zero and zero s at their points.  One work-group per call.
+struct FaceOfs
+{
+   int o[9];
+};
+
+__global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int *ipp, FaceOfs fo, float2 *W)
+{
+   const int *imb = a.imb + (size_t)n * a.bvol;
+   float2 *S = a.scr + ((size_t)n * 3 + k) * a.Ub;
+   float2 *Wn = W + (size_t)n * fo.o[8] * 6;
+   for (int ifc = 0; ifc < 8; ifc++)
+   {
+      const int mu = ifc >> 1;
+      const bool up = ifc & 1;
+      for (int j = fo.o[ifc] + threadIdx.x; j < fo.o[ifc + 1]; j += SWG)
+      {
+         const int x = ipp[j], gx = imb[x];
+         float2 w[6];
+         if (mu == 0 && a.masked && a.bnd[gx])
+         {
+#pragma unroll
+            for (int c = 0; c < 6; c++) w[c] = make_float2(0.0f, 0.0f);
+            zero_spinor(S, x);
+         }
+         else
+         {
+            float2 sp[12], h[6], u[9];
+            load_spinor((const float2 *)S, x, sp);
+            // chi (spin components 1, 2) per face, from Dw_bnd.c:825-1009
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+               const float2 s1 = sp[c], s2 = sp[3 + c], s3 = sp[6 + c], s4 = sp[9 + c];
+               float2 a1, a2;
+               if (mu == 0)
+               {
+                  a1 = up ? csub(s1, s3) : cadd(s1, s3);
+                  a2 = up ? csub(s2, s4) : cadd(s2, s4);
+               }
+               else if (mu == 1)
+               {
+                  a1 = up ? csubi(s1, s4) : caddi(s1, s4);
+                  a2 = up ? csubi(s2, s3) : caddi(s2, s3);
+               }
+               else if (mu == 2)
+               {
+                  a1 = up ? csub(s1, s4) : cadd(s1, s4);
+                  a2 = up ? cadd(s2, s3) : csub(s2, s3);
+               }
+               else
+               {
+                  a1 = up ? csubi(s1, s3) : caddi(s1, s3);
+                  a2 = up ? caddi(s2, s4) : csubi(s2, s4);
+               }
+               h[c] = cscale(a1, -0.5f);
+               h[3 + c] = cscale(a2, -0.5f);
+            }
+            const int gy = a.nbr[(size_t)(up ? 2 * mu : 2 * mu + 1) * a.V + gx] & 0x7fffffff;
+            const float2 *lp = blk_link(a, gx, gy, mu, up);
+#pragma unroll
+            for (int q = 0; q < 9; q++) u[q] = lp[q * SB];
+            if (up)
+            {
+               su3_mul<float2, true>(u, h, w);
+               su3_mul<float2, true>(u, h + 3, w + 3);
+            }
+            else
+            {
+               su3_mul<float2, false>(u, h, w);
+               su3_mul<float2, false>(u, h + 3, w + 3);
+            }
+         }
+#pragma unroll
+         for (int c = 0; c < 6; c++) Wn[(size_t)j * 6 + c] = w[c];
+      }
+   }
+}
+
 __global__ void k_blk_copy(float2 *glob, float2 *blk, const int *imb, int i0, int i1, int to_blk)
 {
    int i = i0 + blockIdx.x * 256 + threadIdx.x;
@@ -1226,6 +1312,7 @@ static void free_grid()
    dev_free(G.d_u16); dev_free(G.d_swb16);
    for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); dev_free(G.d_h64[k]); }
    dev_free(G.d_swb64); dev_free(G.d_scr64); dev_free(G.d_s0gm);
+   dev_free(G.d_ipp); dev_free(G.d_bw);
    G = SapGrid();
 }
 
@@ -1331,6 +1418,30 @@ static bool build_grid(const char *where)
          fmask[colour_of_site[ix]].push_back(fm[ix]);
       }
    G.d_imb = (int *)dev_alloc(sizeof(int) * G.imb.size());
+   // face points of the block boundaries in the order of bnd_geometry
+   // (lattice/blk_geometry.c:241-313): odd points first, then even, each
+   // point entered on the faces -mu/+mu whose neighbour lies outside
+   {
+      std::vector<int> fl[8];
+      for (int ix = 0; ix < G.bvol; ix++)
+      {
+         const int iy = ix < G.bh ? ix + G.bh : ix - G.bh;
+         for (int mu = 0; mu < 4; mu++)
+         {
+            if (bnb[(size_t)(2 * mu) * G.bvol + iy] == G.bvol) fl[2 * mu + 1].push_back(iy);
+            if (bnb[(size_t)(2 * mu + 1) * G.bvol + iy] == G.bvol) fl[2 * mu].push_back(iy);
+         }
+      }
+      std::vector<int> ipp;
+      for (int ifc = 0; ifc < 8; ifc++)
+      {
+         G.fofs[ifc] = (int)ipp.size();
+         ipp.insert(ipp.end(), fl[ifc].begin(), fl[ifc].end());
+      }
+      G.fofs[8] = (int)ipp.size();
+      G.d_ipp = (int *)dev_alloc(sizeof(int) * ipp.size());
+      HIP_OK(hipMemcpy(G.d_ipp, ipp.data(), sizeof(int) * ipp.size(), hipMemcpyHostToDevice));
+   }
    G.d_bnb = (int *)dev_alloc(sizeof(int) * bnb.size());
    HIP_OK(hipMemcpy(G.d_imb, G.imb.data(), sizeof(int) * G.imb.size(), hipMemcpyHostToDevice));
    HIP_OK(hipMemcpy(G.d_bnb, bnb.data(), sizeof(int) * bnb.size(), hipMemcpyHostToDevice));
@@ -2116,6 +2227,52 @@ extern "C" void Dweo_blk(blk_grid_t grid, int n, int k, int l)
 {
    (void)grid;
    blk_op(n, BOP_DWEO, k, l, 0.0f, 0, "Dweo_blk [Dw.c]");
+}
+
+extern "C" void Dw_bnd(blk_grid_t grid, int n, int k, int l)
+{
+   Lib &L = lib();
+   const char *where = "Dw_bnd [Dw_bnd.c]";
+   (void)grid;
+   if (!blk_check(n, k, where)) return;
+   if (l != 0)
+   {
+      fail(where, "Parameters are out of range");
+      return;
+   }
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return;
+   }
+   ensure_u();
+   if (!G.d_bw)
+   {
+      G.d_bw = (float2 *)dev_alloc(sizeof(float2) * 6 * (size_t)G.nb * G.fofs[8]);
+      if (!G.d_bw) return;
+   }
+   SArgs a = sargs(0.0f, 0);
+   a.scr = G.d_bs;
+   FaceOfs fo;
+   memcpy(fo.o, G.fofs, sizeof(fo.o));
+   k_dw_bnd<<<1, SWG, 0, L.stream>>>(a, n, k, G.d_ipp, fo, G.d_bw);
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+extern "C" void oqh_export_bnd(int n, int ifc, int l, weyl *w)
+{
+   Lib &L = lib();
+   const char *where = "oqh_export_bnd";
+   if (!blk_check(n, 0, where)) return;
+   if (ifc < 0 || ifc > 7 || l != 0 || !G.d_bw)
+   {
+      fail(where, "no such boundary field (Dw_bnd has not been called or parameters out of range)");
+      return;
+   }
+   const size_t off = ((size_t)n * G.fofs[8] + G.fofs[ifc]) * 6;
+   const size_t cnt = (size_t)(G.fofs[ifc + 1] - G.fofs[ifc]) * 6;
+   HIP_OK(hipMemcpyAsync(w, G.d_bw + off, cnt * sizeof(float2), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
 }
 
 extern "C" void blk_mres(int n, float mu, int nmr)

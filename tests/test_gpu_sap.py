@@ -135,6 +135,60 @@ def test_block_eo_operators(setup):
 
 
 @pytest.mark.gpu
+def test_dw_bnd(setup):
+    """Dw_bnd (Dw_bnd.c:801-1011) against the oracle's global Dw: the Weyl
+    field of face ifc, expanded to the Dirac spinor psi with
+    (1 + gamma_mu) psi = 0 on the face -mu and (1 - gamma_mu) psi = 0 on
+    +mu, is the value at the exterior neighbour of Dw applied to a field
+    that equals the block's field on the block and vanishes elsewhere."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    lib.Dw_bnd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.oqh_export_bnd.argtypes = [C.c_int, C.c_int, C.c_int, P]
+    assert o.sap_prepare() == 0
+    o.assign_ud2u()
+    o.assign_swd2sw()
+    iup, idn = o.iupdn()
+    gam = np.zeros((4, 4, 4), complex)
+    gam[0] = [[0, 0, -1, 0], [0, 0, 0, -1], [-1, 0, 0, 0], [0, -1, 0, 0]]
+    gam[1] = [[0, 0, 0, -1j], [0, 0, -1j, 0], [0, 1j, 0, 0], [1j, 0, 0, 0]]
+    gam[2] = [[0, 0, 0, -1], [0, 0, 1, 0], [0, 1, 0, 0], [-1, 0, 0, 0]]
+    gam[3] = [[0, 0, -1j, 0], [0, 0, 0, 1j], [1j, 0, 0, 0], [0, -1j, 0, 0]]
+    s = lat.alloc_s()
+    lat.put_s(s, s32)
+    nb = o.nblocks()
+    for n in (0, nb // 2 + 1, nb - 1):
+        imb = o.block_imb(n)
+        bvol, bh = len(imb), len(imb) // 2
+        inblk = set(imb.tolist())
+        lat.lib.assign_s2sblk(0, n, 0, s, 1)
+        lib.Dw_bnd(0, n, 1, 0)
+        F = np.zeros_like(s32)
+        F[imb] = s32[imb]
+        D = o.Dw(np.float32(0.0), F.copy()).astype(np.float64)
+        D = (D[:, 0::2] + 1j * D[:, 1::2]).reshape(-1, 4, 3)
+        order = list(range(bh, bvol)) + list(range(bh))
+        for ifc in range(8):
+            mu, up = ifc // 2, ifc & 1
+            nbr = iup if up else idn
+            pts = [iy for iy in order if nbr[imb[iy], mu] not in inblk]
+            assert len(pts) == bvol // 4
+            w = np.empty((len(pts), 12), np.float32)
+            lib.oqh_export_bnd(n, ifc, 0, w.ctypes.data_as(P))
+            u = (w[:, 0::2] + 1j * w[:, 1::2]).astype(complex).reshape(-1, 2, 3)
+            e = gam[mu][:2, 2:]
+            low = np.einsum("ij,njc->nic", np.conj(e.T), u) * (1.0 if up else -1.0)
+            psi = np.concatenate([u, low], axis=1)
+            ref = D[[nbr[imb[iy], mu] for iy in pts]]
+            err = np.abs(psi - ref).max(axis=(1, 2))
+            scale = np.abs(ref).max(axis=(1, 2))
+            print("block", n, "face", ifc, float((err / np.maximum(scale, 1e-30)).max()))
+            assert (err <= 1e-5 * np.maximum(scale, 1.0)).all()
+            assert np.array_equal(err[scale == 0], np.zeros((scale == 0).sum()))
+    lat.free(s)
+
+
+@pytest.mark.gpu
 def test_sap_cycles(setup):
     c, o, eta, s32, lat = setup
     assert o.sap_prepare() == 0
