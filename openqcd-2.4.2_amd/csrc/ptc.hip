@@ -87,14 +87,32 @@ __device__ inline void ptc_transport(const float2 *in, int y, int i, int p, cons
       su3_mul<float2, false>(um, v, w);
 }
 
+// activations: FP32 complex numbers (training, the network's input and
+// output), or BF16 pairs between the layers of the preconditioner
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline float2 ld_act(const float2 *p, size_t i) { return p[i]; }
+__device__ inline float2 ld_act(const bf16x2 *p, size_t i)
+{
+   const bf16x2 b = p[i];
+   return make_float2((float)b.x, (float)b.y);
+}
+__device__ inline void st_act(float2 *p, size_t i, float2 v) { p[i] = v; }
+__device__ inline void st_act(bf16x2 *p, size_t i, float2 v)
+{
+   bf16x2 b;
+   b.x = (__bf16)v.x;
+   b.y = (__bf16)v.y;
+   p[i] = b;
+}
+
 // out (+)= layer(in); wp: packed BF16 weights [9][32][kpad(CI)].  Two
 // directions per phase (five phases): the two halves of the work-group
 // transport all input channels of their point for direction 2q and 2q+1,
 // so every link is loaded once, and one barrier pair serves two directions.
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-
-template <int CI, int CO>
-__global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, float2 *__restrict__ out,
+// accum (out += ...) only with FP32 output.
+template <int CI, int CO, typename TI = float2, typename TO = float2>
+__global__ __launch_bounds__(PWG) void k_ptc(const TI *__restrict__ in, TO *__restrict__ out,
                                              const __bf16 *__restrict__ wp, const float2 *__restrict__ u,
                                              const int *__restrict__ nbr, int V, int accum)
 {
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, floa
 #pragma unroll
          for (int i = 0; i < CI; i++)
 #pragma unroll
-            for (int a = 0; a < 3; a++) v[i][a] = in[unit_index(y, 3 * i + a, 3 * CI)];
+            for (int a = 0; a < 3; a++) v[i][a] = ld_act(in, unit_index(y, 3 * i + a, 3 * CI));
 #pragma unroll
          for (int i = 0; i < CI; i++)
          {
@@ -206,11 +224,11 @@ __global__ __launch_bounds__(PWG) void k_ptc(const float2 *__restrict__ in, floa
             const size_t id = unit_index(xo, 3 * o + a, 3 * CO);
             if (accum)
             {
-               float2 w = out[id];
+               float2 w = ld_act(out, id);
                v.x += w.x;
                v.y += w.y;
             }
-            out[id] = v;
+            st_act(out, id, v);
          }
       }
    }
@@ -364,13 +382,6 @@ __global__ void k_ptc_lgrad(const float2 *out, const float2 *e, float c, int n, 
    if (k < n) g[k] = make_float2(c * (out[k].x - e[k].x), c * (out[k].y - e[k].y));
 }
 
-// phi += t
-__global__ void k_ptc_add(const float2 *t, float2 *phi, size_t n)
-{
-   const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-   if (k < n) phi[k] = make_float2(phi[k].x + t[k].x, phi[k].y + t[k].y);
-}
-
 // Adam step on n reals
 __global__ void k_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps,
                        float c1, float c2)
@@ -405,6 +416,7 @@ struct Net
    float2 *d_part = nullptr;        // weight-gradient partials
    double *d_lpart = nullptr, *h_lpart = nullptr;
    float2 *d_tmp = nullptr;         // spinor work field of the preconditioner
+   bf16x2 *d_inf[2] = {};           // BF16 activations of the preconditioner (ping-pong)
 };
 
 Net N;
@@ -417,6 +429,7 @@ void free_net()
    for (int l = 0; l < NLMAX; l++) { dev_free(N.d_pf[l]); dev_free(N.d_pa[l]); }
    for (int l = 0; l <= NLMAX; l++) dev_free(N.d_act[l]);
    dev_free(N.d_gb[0]); dev_free(N.d_gb[1]); dev_free(N.d_part); dev_free(N.d_lpart); dev_free(N.d_tmp);
+   dev_free(N.d_inf[0]); dev_free(N.d_inf[1]);
    if (N.h_lpart) HIP_OK(hipHostFree(N.h_lpart));
    N = Net();
 }
@@ -442,6 +455,7 @@ bool ensure_fields()
    if (N.V == L.g.V && N.d_gb[0]) return true;
    for (int l = 0; l <= NLMAX; l++) { dev_free(N.d_act[l]); N.d_act[l] = nullptr; }
    for (int k = 0; k < 2; k++) { dev_free(N.d_gb[k]); N.d_gb[k] = nullptr; }
+   for (int k = 0; k < 2; k++) { dev_free(N.d_inf[k]); N.d_inf[k] = nullptr; }
    dev_free(N.d_part); dev_free(N.d_tmp);
    N.V = L.g.V;
    for (int l = 1; l < N.nl; l++) N.d_act[l] = (float2 *)dev_alloc(units_for(N.V, 3 * CMAX) * sizeof(float2));
@@ -449,7 +463,8 @@ bool ensure_fields()
    const int nb = (N.V + PNS - 1) / PNS;
    N.d_part = (float2 *)dev_alloc((size_t)nb * NDIR * CMAX * CMAX * sizeof(float2));
    N.d_tmp = (float2 *)dev_alloc(units_for(N.V, 12) * sizeof(float2));
-   return N.d_gb[0] && N.d_gb[1] && N.d_part && N.d_tmp;
+   for (int k = 0; k < 2; k++) N.d_inf[k] = (bf16x2 *)dev_alloc(units_for(N.V, 3 * CMAX) * sizeof(bf16x2));
+   return N.d_gb[0] && N.d_gb[1] && N.d_part && N.d_tmp && N.d_inf[0] && N.d_inf[1];
 }
 
 template <int CI, int CO>
@@ -458,6 +473,24 @@ void launch_layer(const float2 *in, float2 *out, const __bf16 *wp, int accum)
    Lib &L = lib();
    const int V = L.g.V;
    k_ptc<CI, CO><<<(V + PNS - 1) / PNS, PWG, 0, L.stream>>>(in, out, wp, L.d_u, L.d_nbr, V, accum);
+}
+
+// one layer of the preconditioner: mode 0 FP32 -> BF16, 1 BF16 -> BF16,
+// 2 BF16 -> FP32 added to out, 3 FP32 -> FP32 added to out
+template <int CI, int CO>
+void launch_inf(const void *in, void *out, const __bf16 *wp, int mode)
+{
+   Lib &L = lib();
+   const int V = L.g.V;
+   const unsigned nb = (V + PNS - 1) / PNS;
+   if (mode == 0)
+      k_ptc<CI, CO, float2, bf16x2><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (bf16x2 *)out, wp, L.d_u, L.d_nbr, V, 0);
+   else if (mode == 1)
+      k_ptc<CI, CO, bf16x2, bf16x2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (bf16x2 *)out, wp, L.d_u, L.d_nbr, V, 0);
+   else if (mode == 2)
+      k_ptc<CI, CO, bf16x2, float2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (float2 *)out, wp, L.d_u, L.d_nbr, V, 1);
+   else
+      k_ptc<CI, CO><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (float2 *)out, wp, L.d_u, L.d_nbr, V, 1);
 }
 
 template <int CI, int CO> void launch_wgrad(const float2 *in, const float2 *g)
@@ -593,14 +626,22 @@ bool fields_of(int n, spinor **s, std::vector<float2 *> &out, const char *where)
 
 bool nn_ready() { return N.nl > 0 && lib().nn_prec; }
 
-// phi += NN(r) for the preconditioner (device fields of the lattice)
+// phi += NN(r) for the preconditioner (device fields of the lattice): the
+// activations between the layers are kept in BF16, the values the channel
+// mixing rounds them to anyway, and the last layer adds its output to phi
 bool nn_correct(const float2 *r, float2 *phi)
 {
    if (!N.nl || !ensure_fields()) return false;
-   forward(r, N.d_tmp);
-   Lib &L = lib();
-   const size_t nu = units_for(L.g.V, 12);
-   k_ptc_add<<<(unsigned)((nu + 255) / 256), 256, 0, L.stream>>>(N.d_tmp, phi, nu);
+   pack();
+   const void *x = r;
+   for (int l = 0; l < N.nl; l++)
+   {
+      const bool first = l == 0, last = l == N.nl - 1;
+      const int mode = last ? (first ? 3 : 2) : (first ? 0 : 1);
+      void *y = last ? (void *)phi : (void *)N.d_inf[l & 1];
+      OQH_PTC_DISPATCH(launch_inf, N.w[l], N.w[l + 1], x, y, N.d_pf[l], mode);
+      x = y;
+   }
    return true;
 }
 
