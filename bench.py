@@ -62,7 +62,7 @@ def measured_traffic():
     return None
 
 
-NN_WIDTHS = [4, 12, 16, 12, 4]
+NN_WIDTHS = [4, 8, 4]
 
 
 def train_network():
@@ -240,7 +240,59 @@ def solve_times(args):
                                                           out[name]["seconds"], 3)
     finally:
         lat.close()
+    out["learned_held_out"] = learned_held_out(wts)
     return out
+
+
+def learned_held_out(wts):
+    """The trained network on fields it was not trained on: 32^3x64
+    periodic, random_ud of seeds 5-8 flowed 50 x 0.01, quark phase,
+    csw=1.77, m0=-0.35, a Gaussian source each; plain, FP16 and FP16 + network
+    SAP+GCR (SAP 4^4 isolv=1 nmr=4 ncy=5, nkv=24) to relative residual 1e-10."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    res = []
+    lat = pyoqh.Lattice((64, 32, 32, 32), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
+    try:
+        lib = lat.lib
+        lib.sap_gcr.restype = C.c_double
+        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        w = np.array(NN_WIDTHS, np.int32)
+        lib.oqh_nn_create(len(NN_WIDTHS) - 1, w.ctypes.data_as(P))
+        lib.oqh_nn_set_weights(wts.ctypes.data_as(P))
+        e, p = lat.alloc_sd(), lat.alloc_sd()
+        for seed in (5, 6, 7, 8):
+            lib.start_ranlux(0, seed)
+            lib.random_ud()
+            lib.fwd_euler(50, 0.01)
+            lib.set_ud_phase()
+            lib.random_sd(lat.V, e, 1.0)
+            rec = {"seed": seed}
+            for name, prec, nn in (("plain", 0, 0), ("fp16", 1, 0), ("fp16_learned", 1, 1)):
+                lib.oqh_set_sap_prec(prec)
+                lib.oqh_set_nn_prec(nn)
+                ifail, status = C.c_int(), C.c_int()
+                lat.sync()
+                t0 = time.perf_counter()
+                lib.sap_gcr(24, 1024, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+                lat.sync()
+                rec[name] = {"seconds": round(time.perf_counter() - t0, 3), "status": status.value,
+                             "ifail": ifail.value}
+            lib.oqh_set_sap_prec(0)
+            lib.oqh_set_nn_prec(0)
+            log("bench: held-out field seed %d: plain %.3f s (%d), fp16 %.3f s (%d), learned %.3f s (%d)" %
+                (seed, rec["plain"]["seconds"], rec["plain"]["status"], rec["fp16"]["seconds"],
+                 rec["fp16"]["status"], rec["fp16_learned"]["seconds"], rec["fp16_learned"]["status"]))
+            res.append(rec)
+    finally:
+        lat.close()
+    return {"workload": "32^3x64 periodic, random_ud seeds 5-8 flowed 50 x 0.01, csw=1.77, m0=-0.35, "
+                        "nkv=24, res=1e-10, SAP 4^4 isolv=1 nmr=4 ncy=5", "fields": res}
 
 
 def multi_rhs():
