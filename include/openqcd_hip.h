@@ -141,7 +141,9 @@ void ranlxd(double *r, int n);
    widths[0] = widths[nlayer] = 4; returns 0 on success.  Weights are
    complex, per layer [9][C_out][C_in], layers in order, as (re, im)
    doubles (2 * oqh_nn_nweights() of them).  oqh_nn_apply: out = NN(in)
-   (single-precision full fields, in != out).  oqh_nn_grad: mean over the n
+   (single-precision full fields, in != out).  oqh_nn_correct: phi +=
+   NN(r) as the preconditioner of sap_gcr applies it, with the activations
+   between the layers rounded to BF16 (device fields).  oqh_nn_grad: mean over the n
    pairs of the loss |NN(r_k) - e_k|^2 / |e_k|^2, its gradient with respect
    to the weights (dL/dRe w + i dL/dIm w) in dw.  oqh_nn_train_step: one Adam
    step (beta 0.9, 0.999) of step size lr on that loss; returns the loss
@@ -153,6 +155,7 @@ int oqh_nn_nweights(void);
 void oqh_nn_set_weights(const double *w);
 void oqh_nn_get_weights(double *w);
 void oqh_nn_apply(spinor *in, spinor *out);
+void oqh_nn_correct(spinor *r, spinor *phi);
 double oqh_nn_grad(int n, spinor **r, spinor **e, double *dw);
 double oqh_nn_train_step(int n, spinor **r, spinor **e, double lr);
 void oqh_set_nn_prec(int on);

@@ -744,6 +744,23 @@ extern "C" void oqh_nn_apply(spinor *in, spinor *out)
    finish(b, F32, V, true);
 }
 
+// phi += NN(r) as the preconditioner applies it (BF16 activations between
+// the layers); full single-precision device fields
+extern "C" void oqh_nn_correct(spinor *r, spinor *phi)
+{
+   const char *where = "oqh_nn_correct";
+   if (!require_net(where)) return;
+   int o0 = 0, o1 = 0;
+   float2 *a = (float2 *)device_field(r, F32, &o0), *b = (float2 *)device_field(phi, F32, &o1);
+   if (!a || !b || o0 || o1 || a == b)
+   {
+      fail(where, "the fields must be distinct full single-precision device fields");
+      return;
+   }
+   nn_correct(a, b);
+   HIP_OK(hipStreamSynchronize(lib().stream));
+}
+
 extern "C" double oqh_nn_grad(int n, spinor **r, spinor **e, double *dw)
 {
    const char *where = "oqh_nn_grad";

@@ -71,7 +71,7 @@ _SIG = {
     "sap_gcr_multi": (None, [_I, _I, _I, _I, _D, _D, P, P, P, P, P]),
     "oqh_nn_create": (_I, [_I, P]), "oqh_nn_nweights": (_I, []),
     "oqh_nn_set_weights": (None, [P]), "oqh_nn_get_weights": (None, [P]),
-    "oqh_nn_apply": (None, [P, P]), "oqh_nn_grad": (_D, [_I, P, P, P]),
+    "oqh_nn_apply": (None, [P, P]), "oqh_nn_correct": (None, [P, P]), "oqh_nn_grad": (_D, [_I, P, P, P]),
     "oqh_nn_train_step": (_D, [_I, P, P, _D]), "oqh_set_nn_prec": (None, [_I]),
     "oqh_nn_free": (None, []),
     "oqh_Dw_dble_multi": (None, [_D, _I, P, P]), "oqh_Dw_multi": (None, [_F, _I, P, P]),

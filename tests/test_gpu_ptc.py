@@ -75,6 +75,14 @@ def test_forward(cfg, widths, tol):
     r = rel(got, ref)
     print("widths", widths, "forward deviation", r)
     assert r < tol
+    # the preconditioner's form: phi += NN(r), BF16 activations between layers
+    phi0 = rng.standard_normal((lat.V, 24)).astype(np.float32)
+    lat.put_s(b, phi0)
+    lib.oqh_nn_correct(a, b)
+    got2 = ptc.spinor_to_field(lat.get_s(b) - phi0)
+    r2 = rel(got2, ref)
+    print("widths", widths, "preconditioner-form deviation", r2)
+    assert r2 < 2 * tol
     lat.free(a)
     lat.free(b)
 
