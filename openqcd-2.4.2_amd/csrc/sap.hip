@@ -1274,6 +1274,9 @@ __global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int
 #pragma unroll
          for (int c = 0; c < 6; c++) Wn[(size_t)j * 6 + c] = w[c];
       }
+      // the time faces zero s on the open boundary before the other faces
+      // read it (the faces are processed in the order -0, +0, ..., +3)
+      if (ifc < 2) __syncthreads();
    }
 }
 
