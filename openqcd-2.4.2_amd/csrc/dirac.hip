@@ -23,6 +23,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <algorithm>
+
 #include "dirac_dev.h"
 
 namespace oqh {
@@ -50,6 +52,7 @@ template <typename T> struct DArgs
    const int *x0;  // time coordinates (diagonal constant of the S-form)
    int N0, bc;
    T cx1, cx3a, cx3b;  // 4 + m0 in the bulk, + cF - 1 next to open boundaries
+   int xcdc;           // tiles per XCD run (xcd_tile)
 };
 
 
@@ -90,21 +93,21 @@ __device__ inline void hop_all(const DArgs<T> &a, const typename cplx_of<T>::t *
 }
 
 // Work-groups are dispatched round-robin over the 8 XCDs of the chip, each
-// with its own L2.  Tile t of XCD j is made the t-th of a contiguous range of
-// tiles, so that the neighbour points a tile gathers were mostly brought into
-// the same L2 by the tiles processed next to it.
-#ifndef OQH_XCD_REMAP
-#define OQH_XCD_REMAP 1
-#endif
-__device__ inline int xcd_tile(int b, int nb)
+// with its own L2.  The tile of work-group b: with C > 0 each XCD takes runs
+// of C consecutive tiles (C = 1 is the plain dispatch order), with C = 0 one
+// contiguous range of tiles.  Consecutive tiles are neighbours within a
+// thread block of the site order, so a run shares its face lines in one L2.
+__device__ inline int xcd_tile(int b, int nb, int C)
 {
-#if OQH_XCD_REMAP
-   int q = nb >> 3, r = nb & 7, j = b & 7;
-   return j * q + min(j, r) + (b >> 3);
-#else
-   (void)nb;
-   return b;
-#endif
+   if (C == 0)
+   {
+      int q = nb >> 3, r = nb & 7, j = b & 7;
+      return j * q + min(j, r) + (b >> 3);
+   }
+   const int span = 8 * C;
+   if (C <= 1 || b >= nb / span * span) return b;
+   const int g = b / span, rr = b % span;
+   return g * span + (rr & 7) * C + (rr >> 3);
 }
 
 #ifndef OQH_DW_WAVES
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(256) DW_ATTR void k_dirac(DArgs<T> a)
 {
    typedef typename cplx_of<T>::t C;
    int x, odd;
-   int b = xcd_tile(blockIdx.x, gridDim.x);
+   int b = xcd_tile(blockIdx.x, gridDim.x, a.xcdc);
    if (OP == OP_DW)
    {
       int tid = threadIdx.x;
@@ -420,7 +423,7 @@ __global__ __launch_bounds__(TP) void k_dw_tile(DArgs<T> a)
    __shared__ C S[12][TP];
    __shared__ C X[XBUF][6][TP];
    const int t = threadIdx.x;
-   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x, a.xcdc) * TP;
    const int e = e0 + t, o = a.h + e;
    const bool valid = e < a.h;
    bool eb = false, ob = false;
@@ -510,7 +513,7 @@ __global__ __launch_bounds__(TP) void k_hat1_tile(DArgs<T> a)
    typedef typename cplx_of<T>::t C;
    __shared__ C S[12][TP];
    const int t = threadIdx.x;
-   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x, a.xcdc) * TP;
    const int e = e0 + t, o = a.h + e;
    const bool valid = e < a.h;
    bool ob = false;
@@ -564,7 +567,7 @@ __global__ __launch_bounds__(TP) void k_hat2_tile(DArgs<T> a)
    typedef typename cplx_of<T>::t C;
    __shared__ C X[XBUF][6][TP];
    const int t = threadIdx.x;
-   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TP;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x, a.xcdc) * TP;
    const int e = e0 + t, o = a.h + e;
    const bool valid = e < a.h;
    bool eb = false;
@@ -633,6 +636,7 @@ template <typename T> struct MArgs
    int V, h, n;  // n: number of fields of this launch
    T mu, muo, coef;
    int masked;
+   int xcdc;     // tiles per XCD run (xcd_tile; 0: one contiguous range per XCD)
 };
 
 template <typename T, int MU, bool FWD, int TPP, int NR>
@@ -719,7 +723,7 @@ __global__ __launch_bounds__(TPP * NR) void k_dw_multi(MArgs<T> a)
    __shared__ C S[12][NL];
    __shared__ C X[6][NL];
    const int t = threadIdx.x, p = t / NR, k = t % NR;
-   const int e0 = xcd_tile(blockIdx.x, gridDim.x) * TPP;
+   const int e0 = xcd_tile(blockIdx.x, gridDim.x, a.xcdc) * TPP;
    const int e = e0 + p, o = a.h + e;
    const bool valid = e < a.h && k < a.n;
    const C *sk = a.s[k < a.n ? k : 0];
@@ -896,6 +900,10 @@ template <typename T> static DArgs<T> base_args()
    a.cx1 = (T)(4.0 + L.m0);
    a.cx3a = (T)(4.0 + L.m0 + L.cF[0] - 1.0);
    a.cx3b = (T)(4.0 + L.m0 + L.cF[1] - 1.0);
+   // runs of as many tiles as there are thread blocks along x3, between 2
+   // and 8 (measured: 32^3x64 with 8^4 thread blocks best at 4, 48^4 at 6;
+   // the plain order and one contiguous range per XCD are 3-6 % slower)
+   a.xcdc = std::max(2, std::min(8, L.g.L[3] / L.g.Ltrd[3]));
    return a;
 }
 
