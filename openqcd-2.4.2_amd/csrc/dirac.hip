@@ -1110,6 +1110,9 @@ template <typename T> static void apply_multi(T mu, int n, void **sp, void **rp,
       return;
    }
    MArgs<T> a{};
+   // runs of 32 tiles per XCD (measured at 32^3x64: 0.517 ms per field
+   // against 0.547 with one contiguous range per XCD; 2-128 within 2 %)
+   a.xcdc = 32;
    a.u = (const C *)(sizeof(T) == 8 ? (const void *)L.d_ud : (const void *)L.d_u);
    a.sw = (const C *)(sizeof(T) == 8 ? (const void *)L.d_swd : (const void *)L.d_sw);
    a.nbr = L.d_nbr;
