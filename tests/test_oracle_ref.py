@@ -1,6 +1,7 @@
 """Oracle versus the reference build (oracle/_ref, compiled from the
 reference's own sources by oracle/build_ref.py).  Skipped where the
-reference build is absent (it is made only where /root/reference exists)."""
+reference build is absent (it is made only where the reference sources are
+present)."""
 import json
 import os
 
