@@ -38,6 +38,7 @@
 namespace oqh {
 
 constexpr int SWG = 256;  // work-group size of the block kernels
+constexpr int NBF = 4;    // block fields of the API per block: s[0..2] + the Dwhat_blk intermediate
 #ifndef OQH_SAP_REG
 #define OQH_SAP_REG 1
 #endif
@@ -263,10 +264,11 @@ __device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0,
    }
 }
 
-// even points: S2_e = Dwhat_blk S1 (Dw.c:1828), odd half of S2 used for the
-// intermediate A_oo^-1 D_oe S1
+// even points: S2_e = Dwhat_blk S1 (Dw.c:1828); the odd half of T holds the
+// intermediate A_oo^-1 D_oe S1 (T = S2 inside the Schwarz cycles, a separate
+// field for the block API, which leaves the odd half of S2 unchanged)
 template <typename C>
-__device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decltype(C::x) mu)
+__device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T, decltype(C::x) mu)
 {
    typedef decltype(C::x) R;
    for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
@@ -276,7 +278,7 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decl
       if (a.masked && a.bnd[gx])
       {
          zero12(o);
-         store_spinor(S2, i, o);
+         store_spinor(T, i, o);
          continue;
       }
       C acc[12];
@@ -285,7 +287,7 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decl
 #pragma unroll
       for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], (R)-0.5);
       pauli_site(a.sw, gx, (R)0, acc, o);
-      store_spinor(S2, i, o);
+      store_spinor(T, i, o);
    }
    __syncthreads();
    for (int i = threadIdx.x; i < a.bh; i += SWG)
@@ -300,7 +302,7 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decl
       }
       C acc[12], v[12];
       zero12(acc);
-      blk_hop(a, imb, S2, i, gx, acc);
+      blk_hop(a, imb, T, i, gx, acc);
       load_spinor((const C *)S1, i, v);
       pauli_site(a.sw, gx, mu, v, o);
 #pragma unroll
@@ -309,9 +311,10 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, decl
    }
 }
 
-// minimal residual steps on points [0, nv) (blk_solv.c:1117-1186)
+// minimal residual steps on points [0, nv) (blk_solv.c:1117-1186); T: odd
+// intermediate of Dwhat_blk (EO)
 template <bool EO, typename C>
-__device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C *S2, decltype(C::x) mu)
+__device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C *S2, C *T, decltype(C::x) mu)
 {
    typedef decltype(C::x) R;
    const R rmin = sizeof(R) == 4 ? (R)FLT_MIN : (R)DBL_MIN;
@@ -321,7 +324,7 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
    for (int imr = 0; imr < a.nmr; imr++)
    {
       if (EO)
-         blk_dwhat(a, imb, S1, S2, mu);
+         blk_dwhat(a, imb, S1, S2, T, mu);
       else
          blk_dw(a, imb, S1, S2, 0, a.bvol, mu);
       __syncthreads();
@@ -464,12 +467,12 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgsT<C> a)
    if (ISOLV)
    {
       blk_eo_prepare(a, imb, S1);
-      blk_mres_dev<true>(a, imb, S0, S1, S2, a.mu);
+      blk_mres_dev<true>(a, imb, S0, S1, S2, S2, a.mu);
       blk_eo_finish(a, imb, S0, S1);
       __syncthreads();
    }
    else
-      blk_mres_dev<false>(a, imb, S0, S1, S2, a.mu);
+      blk_mres_dev<false>(a, imb, S0, S1, S2, S2, a.mu);
 
    for (int i = threadIdx.x; i < a.bvol; i += SWG)
    {
@@ -1116,8 +1119,9 @@ enum BlkOp { BOP_DW = 0, BOP_DWHAT, BOP_MRES, BOP_EO_MRES, BOP_DWEE, BOP_DWOO, B
 __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, int l, float mu)
 {
    const int *imb = a.imb + (size_t)n * a.bvol;
-   float2 *F = a.scr + (size_t)n * 3 * a.Ub;  // the block's fields 0,1,2
+   float2 *F = a.scr + (size_t)n * NBF * a.Ub;  // the block's fields 0,1,2
    float2 *Sk = F + (size_t)k * a.Ub, *Sl = F + (size_t)l * a.Ub;
+   float2 *T = F + 3 * a.Ub;  // odd intermediate of Dwhat_blk
    // open bc: the operators zero their input on the boundary points (all of
    // them for Dw_blk, the even ones for Dwhat_blk, Dwee_blk and Dwoe_blk,
    // the odd ones for Dwoo_blk and Dweo_blk); in the MinRes solvers the
@@ -1135,8 +1139,8 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
       blk_dw(a, imb, Sk, Sl, 0, a.bvol, mu);
    else if (op == BOP_DWHAT)
    {
-      // Dwhat_blk keeps the odd half of the output: use field 2 only if l != 2
-      blk_dwhat(a, imb, Sk, Sl, mu);
+      // Dwhat_blk leaves the odd half of the output unchanged
+      blk_dwhat(a, imb, Sk, Sl, T, mu);
    }
    else if (op == BOP_DWEE || op == BOP_DWOO)
    {
@@ -1189,9 +1193,9 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
       }
    }
    else if (op == BOP_MRES)
-      blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
+      blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu);
    else
-      blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, mu);
+      blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu);
 }
 
 // Dw_bnd (dirac/Dw_bnd.c:801-1011): for the face points x of block n the
@@ -1207,7 +1211,7 @@ struct FaceOfs
 __global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int *ipp, FaceOfs fo, float2 *W)
 {
    const int *imb = a.imb + (size_t)n * a.bvol;
-   float2 *S = a.scr + ((size_t)n * 3 + k) * a.Ub;
+   float2 *S = a.scr + ((size_t)n * NBF + k) * a.Ub;
    float2 *Wn = W + (size_t)n * fo.o[8] * 6;
    for (int ifc = 0; ifc < 8; ifc++)
    {
@@ -2126,7 +2130,7 @@ static bool blk_check(int n, int k, const char *where)
       fail(where, "Block number or field index is out of range");
       return false;
    }
-   if (!G.d_bs) G.d_bs = (float2 *)dev_alloc((size_t)G.nb * 3 * G.Ub * sizeof(float2));
+   if (!G.d_bs) G.d_bs = (float2 *)dev_alloc((size_t)G.nb * NBF * G.Ub * sizeof(float2));
    return true;
 }
 
@@ -2138,7 +2142,7 @@ static void blk_copy(int n, ptset_t set, int k, spinor *s, int to_blk, const cha
    if (!r.dev) return;
    int i0 = (set == ODD_PTS) ? G.bh : 0, i1 = (set == EVEN_PTS) ? G.bh : G.bvol;
    if (set == NO_PTS) i1 = i0;
-   float2 *blk = G.d_bs + ((size_t)n * 3 + k) * G.Ub;
+   float2 *blk = G.d_bs + ((size_t)n * NBF + k) * G.Ub;
    if (i1 > i0)
       k_blk_copy<<<(i1 - i0 + 255) / 256, 256, 0, L.stream>>>((float2 *)r.dev, blk, G.d_imb + (size_t)n * G.bvol,
                                                               i0, i1, to_blk);
@@ -2294,7 +2298,7 @@ extern "C" void oqh_export_sblk(int n, int k, spinor *host)
    if (!blk_check(n, k, "oqh_export_sblk")) return;
    size_t bytes = (size_t)G.bvol * sizeof(spinor);
    void *aos = tmp_buffer(bytes);
-   blocks_to_aos_f32(G.d_bs + ((size_t)n * 3 + k) * G.Ub, (float2 *)aos, G.bvol, 12, L.stream);
+   blocks_to_aos_f32(G.d_bs + ((size_t)n * NBF + k) * G.Ub, (float2 *)aos, G.bvol, 12, L.stream);
    HIP_OK(hipMemcpyAsync(host, aos, bytes, hipMemcpyDeviceToHost, L.stream));
    HIP_OK(hipStreamSynchronize(L.stream));
 }

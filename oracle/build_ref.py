@@ -45,6 +45,10 @@ CONFIGS = {
     "8x8x8x8_t8": ((8, 8, 8, 8), (8, 8, 8, 8), False),
     "8x8x8x8_t4": ((8, 8, 8, 8), (4, 4, 4, 4), False),
     "16x8x8x8_t8": ((16, 8, 8, 8), (8, 8, 8, 8), False),
+    # generic SAP block sizes: 8^4 blocks on 16^4 and 6^4 blocks on 12^4
+    # (16 blocks, an even multiple of NTHREAD = 8 and 4)
+    "16x16x16x16_t8": ((16, 16, 16, 16), (8, 8, 8, 16), False),
+    "12x12x12x12_t4": ((12, 12, 12, 12), (6, 6, 12, 12), False),
     "64x32x32x32_avx": ((64, 32, 32, 32), (16, 16, 16, 32), True),
 }
 

@@ -157,7 +157,7 @@ void oc_destroy(oc_ctx *c)
    if (!c) return;
    free(c->ipt); free(c->lex); free(c->iup); free(c->idn); free(c->x0);
    free(c->rs); free(c->rd);
-   free(c->ud); free(c->u); free(c->swd); free(c->sw); free(c->sw_bgr);
+   free(c->ud); free(c->u); free(c->swd); free(c->sw); free(c->sw_bgr); free(c->swd_bgr);
    if (c->blk)
    {
       for (int n = 0; n < c->nb; n++)

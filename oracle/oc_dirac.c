@@ -12,6 +12,12 @@
 #define SWFLD(c) ((c)->swd)
 #define MULP oc_mul_pauli_dble
 #include "oc_dirac_tmpl.h"
+
+/* double hopping term shared with the FP64 block operators in oc_sap.c */
+void oc_hop_term_d(int mu, int sgn, int dag, const double *u, const double *v, double coef, double *r)
+{
+   hop_term_dble(mu, sgn, dag, u, v, coef, r);
+}
 #undef R
 #undef SFX
 #undef UFLD

@@ -58,6 +58,7 @@ struct oc_ctx
    int nb, nbh, bvol;
    oc_block *blk;
    float *sw_bgr;   /* FP32 Pauli for block solver, odd part inverted if isolv */
+   double *swd_bgr; /* FP64 version of it (odd part inverted, not rounded) */
    int sap_ready;
 };
 

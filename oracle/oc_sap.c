@@ -27,6 +27,7 @@
 #include "oc_internal.h"
 
 void oc_hop_term_f(int mu, int sgn, int dag, const float *u, const float *v, float coef, float *r);
+void oc_hop_term_d(int mu, int sgn, int dag, const double *u, const double *v, double coef, double *r);
 
 static const int *blk_cbs(const int *bs, int *cbs)
 {
@@ -135,7 +136,9 @@ int oc_sap_prepare(oc_ctx *c)
    oc_assign_ud2u(c);
    if (c->swd_inv[0] || c->swd_inv[1]) oc_sw_term(c, 3);
    free(c->sw_bgr);
+   free(c->swd_bgr);
    c->sw_bgr = malloc(sizeof(float) * 72 * (size_t)c->V);
+   c->swd_bgr = malloc(sizeof(double) * 72 * (size_t)c->V);
    for (int ix = 0; ix < c->V; ix++)
    {
       const double *p = c->swd + (size_t)72 * ix;
@@ -146,7 +149,11 @@ int oc_sap_prepare(oc_ctx *c)
          ifail |= oc_inv_pauli_dble(0.0, p + 36, ms + 36);
          p = ms;
       }
-      for (int i = 0; i < 72; i++) c->sw_bgr[(size_t)72 * ix + i] = (float)p[i];
+      for (int i = 0; i < 72; i++)
+      {
+         c->sw_bgr[(size_t)72 * ix + i] = (float)p[i];
+         c->swd_bgr[(size_t)72 * ix + i] = p[i];
+      }
    }
    c->sap_ready = 1;
    return ifail;
@@ -161,29 +168,6 @@ static int blk_bnd_pt(const oc_ctx *c, const oc_block *b, int x)
    return (t == 0) || (t == c->L[0] - 1 && c->bc == 0);
 }
 
-static void blk_pauli(const oc_ctx *c, const oc_block *b, int x, float mu, const float *s, float *r)
-{
-   const float *m = c->sw_bgr + (size_t)72 * b->imb[x];
-   oc_mul_pauli(mu, m, s, r);
-   oc_mul_pauli(-mu, m + 36, s + 12, r + 12);
-}
-
-/* r[x] += coef * hopping terms from in-block neighbours of x */
-static void blk_hop(const oc_ctx *c, const oc_block *b, int x, const float *s,
-                    const unsigned char *skip, float coef, float *r)
-{
-   int gx = b->imb[x];
-   for (int mu = 0; mu < 4; mu++)
-   {
-      int y = b->iup[x][mu];
-      if (y < b->vol && !(skip && skip[y]))
-         oc_hop_term_f(mu, +1, 0, c->u + (size_t)18 * oc_link_fwd(c, gx, mu), s + 24 * (size_t)y, coef, r);
-      y = b->idn[x][mu];
-      if (y < b->vol && !(skip && skip[y]))
-         oc_hop_term_f(mu, -1, 1, c->u + (size_t)18 * oc_link_bwd(c, gx, mu), s + 24 * (size_t)y, coef, r);
-   }
-}
-
 static unsigned char *bnd_mask(const oc_ctx *c, const oc_block *b)
 {
    unsigned char *m = calloc(b->vol + 1, 1);
@@ -191,251 +175,72 @@ static unsigned char *bnd_mask(const oc_ctx *c, const oc_block *b)
    return m;
 }
 
+/* ---- FP64 BLAS-1 of the FP64 instance: plain double sums ---- */
+
+static double nsq_d(int n, const double *s)
+{
+   double x = 0.0;
+   for (size_t i = 0; i < (size_t)24 * n; i++) x += s[i] * s[i];
+   return x;
+}
+
+static void sprod_d(int n, const double *s, const double *r, double *z)
+{
+   double vr = 0.0, vi = 0.0;
+   for (size_t i = 0; i < (size_t)12 * n; i++)
+   {
+      vr += s[2 * i] * r[2 * i] + s[2 * i + 1] * r[2 * i + 1];
+      vi += s[2 * i] * r[2 * i + 1] - s[2 * i + 1] * r[2 * i];
+   }
+   z[0] = vr;
+   z[1] = vi;
+}
+
+#define R float
+#define SFX
+#define UFLD(c) ((c)->u)
+#define SWB(c) ((c)->sw_bgr)
+#define MULP oc_mul_pauli
+#define HOP oc_hop_term_f
+#define NSQ oc_norm_square
+#define SPROD oc_spinor_prod
+#define MULCADD oc_mulc_spinor_add
+#define RMIN FLT_MIN
+#include "oc_sap_tmpl.h"
+#undef R
+#undef SFX
+#undef UFLD
+#undef SWB
+#undef MULP
+#undef HOP
+#undef NSQ
+#undef SPROD
+#undef MULCADD
+#undef RMIN
+
+#define R double
+#define SFX _dble
+#define UFLD(c) ((c)->ud)
+#define SWB(c) ((c)->swd_bgr)
+#define MULP oc_mul_pauli_dble
+#define HOP oc_hop_term_d
+#define NSQ nsq_d
+#define SPROD sprod_d
+#define MULCADD oc_mulc_spinor_add_dble
+#define RMIN DBL_MIN
+#include "oc_sap_tmpl.h"
+#undef R
+#undef SFX
+#undef UFLD
+#undef SWB
+#undef MULP
+#undef HOP
+#undef NSQ
+#undef SPROD
+#undef MULCADD
+#undef RMIN
+
 #define ZS(p, x) memset((p) + 24 * (size_t)(x), 0, 24 * sizeof(float))
-
-void oc_Dw_blk(oc_ctx *c, int n, float mu, const float *sin, float *r)
-{
-   oc_block *b = c->blk + n;
-   int vol = b->vol, h = vol / 2;
-   float muo = c->eoflg ? 0.0f : mu;
-   float *s = malloc(sizeof(float) * 24 * (vol + 1));
-   memcpy(s, sin, sizeof(float) * 24 * vol);
-   ZS(s, vol);
-   unsigned char *bp = bnd_mask(c, b);
-   for (int x = 0; x < vol; x++)
-      if (bp[x]) ZS(s, x);
-   for (int x = 0; x < vol; x++)
-   {
-      float *rr = r + 24 * (size_t)x;
-      if (bp[x]) { ZS(r, x); continue; }
-      blk_pauli(c, b, x, (x < h) ? mu : muo, s + 24 * (size_t)x, rr);
-      blk_hop(c, b, x, s, NULL, -0.5f, rr);
-   }
-   ZS(r, vol);
-   free(bp);
-   free(s);
-}
-
-/* even-site Schur complement on the block; sw_bgr odd part must be inverted */
-void oc_Dwhat_blk(oc_ctx *c, int n, float mu, const float *sin, float *r)
-{
-   oc_block *b = c->blk + n;
-   int vol = b->vol, h = vol / 2;
-   float *s = calloc((size_t)24 * (vol + 1), sizeof(float));
-   float *t = calloc((size_t)24 * (vol + 1), sizeof(float));
-   memcpy(s, sin, sizeof(float) * 24 * vol);
-   unsigned char *bp = bnd_mask(c, b);
-   for (int x = 0; x < h; x++) blk_pauli(c, b, x, mu, s + 24 * (size_t)x, r + 24 * (size_t)x);
-   for (int x = 0; x < h; x++)
-      if (bp[x]) ZS(s, x);
-   for (int x = h; x < vol; x++)
-   {
-      if (bp[x]) continue;
-      float acc[24] = {0};
-      blk_hop(c, b, x, s, NULL, -0.5f, acc);
-      blk_pauli(c, b, x, 0.0f, acc, t + 24 * (size_t)x);
-   }
-   for (int x = 0; x < h; x++) blk_hop(c, b, x, t, bp, 0.5f, r + 24 * (size_t)x);
-   for (int x = 0; x < h; x++)
-      if (bp[x]) ZS(r, x);
-   free(bp);
-   free(s);
-   free(t);
-}
-
-static void blk_Dwoo(oc_ctx *c, const oc_block *b, float mu, float *s, float *r)
-{
-   if (c->eoflg) mu = 0.0f;
-   for (int x = b->vol / 2; x < b->vol; x++)
-   {
-      if (blk_bnd_pt(c, b, x)) { ZS(s, x); ZS(r, x); continue; }
-      float out[24];
-      blk_pauli(c, b, x, mu, s + 24 * (size_t)x, out);
-      memcpy(r + 24 * (size_t)x, out, sizeof(out));
-   }
-}
-
-static void blk_Dwoe(oc_ctx *c, const oc_block *b, float *s, float *r)
-{
-   int h = b->vol / 2;
-   for (int x = 0; x < h; x++)
-      if (blk_bnd_pt(c, b, x)) ZS(s, x);
-   for (int x = h; x < b->vol; x++)
-   {
-      float acc[24] = {0};
-      if (!blk_bnd_pt(c, b, x)) blk_hop(c, b, x, s, NULL, -0.5f, acc);
-      memcpy(r + 24 * (size_t)x, acc, sizeof(acc));
-   }
-}
-
-static void blk_Dweo(oc_ctx *c, const oc_block *b, float *s, float *r)
-{
-   int h = b->vol / 2;
-   unsigned char *bp = bnd_mask(c, b);
-   for (int x = h; x < b->vol; x++)
-      if (bp[x]) ZS(s, x);
-   for (int x = 0; x < h; x++) blk_hop(c, b, x, s, bp, 0.5f, r + 24 * (size_t)x);
-   for (int x = 0; x < h; x++)
-      if (bp[x]) ZS(r, x);
-   free(bp);
-}
-
-static void blk_Dwee(oc_ctx *c, const oc_block *b, float mu, float *s, float *r)
-{
-   for (int x = 0; x < b->vol / 2; x++)
-   {
-      if (blk_bnd_pt(c, b, x)) { ZS(s, x); ZS(r, x); continue; }
-      float out[24];
-      blk_pauli(c, b, x, mu, s + 24 * (size_t)x, out);
-      memcpy(r + 24 * (size_t)x, out, sizeof(out));
-   }
-}
-
-/* Dwee_blk, Dwoo_blk, Dwoe_blk, Dweo_blk (dirac/Dw.c:1584-1826) on copies
-   of the block fields b.s[k] = s and b.s[l] = r (vol points each, block
-   order); s and r are written back, with the zeros the operators put on the
-   boundary points of open bc.  op: 0 Dwee, 1 Dwoo, 2 Dwoe, 3 Dweo. */
-void oc_blk_eo_op(oc_ctx *c, int n, int op, float mu, float *s, float *r)
-{
-   oc_block *b = c->blk + n;
-   size_t nb = sizeof(float) * 24 * (size_t)b->vol;
-   float *sc = calloc((size_t)24 * (b->vol + 1), sizeof(float));
-   float *rc = calloc((size_t)24 * (b->vol + 1), sizeof(float));
-   memcpy(sc, s, nb);
-   memcpy(rc, r, nb);
-   if (op == 0) blk_Dwee(c, b, mu, sc, rc);
-   else if (op == 1) blk_Dwoo(c, b, mu, sc, rc);
-   else if (op == 2) blk_Dwoe(c, b, sc, rc);
-   else blk_Dweo(c, b, sc, rc);
-   memcpy(s, sc, nb);
-   memcpy(r, rc, nb);
-   free(sc);
-   free(rc);
-}
-
-static void mres(oc_ctx *c, int n, float mu, int nmr, int eo, float *s0, float *s1)
-{
-   int vol = c->blk[n].vol, nv = eo ? vol / 2 : vol;
-   float *s2 = calloc((size_t)24 * (vol + 1), sizeof(float));
-   memset(s0, 0, sizeof(float) * 24 * nv);
-   for (int imr = 0; imr < nmr; imr++)
-   {
-      if (eo) oc_Dwhat_blk(c, n, mu, s1, s2);
-      else oc_Dw_blk(c, n, mu, s1, s2);
-      float r = oc_norm_square(nv, s2), z[2];
-      if (r < 2.0f * FLT_MIN) break;
-      oc_spinor_prod(nv, s2, s1, z);
-      r = 1.0f / r;
-      z[0] *= r;
-      z[1] *= r;
-      oc_mulc_spinor_add(nv, s0, s1, z[0], z[1]);
-      oc_mulc_spinor_add(nv, s1, s2, -z[0], -z[1]);
-   }
-   free(s2);
-}
-
-void oc_blk_mres(oc_ctx *c, int n, float mu, int nmr, float *s0, float *s1)
-{
-   mres(c, n, mu, nmr, 0, s0, s1);
-}
-
-/* rho(y) -= hopping term from s0 for every exterior neighbour y */
-static void blk_bnd_update(oc_ctx *c, const oc_block *b, const float *s0, float *rho)
-{
-   for (int x = 0; x < b->vol; x++)
-   {
-      int gx = b->imb[x];
-      for (int mu = 0; mu < 4; mu++)
-      {
-         if (b->idn[x][mu] == b->vol)
-         {
-            /* y = x - mu receives -1/2 (1-gamma_mu) U(y,mu) s0(x) */
-            int gy = c->idn[gx][mu];
-            float t[24] = {0};
-            oc_hop_term_f(mu, +1, 0, c->u + (size_t)18 * oc_link_bwd(c, gx, mu), s0 + 24 * (size_t)x, -0.5f, t);
-            for (int i = 0; i < 24; i++) rho[24 * (size_t)gy + i] -= t[i];
-         }
-         if (b->iup[x][mu] == b->vol)
-         {
-            /* y = x + mu receives -1/2 (1+gamma_mu) U(x,mu)^dag s0(x) */
-            int gy = c->iup[gx][mu];
-            float t[24] = {0};
-            oc_hop_term_f(mu, -1, 1, c->u + (size_t)18 * oc_link_fwd(c, gx, mu), s0 + 24 * (size_t)x, -0.5f, t);
-            for (int i = 0; i < 24; i++) rho[24 * (size_t)gy + i] -= t[i];
-         }
-      }
-   }
-}
-
-void oc_sap(oc_ctx *c, float mu, int isolv, int nmr, float *psi, float *rho)
-{
-   if (!c->sap_ready) oc_sap_prepare(c);
-   int vol = c->bvol, h = vol / 2;
-   float *s0 = calloc((size_t)24 * (vol + 1) * c->nb, sizeof(float));
-   float *s1 = calloc((size_t)24 * (vol + 1), sizeof(float));
-   for (int ic = 0; ic < 2; ic++)
-   {
-      int n0 = ic ? c->nbh : 0, n1 = ic ? c->nb : c->nbh;
-      for (int n = n0; n < n1; n++)
-      {
-         oc_block *b = c->blk + n;
-         float *b0 = s0 + (size_t)24 * (vol + 1) * n;
-         for (int x = 0; x < vol; x++)
-            memcpy(s1 + 24 * (size_t)x, rho + 24 * (size_t)b->imb[x], 24 * sizeof(float));
-         ZS(s1, vol);
-         if (isolv == 0)
-         {
-            mres(c, n, mu, nmr, 0, b0, s1);
-            for (int x = 0; x < vol; x++)
-            {
-               float *p = psi + 24 * (size_t)b->imb[x], *q = rho + 24 * (size_t)b->imb[x];
-               for (int i = 0; i < 24; i++) p[i] += b0[24 * (size_t)x + i];
-               memcpy(q, s1 + 24 * (size_t)x, 24 * sizeof(float));
-            }
-         }
-         else
-         {
-            blk_Dwoo(c, b, 0.0f, s1, s1);
-            blk_Dweo(c, b, s1, s1);
-            mres(c, n, mu, nmr, 1, b0, s1);
-            blk_Dwoe(c, b, b0, b0);
-            blk_Dwoo(c, b, 0.0f, b0, b0);
-            for (int x = 0; x < vol; x++)
-            {
-               float *p = psi + 24 * (size_t)b->imb[x], *q = rho + 24 * (size_t)b->imb[x];
-               float *a = b0 + 24 * (size_t)x, *r1 = s1 + 24 * (size_t)x;
-               if (x < h)
-               {
-                  for (int i = 0; i < 24; i++) p[i] += a[i];
-                  memcpy(q, r1, 24 * sizeof(float));
-               }
-               else
-               {
-                  for (int i = 0; i < 24; i++)
-                  {
-                     a[i] = r1[i] - a[i];
-                     p[i] += a[i];
-                  }
-                  if (c->eoflg)
-                     memset(q, 0, 24 * sizeof(float));
-                  else
-                     for (int i = 0; i < 12; i++)
-                     {
-                        /* rho = -i mu gamma5 s0: spins 1,2 get -i mu, spins 3,4 +i mu */
-                        float cm = (i < 6) ? -mu : mu;
-                        q[2 * i] = -cm * a[2 * i + 1];
-                        q[2 * i + 1] = cm * a[2 * i];
-                     }
-               }
-            }
-         }
-      }
-      for (int n = n0; n < n1; n++)
-         blk_bnd_update(c, c->blk + n, s0 + (size_t)24 * (vol + 1) * n, rho);
-   }
-   free(s0);
-   free(s1);
-}
 
 /* ------------------------------------------------------------------ */
 /* flexible GCR and sap_gcr                                             */

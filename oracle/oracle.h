@@ -121,6 +121,15 @@ void oc_Dw_blk(oc_ctx *c, int n, float mu, const float *s, float *r); /* block-l
 void oc_Dwhat_blk(oc_ctx *c, int n, float mu, const float *s, float *r);
 void oc_blk_eo_op(oc_ctx *c, int n, int op, float mu, float *s, float *r); /* 0 Dwee 1 Dwoo 2 Dwoe 3 Dweo _blk */
 void oc_blk_mres(oc_ctx *c, int n, float mu, int nmr, float *s0, float *s1);
+void oc_blk_eo_mres(oc_ctx *c, int n, float mu, int nmr, float *s0, float *s1);
+/* FP64 block operators (Dw_dble.c:1891-2335) and the FP64 Schwarz sweep on
+   the FP64 links and the FP64 block SW field */
+void oc_Dw_blk_dble(oc_ctx *c, int n, double mu, const double *s, double *r);
+void oc_Dwhat_blk_dble(oc_ctx *c, int n, double mu, const double *s, double *r);
+void oc_blk_eo_op_dble(oc_ctx *c, int n, int op, double mu, double *s, double *r);
+void oc_blk_mres_dble(oc_ctx *c, int n, double mu, int nmr, double *s0, double *s1);
+void oc_blk_eo_mres_dble(oc_ctx *c, int n, double mu, int nmr, double *s0, double *s1);
+void oc_sap_dble(oc_ctx *c, double mu, int isolv, int nmr, double *psi, double *rho);
 void oc_sap(oc_ctx *c, float mu, int isolv, int nmr, float *psi, float *rho);
 double oc_sap_gcr(oc_ctx *c, int nkv, int nmx, int istop, double res, double mu,
                   double *eta, double *psi, int *ifail, int *status);

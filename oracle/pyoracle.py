@@ -58,7 +58,13 @@ def lib():
             "oc_Dw_blk": (None, [P, i, f, P, P]), "oc_Dwhat_blk": (None, [P, i, f, P, P]),
             "oc_blk_eo_op": (None, [P, i, i, f, P, P]),
             "oc_blk_mres": (None, [P, i, f, i, P, P]),
+            "oc_blk_eo_mres": (None, [P, i, f, i, P, P]),
             "oc_sap": (None, [P, f, i, i, P, P]),
+            "oc_Dw_blk_dble": (None, [P, i, d, P, P]), "oc_Dwhat_blk_dble": (None, [P, i, d, P, P]),
+            "oc_blk_eo_op_dble": (None, [P, i, i, d, P, P]),
+            "oc_blk_mres_dble": (None, [P, i, d, i, P, P]),
+            "oc_blk_eo_mres_dble": (None, [P, i, d, i, P, P]),
+            "oc_sap_dble": (None, [P, d, i, i, P, P]),
             "oc_sap_gcr": (d, [P, i, i, i, d, d, P, P, P, P]),
             "oc_plaq_frc": (None, [P, P]), "oc_expXsu3": (None, [d, P, P]),
             "oc_fwd_euler": (None, [P, i, d]),
@@ -272,8 +278,51 @@ class Oracle:
         lib().oc_blk_mres(self.h, n, mu, nmr, ptr(s0), ptr(s1))
         return s0, s1
 
+    def Dwhat_blk(self, n, mu, s):
+        """Dwhat_blk on block field s (vol+1 points); even half of the result
+        (the odd half is zero)."""
+        r = np.zeros_like(s)
+        lib().oc_Dwhat_blk(self.h, n, mu, ptr(s), ptr(r))
+        return r
+
+    def blk_eo_mres(self, n, mu, nmr, s1):
+        s0 = np.zeros_like(s1)
+        lib().oc_blk_eo_mres(self.h, n, mu, nmr, ptr(s0), ptr(s1))
+        return s0, s1
+
     def sap(self, mu, isolv, nmr, psi, rho):
         lib().oc_sap(self.h, mu, isolv, nmr, ptr(psi), ptr(rho))
+
+    # FP64 block operators and the FP64 Schwarz sweep (FP64 links, FP64 block
+    # SW field with the odd part inverted when isolv=1)
+    def Dw_blk_dble(self, n, mu, s):
+        r = np.zeros_like(s)
+        lib().oc_Dw_blk_dble(self.h, n, mu, ptr(s), ptr(r))
+        return r
+
+    def Dwhat_blk_dble(self, n, mu, s):
+        r = np.zeros_like(s)
+        lib().oc_Dwhat_blk_dble(self.h, n, mu, ptr(s), ptr(r))
+        return r
+
+    def blk_eo_op_dble(self, n, op, mu, s, r):
+        s = np.array(s, np.float64)
+        r = np.array(r, np.float64)
+        lib().oc_blk_eo_op_dble(self.h, n, op, mu, ptr(s), ptr(r))
+        return s, r
+
+    def blk_mres_dble(self, n, mu, nmr, s1):
+        s0 = np.zeros_like(s1)
+        lib().oc_blk_mres_dble(self.h, n, mu, nmr, ptr(s0), ptr(s1))
+        return s0, s1
+
+    def blk_eo_mres_dble(self, n, mu, nmr, s1):
+        s0 = np.zeros_like(s1)
+        lib().oc_blk_eo_mres_dble(self.h, n, mu, nmr, ptr(s0), ptr(s1))
+        return s0, s1
+
+    def sap_dble(self, mu, isolv, nmr, psi, rho):
+        lib().oc_sap_dble(self.h, mu, isolv, nmr, ptr(psi), ptr(rho))
 
     def sap_gcr(self, nkv, nmx, istop, res, mu, eta, psi=None):
         if psi is None:

@@ -17,9 +17,16 @@ CASES = [
     ("8x8x8x8_t8", dict(bc=0, csw=1.77, m0=-0.3, seed=2, mu=0.0, isolv=0), {"cF": (1.0, 1.0)}),
     ("16x8x8x8_t8", dict(bc=0, csw=1.77, m0=-0.3, seed=5, mu=0.1, isolv=1, cF=(0.95, 0.9)), {}),
     ("8x8x8x8_t4", dict(bc=3, csw=1.2, m0=-0.35, seed=4, mu=0.0, isolv=0), {}),
+    # generic block sizes: 8^4 blocks on 16^4, 6^4 blocks on 12^4
+    ("16x16x16x16_t8", dict(bc=3, csw=1.77, m0=-0.35, seed=6, mu=0.0, isolv=1, bs=(8, 8, 8, 8)), {}),
+    ("16x16x16x16_t8", dict(bc=3, csw=1.77, m0=-0.35, seed=7, mu=0.05, isolv=0, bs=(8, 8, 8, 8)), {}),
+    ("12x12x12x12_t4", dict(bc=0, csw=1.77, m0=-0.3, seed=2, mu=0.1, isolv=1, bs=(6, 6, 6, 6)), {}),
+    ("12x12x12x12_t4", dict(bc=3, csw=1.77, m0=-0.35, seed=4, mu=0.0, isolv=0, bs=(6, 6, 6, 6)), {}),
 ]
 GEOM = {"8x8x8x8_t8": ((8, 8, 8, 8), (8, 8, 8, 8)), "16x8x8x8_t8": ((16, 8, 8, 8), (8, 8, 8, 8)),
-        "8x8x8x8_t4": ((8, 8, 8, 8), (4, 4, 4, 4))}
+        "8x8x8x8_t4": ((8, 8, 8, 8), (4, 4, 4, 4)),
+        "16x16x16x16_t8": ((16, 16, 16, 16), (8, 8, 8, 16)),
+        "12x12x12x12_t4": ((12, 12, 12, 12), (6, 6, 12, 12))}
 
 
 @pytest.fixture(scope="module", params=range(len(CASES)))
@@ -30,14 +37,15 @@ def case(request, tmp_path_factory):
     out = str(tmp_path_factory.mktemp("ref_%s_%d" % (tag, request.param)))
     args = dict(kw)
     cF = args.pop("cF", (1.0, 1.0))
+    bs = args.pop("bs", (4, 4, 4, 4))
     ld = po.ref_dump(tag, out, bc=args["bc"], csw=args["csw"], m0=args["m0"], seed=args["seed"],
-                     mu=args["mu"], isolv=args["isolv"], cF=cF, bs=(4, 4, 4, 4), nmr=4, ncy=5,
+                     mu=args["mu"], isolv=args["isolv"], cF=cF, bs=bs, nmr=4, ncy=5,
                      nkv=16, nmx=256, res=1e-10)
     L, T = GEOM[tag]
     o, eta, s32 = make_oracle(L, T, bc=args["bc"], cF=cF, m0=args["m0"], csw=args["csw"],
                               seed=args["seed"])
     info = json.load(open(os.path.join(out, "info.json")))
-    return dict(ld=ld, o=o, eta=eta, s32=s32, V=o.V, mu=args["mu"], isolv=args["isolv"], info=info)
+    return dict(ld=ld, o=o, eta=eta, s32=s32, V=o.V, mu=args["mu"], isolv=args["isolv"], info=info, bs=bs)
 
 
 def test_fields(case):
@@ -103,7 +111,7 @@ def test_sap_and_solver(case):
     if not case["info"]["sap"]:
         pytest.skip("block grid not allowed for this thread layout")
     o.sw_term(3)
-    o.set_sap((4, 4, 4, 4), isolv, 4, 5)
+    o.set_sap(case["bs"], isolv, 4, 5)
     assert o.sap_prepare() == case["info"]["ifail_bgr"]
     psi = np.zeros((V, 24), np.float32)
     rho = case["s32"].copy()
