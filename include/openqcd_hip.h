@@ -79,6 +79,9 @@ void oqh_set_bc_parms(int type, double cF, double cF_prime);
 void oqh_set_sw_parms(double m0, double csw);
 void oqh_set_tm_parms(int eoflg);
 void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy);
+/* flags/sap_parms.c:161 (sap_parms): the current SAP parameters (ncy = 0:
+   not set) */
+void oqh_get_sap_parms(int *bs, int *isolv, int *nmr, int *ncy);
 /* storage precision of the SAP block solver (sap, sap_gcr; no counterpart in
    the reference): 0 = FP32 fields as in sap/sap.c (default), 1 = FP16 links,
    SW matrices and spinor fields (psi, rho and the block corrections, with a
@@ -96,9 +99,12 @@ void oqh_set_sap_graph(int on);
    legal block sizes of sap_parms.c (even, >= 4, an even number of blocks per
    axis; here with equal spatial sizes and at most maxvol points, 4096 if
    maxvol <= 0) picks the one with the least time per e-fold of residual
-   reduction in ncy Schwarz cycles of the current precision variant, sets it
-   with the given isolv, nmr, ncy and returns that time in seconds (bs_out:
-   the block size); 0 on failure. */
+   reduction in ncy Schwarz cycles (the given isolv and nmr, the current
+   precision variant) and returns that time in seconds (bs_out: the block
+   size); 0 on failure.  On success only the block size of the SAP
+   parameters changes: isolv, nmr and ncy keep the values set before (if no
+   SAP parameters were set, they become the given ones).  On failure the
+   SAP parameters are left as they were. */
 double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out);
 /* several right-hand sides (no counterpart in the reference): solves
    D psi[s] = eta[s] for s < n as sap_gcr would (per source: rho_out[s],
@@ -149,7 +155,12 @@ void ranlxd(double *r, int n);
    step (beta 0.9, 0.999) of step size lr on that loss; returns the loss
    before the step.  oqh_set_nn_prec(1): sap_gcr adds NN(residual of the
    Schwarz cycles) to the preconditioner's output and takes chi = Dw(phi);
-   a solve that fails with it is repeated with plain SAP. */
+   plain SAP is the fallback: the learned solve may use nmx/2 Krylov
+   vectors (at least 1), and if it has not converged by then, plain SAP+GCR
+   continues from its solution (from psi = 0 if that is no better than the
+   start) with the remaining nmx - nmx/2 vectors to the same tolerance.  A
+   call therefore never uses more than nmx vectors, status counts both
+   parts, and ifail = -1 only if the continuation fails as well. */
 int oqh_nn_create(int nlayer, const int *widths);
 int oqh_nn_nweights(void);
 void oqh_nn_set_weights(const double *w);
@@ -184,6 +195,13 @@ void oqh_export_s(const spinor *dev, spinor *host, int n);
 void oqh_import_sd(spinor_dble *dev, const spinor_dble *host, int n);
 void oqh_export_sd(const spinor_dble *dev, spinor_dble *host, int n);
 void oqh_sync(void);
+
+/* the operators read rows 0 and 1 of every FP64 link and rebuild row 2 =
+   +-conj(row0 x row1) when every stored row 2 deviates from that by less
+   than 64 ulp (1.4e-14); otherwise they read all nine entries.  Returns 1
+   if the current gauge field is rebuilt, and the largest deviation in *dev
+   (no counterpart in the reference). */
+int oqh_link_rebuild(double *dev);
 
 /* timing of ncall applications of an operator on device fields, measured
    with HIP events on the library stream; op: 0 Dw_dble, 1 Dw, 2 Dwhat_dble,

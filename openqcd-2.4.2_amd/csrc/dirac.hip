@@ -20,6 +20,7 @@
 // Boundary rules (open bc): odd points on the boundary time slices are zeroed
 // in input and output of Dw and contribute no hopping terms to even points
 // (Dw_dble.c:1425-1456); the neighbour table carries that flag in bit 31.
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
 
@@ -847,8 +848,15 @@ __global__ void k_link_classes(const double2 *__restrict__ ud, int h, unsigned c
    atomicMax(dev, (unsigned long long)__double_as_longlong(mx));
 }
 
+// largest deviation of row 2 from +-conj(row0 x row1) that lets the
+// operators rebuild row 2 (64 ulp; flowed fields stay near 1e-14 after 200
+// Euler steps, unflowed random fields near 1e-15): the rebuilt operator then
+// differs from the stored links by at most this much
+constexpr double LINK_REC_TOL = 64.0 * DBL_EPSILON;
+
 // link classes of the current FP64 field (once per field change): the
-// operators rebuild row 2 of every link if all deviations are below 1e-13
+// operators rebuild row 2 of every link if all deviations are below
+// LINK_REC_TOL
 static void ensure_link_classes()
 {
    Lib &L = lib();
@@ -869,7 +877,8 @@ static void ensure_link_classes()
    HIP_OK(hipStreamSynchronize(L.stream));
    double dev;
    memcpy(&dev, &bits, sizeof(dev));
-   L.ud_rec = dev < 1e-13;
+   L.ud_rec = dev < LINK_REC_TOL;
+   L.ud_rec_dev = dev;
    L.rec_version = L.ud_version;
 }
 
@@ -1073,6 +1082,21 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
 }  // namespace oqh
 
 using namespace oqh;
+
+// whether the operators rebuild row 2 of the links of the current gauge
+// field, and the largest deviation of a stored row 2 from the rebuilt one
+extern "C" int oqh_link_rebuild(double *dev)
+{
+   Lib &L = lib();
+   if (!L.ud_set)
+   {
+      fail("oqh_link_rebuild", "gauge field is not set");
+      return 0;
+   }
+   ensure_link_classes();
+   if (dev) *dev = L.ud_rec_dev;
+   return L.ud_rec ? 1 : 0;
+}
 
 extern "C" void Dw_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_DW, mu, s, r, "Dw_dble [Dw_dble.c]"); }
 extern "C" void Dwee_dble(double mu, spinor_dble *s, spinor_dble *r) { apply<double>(K_DEE, mu, s, r, "Dwee_dble [Dw_dble.c]"); }

@@ -94,6 +94,7 @@ struct Lib
    int ud_version = 0; // incremented whenever d_ud changes
    int rec_version = -1;            // ud_version the link classes were made for
    bool ud_rec = false;             // every link equals +-(rows 0,1, conj(row0 x row1))
+   double ud_rec_dev = -1.0;        // largest deviation measured for that test
    unsigned char *d_lsign = nullptr; // per odd point: bit l set if link l is -SU(3)
 
    std::map<uintptr_t, Field> reg;
@@ -141,6 +142,7 @@ struct Res
 };
 Res resolve(const void *p, int prec, int nsites, int slot, bool upload);
 void *device_field(const void *p, int prec, int *off);
+void *device_field_n(const void *p, int prec, int *off, int *avail);
 void finish(const Res &r, int prec, int nsites, bool download);
 
 // geometry and tables

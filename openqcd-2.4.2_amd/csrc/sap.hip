@@ -2310,10 +2310,12 @@ extern "C" void oqh_get_imb(int n, int *imb)
 }
 
 // Chooses the SAP block size with the least time per e-fold of residual
-// reduction: for every candidate, ncy Schwarz cycles (current precision
-// variant) on a reproducible test residual, timed with HIP events after one
-// warm-up cycle.  Sets the block size of the SAP parameters (isolv, nmr, ncy
-// as given) and returns the best time per e-fold in seconds (0 on failure).
+// reduction: for every candidate, ncy Schwarz cycles with the given isolv and
+// nmr (current precision variant) on a reproducible test residual, timed
+// with HIP events after one warm-up cycle.  On success only the block size of
+// the SAP parameters changes (isolv, nmr and ncy keep their values; if no SAP
+// parameters were set, they become the given ones); on failure nothing
+// changes.  Returns the best time per e-fold in seconds (0 on failure).
 extern "C" double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out)
 {
    const char *where = "oqh_sap_autotune";
@@ -2343,6 +2345,7 @@ extern "C" double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int 
    double best = 0.0;
    std::array<int, 4> bbest = cand[0];
    const int keep[4] = {L.sap_bs[0], L.sap_bs[1], L.sap_bs[2], L.sap_bs[3]};
+   const int keep_isolv = L.sap_isolv, keep_nmr = L.sap_nmr, keep_ncy = L.sap_ncy;
    for (const auto &bs : cand)
    {
       memcpy(L.sap_bs, bs.data(), sizeof(L.sap_bs));
@@ -2379,6 +2382,13 @@ extern "C" double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int 
    dev_free(psi);
    dev_free(rho);
    dev_free(d_part);
+   if (keep_ncy != 0 || best == 0.0)
+   {
+      // SAP parameters set before keep isolv, nmr and ncy
+      L.sap_isolv = keep_isolv;
+      L.sap_nmr = keep_nmr;
+      L.sap_ncy = keep_ncy;
+   }
    if (best == 0.0)
    {
       memcpy(L.sap_bs, keep, sizeof(L.sap_bs));

@@ -290,21 +290,55 @@ extern "C" double sap_gcr(int nkv, int nmx, int istop, double res, double mu, sp
       double rho0 = unorm_dble(vt, 3, eta);
       if (rho0 != 0.0)
       {
+         nn_use = nn_ready();
+         const int nwsd = nn_use ? 4 : 2;
          alloc_ws(L.ws_used + 2 * nkv + 1);
-         alloc_wsd(L.wsd_used + 2);
+         alloc_wsd(L.wsd_used + nwsd);
          spinor **ws = reserve_ws(2 * nkv + 1);
-         spinor_dble **wsd = reserve_wsd(2);
+         spinor_dble **wsd = reserve_wsd(nwsd);
          spinor_dble *rsd = wsd[1];
          mud = mu;
          assign_sd2sd(vt, 2, eta, rsd);
          scale_dble(vt, 2, 1.0 / rho0, rsd);
-         nn_use = nn_ready();
-         rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+         // with the learned preconditioner the solve may use half of the
+         // Krylov vectors; if it has not converged then, plain SAP continues
+         // from its solution with the rest, so that a call never uses more
+         // than nmx vectors and status counts both parts
+         const int nmx1 = nn_use ? (nmx > 1 ? nmx / 2 : 1) : nmx;
+         rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx1, istop, res, rsd, psi, status);
          if (nn_use && status[0] < 0)
          {
-            // plain SAP as the fallback of the learned preconditioner
             nn_use = false;
-            rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+            status[0] = -1;
+            if (nmx > nmx1)
+            {
+               // r2 = rsd - D psi, solve D dpsi = r2 to the original tolerance
+               spinor_dble *r2 = wsd[2], *dpsi = wsd[3];
+               Dw_dble(mu, psi, r2);
+               combine_spinor_dble(vt, 2, r2, rsd, -1.0, 1.0);
+               double n0, n2;
+               if (istop)
+               {
+                  n0 = unorm_dble(vt, 3, rsd);
+                  n2 = unorm_dble(vt, 3, r2);
+               }
+               else
+               {
+                  n0 = sqrt(norm_square_dble(vt, 3, rsd).q[0]);
+                  n2 = sqrt(norm_square_dble(vt, 3, r2).q[0]);
+               }
+               if (!(n2 < n0))
+               {
+                  // the learned solve made no progress: plain SAP from psi = 0
+                  set_sd2zero(vt, 2, psi);
+                  assign_sd2sd(vt, 2, rsd, r2);
+                  n2 = n0;
+               }
+               int st2 = 0;
+               rho = fgcr(vt, Dop, Mop, ws, wsd, nkv, nmx - nmx1, istop, res * n0 / n2, r2, dpsi, &st2);
+               mulr_spinor_add_dble(vt, 2, psi, dpsi, 1.0);
+               status[0] = (st2 < 0) ? -1 : nmx1 + st2;
+            }
          }
          nn_use = false;
          scale_dble(vt, 2, rho0, psi);
@@ -377,8 +411,9 @@ extern "C" void sap_gcr_multi(int n, int nkv, int nmx, int istop, double res, do
    }
    const int vt = L.g.VT, V = L.g.V;
    // the joint path: FP32 SAP with the register kernel, 2-norm stopping
-   // criterion, no network; otherwise one solve after the other
-   bool joint = (L.sap_prec == 0 && istop == 0 && !nn_ready());
+   // criterion, no network, two-pass Gram-Schmidt (oqh_set_gcr_orth(1));
+   // otherwise one solve after the other
+   bool joint = (L.sap_prec == 0 && istop == 0 && !nn_ready() && L.gcr_cgs == 1);
    if (joint && !sap_prepare(L.sap_isolv, where, true)) joint = false;
    int bs_vol = L.sap_bs[0] * L.sap_bs[1] * L.sap_bs[2] * L.sap_bs[3];
    if (bs_vol != 256) joint = false;

@@ -306,6 +306,17 @@ void *device_field(const void *p, int prec, int *off)
    return f->base;
 }
 
+// as device_field, with the number of sites available from the offset
+void *device_field_n(const void *p, int prec, int *off, int *avail)
+{
+   int o = 0;
+   const Field *f = p ? lookup(p, &o) : nullptr;
+   if (!f || f->prec != prec) return nullptr;
+   *off = o;
+   *avail = f->n - o;
+   return f->base;
+}
+
 Res resolve(const void *p, int prec, int nsites, int slot, bool upload)
 {
    Lib &L = lib();
@@ -590,6 +601,15 @@ extern "C" void oqh_set_sap_parms(const int *bs, int isolv, int nmr, int ncy)
    L.sap_isolv = isolv;
    L.sap_nmr = nmr;
    L.sap_ncy = ncy;
+}
+
+extern "C" void oqh_get_sap_parms(int *bs, int *isolv, int *nmr, int *ncy)
+{
+   Lib &L = lib();
+   memcpy(bs, L.sap_bs, sizeof(L.sap_bs));
+   *isolv = L.sap_isolv;
+   *nmr = L.sap_nmr;
+   *ncy = L.sap_ncy;
 }
 
 extern "C" void oqh_import_ud(const su3_dble *ud)

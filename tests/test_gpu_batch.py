@@ -34,6 +34,17 @@ def test_autotune_and_solve(n, seed):
         t = lib.oqh_sap_autotune(1, 4, 5, 0, bs.ctypes.data_as(P))
         print("block size", bs, "seconds per e-fold", t)
         assert t > 0.0
+        # SAP parameters set before: only the block size changes
+        got = np.zeros(4, np.int32)
+        iv, nm, nc = C.c_int(), C.c_int(), C.c_int()
+        lib.oqh_get_sap_parms(got.ctypes.data_as(P), C.byref(iv), C.byref(nm), C.byref(nc))
+        assert list(got) == list(bs) and (iv.value, nm.value, nc.value) == (1, 4, 5)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 0, 3, 4)
+        bs2 = np.zeros(4, np.int32)
+        assert lib.oqh_sap_autotune(1, 4, 5, 0, bs2.ctypes.data_as(P)) > 0.0
+        lib.oqh_get_sap_parms(got.ctypes.data_as(P), C.byref(iv), C.byref(nm), C.byref(nc))
+        assert list(got) == list(bs2) and (iv.value, nm.value, nc.value) == (0, 3, 4)
+        lib.oqh_set_sap_parms(bs2.ctypes.data_as(P), 1, 4, 5)
         assert all(legal(n, int(b)) for b in bs) and bs[1] == bs[2] == bs[3]
         assert np.prod(bs) <= 4096
         if n == 12:

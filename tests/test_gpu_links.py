@@ -67,3 +67,36 @@ def test_rebuilt_and_full_links(bc):
         print("su3 again", check(o, lat, eta, 0.0))
     finally:
         lat.close()
+
+
+@pytest.mark.gpu
+def test_rebuild_threshold():
+    """Row 2 is rebuilt only while every stored row 2 is within 64 ulp
+    (1.4e-14) of +-conj(row0 x row1): one link perturbed by half of that
+    keeps the rebuilt path, by twice that switches to nine-entry reads; in
+    both cases Dw_dble and Dwhat_dble agree with the oracle on the stored
+    field to 1e-14 per point."""
+    import ctypes as C
+    o, eta, s32 = make_oracle((8, 8, 8, 8), None, bc=3, m0=-0.3, csw=1.77, seed=5)
+    lat = pyoqh.Lattice((8, 8, 8, 8), None, bc=3, m0=-0.3, csw=1.77)
+    try:
+        lib = lat.lib
+        lib.oqh_link_rebuild.restype = C.c_int
+        lib.oqh_link_rebuild.argtypes = [C.c_void_p]
+        ud = o.ud()
+        thr = 64.0 * np.finfo(np.float64).eps
+        for f, rebuilt in ((0.0, 1), (0.5, 1), (2.0, 0)):
+            ud2 = ud.copy()
+            ud2[77, 12] += f * thr          # row 2, real part of column 0
+            o.set_ud(ud2)
+            lat.import_ud(ud2)
+            dev = C.c_double()
+            got = lib.oqh_link_rebuild(C.byref(dev))
+            rels = check(o, lat, eta, 0.05)
+            print("perturbation", f * thr, "rebuilt", got, "deviation", dev.value, "parity", rels, flush=True)
+            assert got == rebuilt
+            assert max(rels) < 1e-14
+            if f > 0:
+                assert abs(dev.value - f * thr) < 0.25 * thr
+    finally:
+        lat.close()

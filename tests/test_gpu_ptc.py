@@ -189,3 +189,56 @@ def test_training_and_solve():
     finally:
         lat.lib.oqh_nn_free()
         lat.close()
+
+
+@pytest.mark.gpu
+def test_learned_fallback_budget():
+    """Plain SAP as the fallback of the learned preconditioner: with weights
+    that spoil the preconditioner the learned solve exhausts its half of the
+    Krylov budget, plain SAP+GCR continues from its solution, and the call
+    uses at most nmx vectors in total (status counts both parts) and reaches
+    the true residual; with nmx = 1 the call fails with ifail = -1 and
+    status = nmx (sap/sap_gcr.c:223-228)."""
+    lat = pyoqh.Lattice((8, 8, 8, 8), None, bc=3, m0=-0.35, csw=1.77)
+    try:
+        lib = lat.lib
+        rng = np.random.default_rng(9)
+        lat.import_ud(pyoqh.haar_su3(4 * lat.V, rng))
+        lib.fwd_euler(20, 0.02)
+        lib.set_ud_phase()
+        assert lat.sw_term(3) == 0
+        widths = [4, 8, 4]
+        w = np.array(widths, np.int32)
+        assert lib.oqh_nn_create(2, w.ctypes.data_as(P)) == 0
+        lib.oqh_nn_set_weights(weights_flat(ptc.init_weights(widths, rng, scale=30.0)).ctypes.data_as(P))
+        lib.sap_gcr.restype = C.c_double
+        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        eta = rng.standard_normal((lat.V, 24))
+        e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
+        lat.put_sd(e, eta)
+        ifail, status = C.c_int(), C.c_int()
+        lib.oqh_set_nn_prec(0)
+        lib.sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+        plain = status.value
+        assert ifail.value == 0
+        nmx = 2 * plain + 10
+        lib.oqh_set_nn_prec(1)
+        lib.sap_gcr(16, nmx, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+        lib.Dw_dble(0.0, p, r)
+        res = np.linalg.norm(eta - lat.get_sd(r)) / np.linalg.norm(eta)
+        print("plain", plain, "nmx", nmx, "learned+fallback status", status.value, "ifail", ifail.value,
+              "true residual", res, flush=True)
+        assert ifail.value == 0 and res <= 1.2e-10
+        assert nmx // 2 < status.value <= nmx, "the fallback must have run within the budget"
+        lib.sap_gcr(16, 1, 0, 1e-10, 0.0, e, p, C.byref(ifail), C.byref(status))
+        assert ifail.value == -1 and status.value == 1
+        assert np.array_equal(lat.get_sd(e), eta)
+        for f in (e, p, r):
+            lat.free(f)
+        lib.oqh_set_nn_prec(0)
+    finally:
+        lat.lib.oqh_nn_free()
+        lat.close()
