@@ -196,6 +196,21 @@ void oqh_import_sd(spinor_dble *dev, const spinor_dble *host, int n);
 void oqh_export_sd(const spinor_dble *dev, spinor_dble *host, int n);
 void oqh_sync(void);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight loops, harnesses that forbid
+   hipMalloc).  spinor_prod_dble(vol, icom, s, r) of two double-precision
+   device fields of the library (alloc_wsd/reserve_wsd, oqh_alloc_sd; host
+   arrays are refused), written as complex_qflt to the device memory z and
+   enqueued on the library stream without synchronisation (oqh_sync waits).
+   Same kernels and summation order as spinor_prod_dble, so the result is
+   bitwise the same.  The reduction's partial sums use a fixed 32 KiB device
+   array per GPU (at most 1024 work-group partials, enough for every vol);
+   calls that overlap on one GPU share it and race.  Allocates nothing
+   (no hipMalloc, hipFree or pinned host memory).  Returns 0, or 1 after
+   reporting an error (library not set up, current device not the
+   library's, host or too short fields, z not device memory). */
+int oqh_spinor_prod_dble_noalloc(int vol, int icom, spinor_dble *s, spinor_dble *r, complex_qflt *z);
+
 /* the operators read rows 0 and 1 of every FP64 link and rebuild row 2 =
    +-conj(row0 x row1) when every stored row 2 deviates from that by less
    than 64 ulp (1.4e-14); otherwise they read all nine entries.  Returns 1

@@ -1062,3 +1062,102 @@ extern "C" double unorm_dble(int vol, int icom, spinor_dble *sd)
 {
    return sqrt(unorm_any<double2>(sd, F64, vol, icom, "unorm_dble [unorm.c]"));
 }
+
+// ---------------------------------------------------------------------------
+// spinor_prod_dble without allocation (no counterpart in the reference)
+//
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight loops, harnesses that forbid hipMalloc).  The reduction's
+// work-group partials live in the file-scope array scratch (RMAXB work-groups
+// x two double-double numbers = 32 KiB per GPU, held until the process
+// ends).  The host name of the array is not a device address: its address on
+// a device is looked up once per device with hipGetSymbolAddress and only
+// that pointer is used.  Every partial the final kernel reads is written by
+// the same call before, so nothing needs clearing; calls that overlap on one
+// GPU share the array and race.
+// ---------------------------------------------------------------------------
+
+namespace oqh {
+
+constexpr int NOALLOC_MAXDEV = 64;
+static __device__ __align__(256) double scratch[4 * RMAXB];
+static dd *scratch_ptr[NOALLOC_MAXDEV] = {};
+
+// scratch of the library's device, looked up on the first call there
+static dd *noalloc_scratch(const char *where)
+{
+   Lib &L = lib();
+   int cur = -1;
+   if (hipGetDevice(&cur) != hipSuccess || cur != L.device)
+   {
+      fail(where, "the current HIP device must be the library's (oqh_init)");
+      return nullptr;
+   }
+   if (cur < 0 || cur >= NOALLOC_MAXDEV)
+   {
+      fail(where, "device number out of range");
+      return nullptr;
+   }
+   if (!scratch_ptr[cur])
+   {
+      void *p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || p == nullptr)
+      {
+         fail(where, e != hipSuccess ? hipGetErrorString(e) : "no device address of the scratch array");
+         return nullptr;
+      }
+      scratch_ptr[cur] = (dd *)p;
+   }
+   return scratch_ptr[cur];
+}
+
+}  // namespace oqh
+
+extern "C" int oqh_spinor_prod_dble_noalloc(int vol, int icom, spinor_dble *s, spinor_dble *r, complex_qflt *z)
+{
+   const char *where = "oqh_spinor_prod_dble_noalloc";
+   Lib &L = lib();
+   if (!L.inited || L.g.V == 0)
+   {
+      fail(where, "the library has no device or no lattice geometry (oqh_init, oqh_geometry)");
+      return 1;
+   }
+   dd *part = noalloc_scratch(where);
+   if (!part) return 1;
+   const int n = nsites(vol, icom);
+   if (n <= 0)
+   {
+      fail(where, "vol must be positive");
+      return 1;
+   }
+   int os = 0, orr = 0, ns = 0, nr = 0;
+   const double2 *ds = (const double2 *)device_field_n(s, F64, &os, &ns);
+   const double2 *dr = (const double2 *)device_field_n(r, F64, &orr, &nr);
+   if (!ds || !dr)
+   {
+      fail(where, "s and r must be double-precision device fields of the library (alloc_wsd, oqh_alloc_sd)");
+      return 1;
+   }
+   if (n > ns || n > nr)
+   {
+      fail(where, "vol exceeds the points of the fields");
+      return 1;
+   }
+   hipPointerAttribute_t at;
+   if (z == nullptr || hipPointerGetAttributes(&at, z) != hipSuccess || at.type != hipMemoryTypeDevice)
+   {
+      (void)hipGetLastError();
+      fail(where, "z must point to device memory");
+      return 1;
+   }
+   const int nb = red_blocks(n);  // at most RMAXB partials: every vol fits the scratch
+   if (nb > RMAXB)
+   {
+      fail(where, "too many work-group partials for the scratch array");
+      return 1;
+   }
+   k_dot_partial<double2, DOT_C><<<nb, RB, 0, L.stream>>>(ds, dr, os, orr, n, part);
+   k_dd_final<<<1, RB, 0, L.stream>>>(part, nb, (dd *)z);
+   return 0;
+}
