@@ -66,83 +66,16 @@ NN_WIDTHS = [4, 8, 4]
 
 
 def train_network():
-    """Trains the learned preconditioner on a 16^4 lattice with the physics
-    of the solver workload (periodic bc, random_ud of seed 1 flowed 50 x
-    0.01, csw=1.77, m0=-0.35, SAP 4^4 isolv=1 nmr=4 ncy=5): 32 pairs from
-    Gaussian sources r, e = D^-1 r by SAP+GCR to 1e-12; the network input is
-    the residual the 5 Schwarz cycles leave of r, its target the error they
-    leave of e.  28 pairs train (1000 Adam steps, batches of 8, learning
-    rate 1e-2 then 2e-3), 4 are held out.  Returns the weights and the
-    training record."""
-    import ctypes as C
-    import numpy as np
-    import pyoqh
-    P = C.c_void_p
-    lat = pyoqh.Lattice((16, 16, 16, 16), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
-    rec = {"lattice": "16^4", "pairs": 32, "held_out": 4, "adam_steps": 1000, "batch": 8,
-           "widths": NN_WIDTHS}
-    try:
-        lib = lat.lib
-        lib.sap_gcr.restype = C.c_double
-        lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
-                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
-        lib.sap.argtypes = [C.c_float, C.c_int, C.c_int, P, P]
-        rng = np.random.default_rng(1)
-        V = lat.V
-        lib.start_ranlux(0, 1)
-        lib.random_ud()
-        lib.fwd_euler(50, 0.01)
-        lib.set_ud_phase()
-        bs = np.array((4, 4, 4, 4), np.int32)
-        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
-        t0 = time.perf_counter()
-        e_, p_ = lat.alloc_sd(), lat.alloc_sd()
-        psi, rho = lat.alloc_s(), lat.alloc_s()
-        fr, fe = [], []
-        for k in range(32):
-            lib.random_sd(V, e_, 1.0)
-            r = lat.get_sd(e_)
-            ifail, st = C.c_int(), C.c_int()
-            lib.sap_gcr(24, 1024, 0, 1e-12, 0.0, e_, p_, C.byref(ifail), C.byref(st))
-            e = lat.get_sd(p_)
-            lat.put_s(psi, np.zeros((V, 24), np.float32))
-            lat.put_s(rho, r.astype(np.float32))
-            for _ in range(5):
-                lib.sap(0.0, 1, 4, psi, rho)
-            a, b = lat.alloc_s(), lat.alloc_s()
-            lat.put_s(a, lat.get_s(rho))
-            lat.put_s(b, (e - lat.get_s(psi)).astype(np.float32))
-            fr.append(a)
-            fe.append(b)
-        rec["pair_generation_s"] = round(time.perf_counter() - t0, 2)
-        w = np.array(NN_WIDTHS, np.int32)
-        lib.oqh_nn_create(len(NN_WIDTHS) - 1, w.ctypes.data_as(P))
-        # small random complex initial weights [9][C_out][C_in] per layer
-        nw = lib.oqh_nn_nweights()
-        w0 = 0.01 * rng.standard_normal(2 * nw)
-        lib.oqh_nn_set_weights(w0.ctypes.data_as(P))
-
-        def ptrs(f):
-            return (C.c_void_p * len(f))(*f)
-        dw = np.empty(2 * lib.oqh_nn_nweights())
-        hold = (ptrs(fr[28:]), ptrs(fe[28:]))
-        rec["held_out_loss_before"] = round(lib.oqh_nn_grad(4, *hold, dw.ctypes.data_as(P)), 4)
-        lat.sync()
-        t0 = time.perf_counter()
-        for it in range(1000):
-            idx = rng.choice(28, 8, replace=False)
-            lib.oqh_nn_train_step(8, ptrs([fr[i] for i in idx]), ptrs([fe[i] for i in idx]),
-                                  1e-2 if it < 500 else 2e-3)
-        lat.sync()
-        rec["training_s"] = round(time.perf_counter() - t0, 2)
-        rec["held_out_loss_after"] = round(lib.oqh_nn_grad(4, *hold, dw.ctypes.data_as(P)), 4)
-        wts = np.empty(2 * lib.oqh_nn_nweights())
-        lib.oqh_nn_get_weights(wts.ctypes.data_as(P))
-        log("bench: network trained, held-out loss %.3f -> %.3f" % (rec["held_out_loss_before"],
-                                                                   rec["held_out_loss_after"]))
-    finally:
-        lat.close()
-    return wts, rec
+    """Trains the learned preconditioner (nntrain.train_preconditioner) on a
+    16^4 lattice with the physics of the solver workload (periodic bc,
+    random_ud of seed 1 flowed 50 x 0.01, csw=1.77, m0=-0.35, SAP 4^4
+    isolv=1 nmr=4 ncy=5): 32 pairs from Gaussian sources r, e = D^-1 r by
+    SAP+GCR to 1e-12; the network input is the residual the 5 Schwarz cycles
+    leave of r, its target the error they leave of e.  28 pairs train (1000
+    Adam steps, batches of 8, learning rate 1e-2 then 2e-3), 4 are held out.
+    Returns the weights and the training record."""
+    import nntrain
+    return nntrain.train_preconditioner(NN_WIDTHS, log=lambda m: log("bench: " + m))
 
 
 def solve_times(args):
@@ -184,7 +117,7 @@ def solve_times(args):
         bs = np.array((4, 4, 4, 4), np.int32)
         lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
         e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
-        lib.random_sd(V, e, 1.0)
+        lib.random_sd(lat.VT, 2, e, 1.0)
         # the operators on this lattice (HIP events, 50 calls each)
         assert lat.sw_term(3) == 0
         ms = lib.oqh_time_dirac(0, 0.0, e, r, 50)
@@ -271,7 +204,7 @@ def learned_held_out(wts):
             lib.random_ud()
             lib.fwd_euler(50, 0.01)
             lib.set_ud_phase()
-            lib.random_sd(lat.V, e, 1.0)
+            lib.random_sd(lat.VT, 2, e, 1.0)
             rec = {"seed": seed}
             for name, prec, nn in (("plain", 0, 0), ("fp16", 1, 0), ("fp16_learned", 1, 1)):
                 lib.oqh_set_sap_prec(prec)
@@ -328,7 +261,7 @@ def multi_rhs():
             e[0, 2 * k] = 1.0
             lat.put_sd(es[k], e)
         for k in range(12, 16):
-            lib.random_sd(V, es[k], 1.0)
+            lib.random_sd(lat.VT, 2, es[k], 1.0)
         rho = np.zeros(n)
         ifail = np.zeros(n, np.int32)
         status = np.zeros(n, np.int32)
@@ -389,7 +322,7 @@ def uneven_batch():
             lib.fwd_euler(50, 0.01)
             lib.set_ud_phase()
             e, p = lat.alloc_sd(), lat.alloc_sd()
-            lib.random_sd(lat.V, e, 1.0)
+            lib.random_sd(lat.VT, 2, e, 1.0)
             lat.sync()
             t0 = time.perf_counter()
             bs = np.zeros(4, np.int32)
@@ -410,9 +343,61 @@ def uneven_batch():
     return rec
 
 
+def host_info():
+    """Cores of the host and of this job, and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity_cores": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def cpu_solves():
+    """SAP+GCR of the reference build on the host (FP32 SAP, FP64 GCR, SAP 4^4
+    isolv=1 nmr=4 ncy=5): the 8^4 configuration (periodic, csw=1.0,
+    m0=-0.5, random_ud of seed 1; NTHREAD=8, the most the block rules allow
+    with 16 blocks) solved to 1e-10, and a bounded sample of the 48^4 solver
+    workload (periodic, csw=1.77, m0=-0.35, random_ud of seed 3 without the
+    flow, nkv=24; NTHREAD=16): 4 Krylov vectors, whose time per vector
+    times the Krylov count of a solve gives its CPU time."""
+    out = {}
+    runs = (("8x8x8x8_avx", "config1_8^4", ["-bc", "3", "-csw", "1.0", "-m0", "-0.5", "-seed", "1",
+                                              "-nkv", "16", "-nmx", "256"], "8"),
+            ("48x48x48x48_avx", "solver_48^4_sample", ["-bc", "3", "-csw", "1.77", "-m0", "-0.35", "-seed",
+                                                       "3", "-nkv", "24", "-nmx", "4"], "16"))
+    for tag, name, extra, nt in runs:
+        exe = os.path.join(ROOT, "oracle", "_ref", "ref_" + tag)
+        if not os.path.exists(exe):
+            continue
+        env = dict(os.environ, OMP_NUM_THREADS=nt)
+        try:
+            r = subprocess.run([exe, "gcr", "-isolv", "1", "-bs", "4", "4", "4", "4", "-nmr", "4", "-ncy", "5",
+                                "-res", "1e-10"] + extra, capture_output=True, text=True, timeout=600,
+                               env=env, cwd="/tmp")
+            j = json.loads(r.stdout.strip().splitlines()[-1])
+        except Exception as e:  # pragma: no cover
+            log("cpu solve %s failed: %s" % (tag, e))
+            continue
+        j["sec_per_krylov_vector"] = j["gcr_sec"] / max(j["gcr_status"], 1)
+        out[name] = j
+        log("bench: reference CPU SAP+GCR %s: status %d in %.3f s (%.3f s per Krylov vector)" %
+            (name, j["gcr_status"], j["gcr_sec"], j["sec_per_krylov_vector"]))
+    return out
+
+
 def cpu_baseline(V):
     """Reference CPU Dw_dble (oracle/_ref/ref_64x32x32x32_avx: the
-    reference's modules built with -DAVX -DFMA3, OpenMP, 16 threads)."""
+    reference's modules built with -DAVX -DFMA3, OpenMP, NTHREAD=16, the CPU
+    share of the job) and the reference's SAP+GCR on the host."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_64x32x32x32_avx")
     if not os.path.exists(exe):
         return None
@@ -427,7 +412,8 @@ def cpu_baseline(V):
         return None
     sec = j["Dw_dble_sec"] / j["Dw_dble_calls"]
     return {"value": round(BYTES["Dw_dble"] * V / sec / 1e9, 2), "unit": "GB/s",
-            "cores": j["nthread"], "kind": "reference",
+            "cores": j["nthread"], "kind": "reference", "host": host_info(),
+            "sap_gcr": cpu_solves(),
             "sample": "Dw_dble of the reference (AVX/FMA3, OpenMP NTHREAD=%d) on its own "
                       "random_ud field, 32^3x64 open bc, %d calls in %.1f s; Dw %.1f GB/s, "
                       "Dwhat_dble %.1f GB/s" % (
@@ -470,7 +456,7 @@ def main():
     lat.lib.fwd_euler(100, 0.01)
     assert lat.sw_term(3) == 0
     s, r = lat.alloc_sd(), lat.alloc_sd()
-    lat.lib.random_sd(V, s, 1.0)
+    lat.lib.random_sd(lat.VT, 2, s, 1.0)
     lib = lat.lib
 
     # warmup, then K timed steps bracketed by device synchronisation

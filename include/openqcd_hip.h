@@ -56,6 +56,12 @@ typedef enum { ALL_PTS, EVEN_PTS, ODD_PTS, NO_PTS, PT_SETS } ptset_t;
 /* include/block.h:48-52 */
 typedef enum { SAP_BLOCKS, DFL_BLOCKS, TEST_BLOCKS, BLK_GRIDS } blk_grid_t;
 #endif
+#ifndef FLAGS_H
+/* include/flags.h:102-118 */
+typedef struct { int bs[4]; int Ns; } dfl_parms_t;
+typedef struct { int nkv, nmx, nmx_gcr; double res, res_gcr; } dfl_pro_parms_t;
+typedef struct { int ninv, nmr, ncy; double kappa, m0, mu; } dfl_gen_parms_t;
+#endif
 
 /* ======================= A. library state (no reference counterpart
    as functions; they replace include/global.h and modules/flags) ====== */
@@ -114,6 +120,16 @@ double oqh_sap_autotune(int isolv, int nmr, int ncy, int maxvol, int *bs_out);
    launch) run in the same kernel launches, consecutive work-groups taking
    the same block for different sources; otherwise the sources are solved one
    after the other.  Each source's result equals that of sap_gcr. */
+/* the propagator loop of main/ms4.c (propagator, ms4.c:496-531, with
+   random_source 383-406, solve_dirac's SAP_GCR branch 409-459 and
+   save_prop's export_sfld branch 470-481) for the current gauge field:
+   nsrc random sources on time slice x0, solved by SAP+GCR with the current
+   SAP parameters (up to 16 together, sap_gcr_multi), written as
+   "<dir>/<nbase>n<icnfg>.s<isrc>".  status[isrc]: Krylov vectors of source
+   isrc.  Returns the number of failed solves (their files are not
+   written), -1 for bad arguments. */
+int oqh_ms4_propagators(int x0, int nsrc, int nkv, int nmx, int istop, double res, double mu,
+                        const char *dir, const char *nbase, int icnfg, int *status);
 void sap_gcr_multi(int n, int nkv, int nmx, int istop, double res, double mu, spinor_dble **eta,
                    spinor_dble **psi, double *rho_out, int *ifail, int *status);
 /* orthogonalisation of the Krylov vectors in fgcr on device fields (no
@@ -127,13 +143,24 @@ void oqh_set_gcr_orth(int mode);
 /* random numbers and random fields on the device: the streams of
    random/ranlux.c:136-171 (ranlxd, one per reference thread of the
    geometry's thread layout), random_ud (uflds/uflds.c:159-222, uflds.h),
-   random_sd (sflds/sflds.c:1161-1200, sflds.h; vol a multiple of the number
-   of threads, each thread filling vol/NTHREAD points), ranlxd
+   random_sd (sflds/sflds.c:1254-1268, sflds.h: with icom & 2 every
+   reference thread k fills vol points from sd + k vol, e.g.
+   random_sd(VOLUME_TRD, 2, sd, sigma) for a full field; otherwise the
+   calling thread, stream 0, fills vol points from sd), ranlxd
    (random/ranlxd.c; the master thread's stream, n numbers into the host
    array r).  The fields are the reference's for the same seed and layout. */
+/* lattice/bcnds.c:750, 709: zero a field on the points of the set at the
+   boundary time slices of the boundary conditions (time 0 and N0-1 for open
+   bc; nothing for periodic bc) */
+void bnd_sd2zero(ptset_t set, spinor_dble *sd);
+void bnd_s2zero(ptset_t set, spinor *s);
 void start_ranlux(int level, int seed);
 void random_ud(void);
-void random_sd(int vol, spinor_dble *sd, double sigma);
+void random_sd(int vol, int icom, spinor_dble *sd, double sigma);
+/* the random source of main/ms4.c:383-406 (random_source): Gaussian spinors
+   (random_sd(1,0,..) by each reference thread in turn over its VOLUME_TRD
+   ipt indices) on the points of time slice t0, zero elsewhere; full field */
+void oqh_random_source(int t0, spinor_dble *sd);
 void ranlxd(double *r, int n);
 
 /* learned preconditioner (no counterpart in the reference): a linear
@@ -410,6 +437,58 @@ double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
             void (*Mop)(int k, spinor *rho, spinor *phi, spinor *chi),
             spinor **ws, spinor_dble **wsd, int nkv, int nmx, int istop,
             double res, spinor_dble *eta, spinor_dble *psi, int *status);
+
+/* Deflation (Luescher's local deflation; include/flags.h:250-260,
+   include/dfl.h:34-55, include/little.h:48-82).  Parameters as
+   flags/dfl_parms.c:209, 277, 351 (set once; the DFL block grid follows the
+   rules of check_block_size).  dfl_subspace (dfl/dfl_subspace.c): block
+   Gram-Schmidt of the Ns modes into the basis; dfl_s2vd / dfl_vd2s: a spinor
+   field's components along the subspace and back (little vectors: Ns
+   complex numbers per DFL block, even blocks first, host arrays of
+   nb * Ns complex_dble); dfl_renormalize_modes: Gram-Schmidt of the global
+   modes.  set_Aw / set_Awhat (little/Aw_ops.c): the little Dirac operator
+   and its even-odd form (set_Awhat returns 1 if an inversion of a diagonal
+   block was not safe, Frobenius condition >= 1e6); Awhat_dble: the even-odd
+   operator on host arrays of nb/2 * Ns entries.  ltl_gcr (dfl/ltl_gcr.c):
+   the little Dirac equation, FGCR in double precision preconditioned by
+   single-pass GCR in single precision (without the little-mode deflation of
+   little/ltl_modes.c).  dfl_modes / dfl_modes2 (dfl/dfl_modes.c): the
+   subspace by inverse iteration from random fields.  dfl_sap_gcr /
+   dfl_sap_gcr2 (dfl/dfl_sap_gcr.c): the deflated SAP+GCR solver, status[0]
+   the Krylov count, status[1], status[2] the average outer and inner counts
+   of the little solver (dfl_sap_gcr2: ifail[2], status[6]).  With
+   oqh_set_sap_prec(1) the Schwarz cycles of the preconditioner run on FP16
+   fields. */
+dfl_parms_t set_dfl_parms(int *bs, int Ns);
+dfl_parms_t dfl_parms(void);
+dfl_pro_parms_t set_dfl_pro_parms(int nkv, int nmx, double res, int nmx_gcr, double res_gcr);
+dfl_pro_parms_t dfl_pro_parms(void);
+dfl_gen_parms_t set_dfl_gen_parms(double kappa, double mu, int ninv, int nmr, int ncy);
+dfl_gen_parms_t dfl_gen_parms(void);
+void dfl_subspace(spinor **mds);
+void dfl_s2vd(spinor *s, complex_dble *vd);
+void dfl_vd2s(complex_dble *vd, spinor *s);
+void dfl_renormalize_modes(spinor **mds);
+void set_Aw(double mu);
+int set_Awhat(double mu);
+void Awhat_dble(complex_dble *v, complex_dble *w);
+double ltl_gcr(double mu, complex_dble *eta, complex_dble *psi, int *ifail, int *status);
+void dfl_modes(int *ifail, int *status);
+void dfl_modes2(int *ifail, int *status);
+double dfl_sap_gcr(int nkv, int nmx, int istop, double res, double mu,
+                   spinor_dble *eta, spinor_dble *psi, int *ifail, int *status);
+double dfl_sap_gcr2(int nkv, int nmx, int istop, double res, double mu,
+                    spinor_dble *eta, spinor_dble *psi, int *ifail, int *status);
+/* test access (no reference counterpart): basis vector k as a host array,
+   the block structure (inn[nb][8] with ifc = 2 mu for -mu and 2 mu + 1 for
+   +mu, the points of every block bpts[nb][bvol]; returns nb), the little
+   operator (Ablk[nb][Ns][Ns], Ahop[nb][8][Ns][Ns], complex, row-major) and
+   the release of the deflation state with its parameters (also done by
+   oqh_finalize) */
+void oqh_dfl_export_basis(int k, spinor *host);
+int oqh_dfl_blocks(int *inn, int *bpts);
+void oqh_dfl_export_Aw(complex_dble *ablk, complex_dble *ahop);
+void oqh_dfl_release(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,1652 @@
+// dfl.hip -- deflated SAP+GCR (local deflation of the low quark modes) on
+// gfx950.
+//
+// Replaces flags/dfl_parms.c (set_dfl_parms, set_dfl_pro_parms,
+// set_dfl_gen_parms), dfl/dfl_geometry.c (the DFL block grid),
+// dfl/dfl_subspace.c (dfl_subspace, dfl_s2vd, dfl_vd2s,
+// dfl_renormalize_modes), little/Aw_gen.c, Aw_ops.c, Aw_dble.c (the little
+// Dirac operator and its even-odd preconditioned form), dfl/ltl_gcr.c with
+// linsolv/fgcr4vd.c and gcr4v.c (the solver of the little Dirac equation),
+// dfl/dfl_modes.c (generation of the subspace by inverse iteration) and
+// dfl/dfl_sap_gcr.c (the deflated solver).
+//
+// Layout.  The DFL blocks partition the lattice, so the Ns orthonormal basis
+// vectors v_{n,k} of all blocks n are kept as Ns global single-precision
+// spinor fields B_k (B_k = v_{n,k} on block n), in the library's site-block
+// layout: the projections and the prolongation then stream whole fields, and
+// the Dirac kernels act on basis vectors directly.  Blocks are numbered even
+// blocks first (block parity = parity of the block coordinates), each colour
+// lexicographically; a little vector holds Ns complex numbers per block in
+// that order.  The little operator is kept per block as Ablk = (v_{n,k},
+// (Dw + i mu gamma5 I_eo) v_{n,l}) and Ahop[ifc] = (v_{n,k}, Dw v_{m,l}) for
+// the 8 neighbouring blocks m = inn[n][ifc] (ifc = 2 mu: -mu, 2 mu + 1: +mu),
+// computed in double precision from the FP32 basis with FP64 links and SW
+// term.  The even-odd form stores Ablk^-1 and Ablk^-1 Ahop[ifc], transposed
+// ([k] fastest) so that the 64 lanes of a matrix-vector product read rows
+// as whole lines; an FP32 copy serves the inner single-precision solver.
+//
+// Differences to the reference (documented where they matter): the little
+// solver is FGCR in double precision preconditioned by single-pass GCR in
+// single precision, as ltl_gcr.c, but without the additional deflation of the
+// little operator by the global modes (little/ltl_modes.c); the random start
+// fields of the subspace generation come from the double-precision RANLUX
+// streams (uniform in [-1/2, 1/2) like ranlxs minus 1/2) rather than from the
+// single-precision generator.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dirac_dev.h"
+#include "oqh_internal.h"
+
+namespace oqh {
+bool sap_prepare(int isolv, const char *where, bool quiet);
+void sap_device(float mu, int isolv, int nmr, float2 *psi, float2 *rho);
+bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, float2 *phi, bool want_res);
+}
+
+using namespace oqh;
+
+namespace {
+
+constexpr int DWG = 256;     // work-group of the block kernels
+constexpr int NS_MAX = 64;   // largest Ns (one lane per basis vector)
+
+struct Dfl
+{
+   // parameters (flags/dfl_parms.c)
+   int bs[4] = {0, 0, 0, 0}, Ns = 0;
+   int nkv = 0, nmx = 0, nmx_gcr = 0;
+   double res = 0.0, res_gcr = 0.0;
+   int ninv = 0, nmr = 0, ncy = 0;
+   double kappa = 0.0, m0 = 0.0, mu = 0.0;
+   // grid
+   bool grid = false;
+   int nb = 0, nbh = 0, bvol = 0;
+   int *d_bpts = nullptr;    // [nb][bvol] points of each block
+   int *d_inn = nullptr;     // [nb][8] neighbouring blocks
+   std::vector<int> inn;     // host copy
+   std::vector<int> bpts;    // host copy
+   // subspace
+   float2 *d_basis = nullptr;  // Ns fields of units_for(V,12)
+   size_t fu = 0;              // units per field
+   bool basis_ok = false;
+   float2 *d_vmds = nullptr;   // [Ns][nb][Ns] coordinates of the generating modes
+   double2 *d_s64 = nullptr;   // FP64 work field of the block Gram-Schmidt
+   // little operator
+   double2 *d_Ablk = nullptr;  // [nb][Ns][Ns]  (row k, column l)
+   double2 *d_Ahop = nullptr;  // [nb][8][Ns][Ns]
+   double2 *d_AinvT = nullptr; // [nb][Ns][Ns]  Ablk^-1 transposed ([l][k])
+   double2 *d_MT = nullptr;    // [nb][8][Ns][Ns] (Ablk^-1 Ahop)^T
+   float2 *d_MT32 = nullptr;   // its FP32 copy
+   int *d_unsafe = nullptr;
+   bool aw_ok = false, awhat_ok = false;
+   double aw_mu = 0.0;
+   int aw_ud = -1, aw_swd = -1, aw_eo = -1;
+   // little vectors (device): work space of the little solver
+   std::vector<double2 *> wvd;
+   std::vector<float2 *> wv;
+   double *d_red = nullptr, *h_red = nullptr;  // reductions
+};
+
+Dfl D;
+
+void dfl_free_grid()
+{
+   dev_free(D.d_bpts); dev_free(D.d_inn); dev_free(D.d_basis); dev_free(D.d_vmds); dev_free(D.d_s64);
+   dev_free(D.d_Ablk); dev_free(D.d_Ahop); dev_free(D.d_AinvT); dev_free(D.d_MT); dev_free(D.d_MT32);
+   dev_free(D.d_unsafe); dev_free(D.d_red);
+   for (auto p : D.wvd) dev_free(p);
+   for (auto p : D.wv) dev_free(p);
+   if (D.h_red) (void)hipHostFree(D.h_red);
+   D.d_bpts = D.d_inn = nullptr;
+   D.d_basis = nullptr; D.d_vmds = nullptr; D.d_s64 = nullptr;
+   D.d_Ablk = D.d_Ahop = D.d_AinvT = D.d_MT = nullptr;
+   D.d_MT32 = nullptr; D.d_unsafe = nullptr; D.d_red = nullptr; D.h_red = nullptr;
+   D.wvd.clear(); D.wv.clear();
+   D.grid = D.basis_ok = D.aw_ok = D.awhat_ok = false;
+}
+
+// ---------------------------------------------------------------------------
+// the DFL block grid (dfl/dfl_geometry.c, block/blk_grid.c)
+// ---------------------------------------------------------------------------
+
+bool dfl_grid(const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return false;
+   if (D.Ns == 0)
+   {
+      fail(where, "Deflation subspace parameters are not set");
+      return false;
+   }
+   if (D.grid) return true;
+   const int *bs = D.bs;
+   int nbl[4];
+   for (int mu = 0; mu < 4; mu++) nbl[mu] = L.g.L[mu] / bs[mu];
+   D.nb = nbl[0] * nbl[1] * nbl[2] * nbl[3];
+   D.nbh = D.nb / 2;
+   D.bvol = bs[0] * bs[1] * bs[2] * bs[3];
+   // block numbers: even blocks first, lexicographic within a colour
+   std::vector<int> bnum(D.nb);
+   int ib = 0, m[4];
+   for (int col = 0; col < 2; col++)
+      for (m[0] = 0; m[0] < nbl[0]; m[0]++)
+      for (m[1] = 0; m[1] < nbl[1]; m[1]++)
+      for (m[2] = 0; m[2] < nbl[2]; m[2]++)
+      for (m[3] = 0; m[3] < nbl[3]; m[3]++)
+      {
+         if (((m[0] + m[1] + m[2] + m[3]) & 1) != col) continue;
+         bnum[m[3] + nbl[3] * (m[2] + nbl[2] * (m[1] + nbl[1] * m[0]))] = ib++;
+      }
+   D.inn.assign(8 * (size_t)D.nb, 0);
+   D.bpts.assign((size_t)D.nb * D.bvol, 0);
+   std::vector<int> fill(D.nb, 0);
+   for (m[0] = 0; m[0] < nbl[0]; m[0]++)
+   for (m[1] = 0; m[1] < nbl[1]; m[1]++)
+   for (m[2] = 0; m[2] < nbl[2]; m[2]++)
+   for (m[3] = 0; m[3] < nbl[3]; m[3]++)
+   {
+      const int n = bnum[m[3] + nbl[3] * (m[2] + nbl[2] * (m[1] + nbl[1] * m[0]))];
+      for (int mu = 0; mu < 4; mu++)
+      {
+         int q[4] = {m[0], m[1], m[2], m[3]};
+         q[mu] = (m[mu] + nbl[mu] - 1) % nbl[mu];
+         D.inn[8 * (size_t)n + 2 * mu] = bnum[q[3] + nbl[3] * (q[2] + nbl[2] * (q[1] + nbl[1] * q[0]))];
+         q[mu] = (m[mu] + 1) % nbl[mu];
+         D.inn[8 * (size_t)n + 2 * mu + 1] = bnum[q[3] + nbl[3] * (q[2] + nbl[2] * (q[1] + nbl[1] * q[0]))];
+      }
+   }
+   int y[4];
+   for (y[0] = 0; y[0] < L.g.L[0]; y[0]++)
+   for (y[1] = 0; y[1] < L.g.L[1]; y[1]++)
+   for (y[2] = 0; y[2] < L.g.L[2]; y[2]++)
+   for (y[3] = 0; y[3] < L.g.L[3]; y[3]++)
+   {
+      int q[4];
+      for (int mu = 0; mu < 4; mu++) q[mu] = y[mu] / bs[mu];
+      const int n = bnum[q[3] + nbl[3] * (q[2] + nbl[2] * (q[1] + nbl[1] * q[0]))];
+      const int ix = L.g.ipt[y[3] + L.g.L[3] * (y[2] + L.g.L[2] * (y[1] + L.g.L[1] * y[0]))];
+      D.bpts[(size_t)n * D.bvol + fill[n]++] = ix;
+   }
+   D.d_bpts = (int *)dev_alloc(sizeof(int) * D.bpts.size());
+   D.d_inn = (int *)dev_alloc(sizeof(int) * D.inn.size());
+   D.fu = units_for(L.g.V, 12);
+   D.d_basis = (float2 *)dev_alloc(sizeof(float2) * D.fu * D.Ns);
+   D.d_vmds = (float2 *)dev_alloc(sizeof(float2) * (size_t)D.Ns * D.nb * D.Ns);
+   D.d_s64 = (double2 *)dev_alloc(sizeof(double2) * D.fu);
+   const size_t nm = (size_t)D.Ns * D.Ns;
+   D.d_Ablk = (double2 *)dev_alloc(sizeof(double2) * nm * D.nb);
+   D.d_Ahop = (double2 *)dev_alloc(sizeof(double2) * nm * 8 * D.nb);
+   D.d_AinvT = (double2 *)dev_alloc(sizeof(double2) * nm * D.nb);
+   D.d_MT = (double2 *)dev_alloc(sizeof(double2) * nm * 8 * D.nb);
+   D.d_MT32 = (float2 *)dev_alloc(sizeof(float2) * nm * 8 * D.nb);
+   D.d_unsafe = (int *)dev_alloc(sizeof(int));
+   D.d_red = (double *)dev_alloc(sizeof(double) * 4 * 1024);
+   HIP_OK(hipHostMalloc((void **)&D.h_red, sizeof(double) * 4 * 1024, 0));
+   if (!D.d_bpts || !D.d_inn || !D.d_basis || !D.d_vmds || !D.d_s64 || !D.d_Ablk || !D.d_Ahop || !D.d_AinvT ||
+       !D.d_MT || !D.d_MT32 || !D.d_unsafe || !D.d_red)
+   {
+      dfl_free_grid();
+      return false;
+   }
+   HIP_OK(hipMemcpy(D.d_bpts, D.bpts.data(), sizeof(int) * D.bpts.size(), hipMemcpyHostToDevice));
+   HIP_OK(hipMemcpy(D.d_inn, D.inn.data(), sizeof(int) * D.inn.size(), hipMemcpyHostToDevice));
+   HIP_OK(hipMemset(D.d_basis, 0, sizeof(float2) * D.fu * D.Ns));
+   D.grid = true;
+   return true;
+}
+
+// ---------------------------------------------------------------------------
+// block reductions
+// ---------------------------------------------------------------------------
+
+// sum of a double2 over the work-group (DWG lanes), result in every lane
+__device__ inline double2 wg_sum(double2 v, double2 *sh)
+{
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      v.x += __shfl_xor(v.x, m);
+      v.y += __shfl_xor(v.y, m);
+   }
+   const int w = threadIdx.x >> 6;
+   __syncthreads();
+   if ((threadIdx.x & 63) == 0) sh[w] = v;
+   __syncthreads();
+   double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+   for (int k = 0; k < DWG / 64; k++)
+   {
+      s.x += sh[k].x;
+      s.y += sh[k].y;
+   }
+   return s;
+}
+
+// (a, b) of two spinors, a FP32 promoted, b of type C
+template <typename C> __device__ inline double2 sdot(const float2 *a, const C *b)
+{
+   double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+   for (int c = 0; c < 12; c++)
+   {
+      const double ax = a[c].x, ay = a[c].y, bx = b[c].x, by = b[c].y;
+      z.x += ax * bx + ay * by;
+      z.y += ax * by - ay * bx;
+   }
+   return z;
+}
+
+// w[n][k] = (B_k, F) on block n (dfl_s2vd: the components of F along the
+// deflation subspace); one work-group per block
+template <typename C>
+__global__ __launch_bounds__(DWG) void k_blk_proj(const float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
+                                                   const C *F, double2 *w)
+{
+   __shared__ double2 sh[DWG / 64];
+   const int n = blockIdx.x;
+   const int *pts = bpts + (size_t)n * bvol;
+   for (int k = 0; k < Ns; k++)
+   {
+      double2 z = make_double2(0.0, 0.0);
+      for (int i = threadIdx.x; i < bvol; i += DWG)
+      {
+         const int x = pts[i];
+         float2 v[12];
+         C f[12];
+         load_spinor(basis + (size_t)k * fu, x, v);
+         load_spinor(F, x, f);
+         const double2 d = sdot(v, f);
+         z.x += d.x;
+         z.y += d.y;
+      }
+      z = wg_sum(z, sh);
+      if (threadIdx.x == 0) w[(size_t)n * Ns + k] = z;
+   }
+}
+
+// s(x) = sum_k (float) w[n(x)][k] B_k(x) (dfl_vd2s: single-precision
+// arithmetic as mulc_spinor_add)
+__global__ __launch_bounds__(DWG) void k_blk_prolong(const float2 *basis, size_t fu, int Ns, const int *bpts,
+                                                      int bvol, const double2 *w, float2 *s)
+{
+   const int n = blockIdx.x;
+   const int *pts = bpts + (size_t)n * bvol;
+   for (int i = threadIdx.x; i < bvol; i += DWG)
+   {
+      const int x = pts[i];
+      float2 acc[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) acc[c] = make_float2(0.0f, 0.0f);
+      for (int k = 0; k < Ns; k++)
+      {
+         const double2 wd = w[(size_t)n * Ns + k];
+         const float2 z = make_float2((float)wd.x, (float)wd.y);
+         float2 v[12];
+         load_spinor(basis + (size_t)k * fu, x, v);
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+            acc[c] = make_float2(acc[c].x + (z.x * v[c].x - z.y * v[c].y), acc[c].y + (z.x * v[c].y + z.y * v[c].x));
+      }
+      store_spinor(s, x, acc);
+   }
+}
+
+// block Gram-Schmidt of mode l (dfl_subspace): S = mds_l in double, S -=
+// (B_j, S) B_j for j < l (modified Gram-Schmidt), B_l = S / |S|; the
+// coordinates of mds_l in the basis go to vm[n][0..Ns) (complex float)
+__global__ __launch_bounds__(DWG) void k_blk_gs(float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
+                                                 const float2 *mds, double2 *S, int l, float2 *vmds, int *degen)
+{
+   __shared__ double2 sh[DWG / 64];
+   const int n = blockIdx.x;
+   const int *pts = bpts + (size_t)n * bvol;
+   float2 *vm = vmds + ((size_t)l * gridDim.x + n) * Ns;
+   for (int i = threadIdx.x; i < bvol; i += DWG)
+   {
+      const int x = pts[i];
+      float2 v[12];
+      double2 d[12];
+      load_spinor(mds, x, v);
+#pragma unroll
+      for (int c = 0; c < 12; c++) d[c] = make_double2(v[c].x, v[c].y);
+      store_spinor(S, x, d);
+   }
+   __syncthreads();
+   for (int j = 0; j < l; j++)
+   {
+      const float2 *Bj = basis + (size_t)j * fu;
+      double2 z = make_double2(0.0, 0.0);
+      for (int i = threadIdx.x; i < bvol; i += DWG)
+      {
+         const int x = pts[i];
+         float2 v[12];
+         double2 d[12];
+         load_spinor(Bj, x, v);
+         load_spinor((const double2 *)S, x, d);
+         const double2 p = sdot(v, d);
+         z.x += p.x;
+         z.y += p.y;
+      }
+      z = wg_sum(z, sh);
+      if (threadIdx.x == 0) vm[j] = make_float2((float)z.x, (float)z.y);
+      for (int i = threadIdx.x; i < bvol; i += DWG)
+      {
+         const int x = pts[i];
+         float2 v[12];
+         double2 d[12];
+         load_spinor(Bj, x, v);
+         load_spinor((const double2 *)S, x, d);
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+            d[c] = make_double2(d[c].x - (z.x * v[c].x - z.y * v[c].y), d[c].y - (z.x * v[c].y + z.y * v[c].x));
+         store_spinor(S, x, d);
+      }
+      __syncthreads();
+   }
+   double2 r2 = make_double2(0.0, 0.0);
+   for (int i = threadIdx.x; i < bvol; i += DWG)
+   {
+      double2 d[12];
+      load_spinor((const double2 *)S, pts[i], d);
+#pragma unroll
+      for (int c = 0; c < 12; c++) r2.x += d[c].x * d[c].x + d[c].y * d[c].y;
+   }
+   r2 = wg_sum(r2, sh);
+   const double r = sqrt(r2.x);
+   if (threadIdx.x == 0)
+   {
+      vm[l] = make_float2((float)r, 0.0f);
+      for (int j = l + 1; j < Ns; j++) vm[j] = make_float2(0.0f, 0.0f);
+      if ((float)r == 0.0f) atomicOr(degen, 1);
+   }
+   const double f = r > 0.0 ? 1.0 / r : 0.0;
+   for (int i = threadIdx.x; i < bvol; i += DWG)
+   {
+      const int x = pts[i];
+      double2 d[12];
+      float2 v[12];
+      load_spinor((const double2 *)S, x, d);
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = make_float2((float)(d[c].x * f), (float)(d[c].y * f));
+      store_spinor(basis + (size_t)l * fu, x, v);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// the little Dirac operator (little/Aw_gen.c, Aw_ops.c)
+// ---------------------------------------------------------------------------
+
+struct AwArgs
+{
+   const float2 *basis;
+   size_t fu;
+   int Ns, bvol, V, h, masked, eoflg;
+   const int *bpts, *nbr, *blk_of_unused;
+   const unsigned char *bnd;
+   const double2 *ud, *swd;
+   double mu;
+   double2 *Ablk, *Ahop;
+};
+
+// spinor of B_l at y promoted to double (zero at open-bc boundary points)
+__device__ inline void ld_basis64(const AwArgs &a, int l, int y, double2 *s)
+{
+   float2 v[12];
+   load_spinor(a.basis + (size_t)l * a.fu, y, v);
+   const bool z = a.masked && a.bnd[y];
+#pragma unroll
+   for (int c = 0; c < 12; c++) s[c] = z ? make_double2(0.0, 0.0) : make_double2(v[c].x, v[c].y);
+}
+
+// column l of Ablk and Ahop[ifc] for every block: for each block point x,
+// Y(x) = ((Dw + i mu gamma5 I_eo) restricted to the block) B_l at x (q = 0)
+// and Z_ifc(x) = the hopping term into x from the neighbouring block in
+// direction ifc (q = 1 + ifc); Ablk[n][k][l] = (B_k, Y)_n,
+// Ahop[n][ifc][k][l] = (B_k, Z_ifc)_n, all in double precision
+
+// hopping term of direction MU (FWD: from x + mu, else from x - mu) of B_l
+// into x, added with factor -1/2 to acc
+template <int MU, bool FWD>
+__device__ inline void aw_hop(const AwArgs &a, int l, int x, int y, double2 *acc)
+{
+   const double2 *up;
+   if (FWD)
+      up = (x >= a.h) ? link_ptr(a.ud, x - a.h, 2 * MU) : link_ptr(a.ud, y - a.h, 2 * MU + 1);
+   else
+      up = (x >= a.h) ? link_ptr(a.ud, x - a.h, 2 * MU + 1) : link_ptr(a.ud, y - a.h, 2 * MU);
+   double2 sp[12], t[12];
+   ld_basis64(a, l, y, sp);
+#pragma unroll
+   for (int c = 0; c < 12; c++) t[c] = make_double2(0.0, 0.0);
+   hop_one<double2, MU, FWD>(sp, up, t);
+#pragma unroll
+   for (int c = 0; c < 12; c++) acc[c] = make_double2(acc[c].x - 0.5 * t[c].x, acc[c].y - 0.5 * t[c].y);
+}
+
+// the hops of direction MU whose neighbour lies inside (IN) or outside the
+// block n (blk[y] == n); ifc selects one outside hop (-1: all inside ones)
+template <int MU>
+__device__ inline void aw_dir(const AwArgs &a, const int *blk, int n, int l, int x, int ifc, double2 *acc)
+{
+   const int yf = a.nbr[(size_t)(2 * MU) * a.V + x] & 0x7fffffff;
+   const int yb = a.nbr[(size_t)(2 * MU + 1) * a.V + x] & 0x7fffffff;
+   if (ifc < 0)
+   {
+      if (blk[yf] == n) aw_hop<MU, true>(a, l, x, yf, acc);
+      if (blk[yb] == n) aw_hop<MU, false>(a, l, x, yb, acc);
+   }
+   else if (ifc == 2 * MU + 1)
+   {
+      if (blk[yf] != n) aw_hop<MU, true>(a, l, x, yf, acc);
+   }
+   else if (ifc == 2 * MU)
+   {
+      if (blk[yb] != n) aw_hop<MU, false>(a, l, x, yb, acc);
+   }
+}
+
+__global__ __launch_bounds__(DWG) void k_aw_column(AwArgs a, const int *blk, int l)
+{
+   __shared__ double2 part[DWG / 64][NS_MAX];
+   const int n = blockIdx.x;
+   const int *pts = a.bpts + (size_t)n * a.bvol;
+   const int Ns = a.Ns;
+   const size_t nm = (size_t)Ns * Ns;
+   const int w = threadIdx.x >> 6;
+   for (int q = 0; q < 9; q++)
+   {
+      for (int k = 0; k < Ns; k++)
+         if ((threadIdx.x & 63) == 0) part[w][k] = make_double2(0.0, 0.0);
+      for (int i0 = 0; i0 < a.bvol; i0 += DWG)
+      {
+         const int i = i0 + threadIdx.x;
+         double2 v[12];
+#pragma unroll
+         for (int c = 0; c < 12; c++) v[c] = make_double2(0.0, 0.0);
+         bool any = false;
+         if (i < a.bvol)
+         {
+            const int x = pts[i];
+            if (!(a.masked && a.bnd[x]))
+            {
+               if (q == 0)
+               {
+                  double2 s[12];
+                  ld_basis64(a, l, x, s);
+                  const double m = (x < a.h) ? a.mu : (a.eoflg ? 0.0 : a.mu);
+                  pauli_site(a.swd, x, m, s, v);
+               }
+               const int ifc = q - 1;
+               aw_dir<0>(a, blk, n, l, x, ifc, v);
+               aw_dir<1>(a, blk, n, l, x, ifc, v);
+               aw_dir<2>(a, blk, n, l, x, ifc, v);
+               aw_dir<3>(a, blk, n, l, x, ifc, v);
+               any = true;
+            }
+         }
+         const bool wave_any = __any(any);
+         if (!wave_any) continue;
+         for (int k = 0; k < Ns; k++)
+         {
+            double2 d = make_double2(0.0, 0.0);
+            if (any)
+            {
+               float2 b[12];
+               load_spinor(a.basis + (size_t)k * a.fu, pts[i], b);
+               d = sdot(b, v);
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1)
+            {
+               d.x += __shfl_xor(d.x, m);
+               d.y += __shfl_xor(d.y, m);
+            }
+            if ((threadIdx.x & 63) == 0)
+            {
+               part[w][k].x += d.x;
+               part[w][k].y += d.y;
+            }
+         }
+      }
+      __syncthreads();
+      for (int k = threadIdx.x; k < Ns; k += DWG)
+      {
+         double2 s = make_double2(0.0, 0.0);
+         for (int ww = 0; ww < DWG / 64; ww++)
+         {
+            s.x += part[ww][k].x;
+            s.y += part[ww][k].y;
+         }
+         if (q == 0)
+            a.Ablk[(size_t)n * nm + (size_t)k * Ns + l] = s;
+         else
+            a.Ahop[((size_t)n * 8 + (q - 1)) * nm + (size_t)k * Ns + l] = s;
+      }
+      __syncthreads();
+   }
+}
+
+// per block: Ablk^-1 by Gauss-Jordan elimination with partial pivoting
+// (double precision), the Frobenius condition number |A| |A^-1| against
+// 1e6 (little/Aw_ops.c: MAX_FROBENIUS), then M[ifc] = Ablk^-1 Ahop[ifc];
+// AinvT and MT are stored transposed ([column][row])
+__global__ __launch_bounds__(64) void k_aw_invert(const double2 *Ablk, const double2 *Ahop, int Ns, double2 *AinvT,
+                                                   double2 *MT, float2 *MT32, int *unsafe)
+{
+   extern __shared__ double2 lds[];
+   double2 *A = lds;              // [Ns][2 Ns] augmented
+   __shared__ int piv;
+   __shared__ double fro[2];
+   const int n = blockIdx.x, t = threadIdx.x;
+   const size_t nm = (size_t)Ns * Ns;
+   const int W = 2 * Ns;
+   for (int e = t; e < Ns * W; e += 64)
+   {
+      const int r = e / W, c = e % W;
+      A[e] = (c < Ns) ? Ablk[(size_t)n * nm + (size_t)r * Ns + c]
+                      : make_double2((c - Ns) == r ? 1.0 : 0.0, 0.0);
+   }
+   if (t == 0) fro[0] = fro[1] = 0.0;
+   __syncthreads();
+   if (t == 0)
+   {
+      double s = 0.0;
+      for (int e = 0; e < Ns * W; e++)
+         if (e % W < Ns) s += A[e].x * A[e].x + A[e].y * A[e].y;
+      fro[0] = s;
+   }
+   for (int c = 0; c < Ns; c++)
+   {
+      __syncthreads();
+      if (t == 0)
+      {
+         int p = c;
+         double best = -1.0;
+         for (int r = c; r < Ns; r++)
+         {
+            const double m = A[r * W + c].x * A[r * W + c].x + A[r * W + c].y * A[r * W + c].y;
+            if (m > best) { best = m; p = r; }
+         }
+         piv = p;
+      }
+      __syncthreads();
+      const int p = piv;
+      if (p != c)
+         for (int e = t; e < W; e += 64)
+         {
+            const double2 tmp = A[c * W + e];
+            A[c * W + e] = A[p * W + e];
+            A[p * W + e] = tmp;
+         }
+      __syncthreads();
+      const double2 pv = A[c * W + c];
+      const double pm = pv.x * pv.x + pv.y * pv.y;
+      if (pm == 0.0)
+      {
+         if (t == 0) atomicOr(unsafe, 1);
+         return;
+      }
+      const double2 ip = make_double2(pv.x / pm, -pv.y / pm);
+      __syncthreads();
+      for (int e = t; e < W; e += 64) A[c * W + e] = cmul(ip, A[c * W + e]);
+      __syncthreads();
+      for (int e = t; e < Ns * W; e += 64)
+      {
+         const int r = e / W, col = e % W;
+         if (r == c) continue;
+         const double2 f = A[r * W + c];
+         A[e] = csub(A[e], cmul(f, A[c * W + col]));
+      }
+   }
+   __syncthreads();
+   if (t == 0)
+   {
+      double s = 0.0;
+      for (int r = 0; r < Ns; r++)
+         for (int c = 0; c < Ns; c++)
+         {
+            const double2 v = A[r * W + Ns + c];
+            s += v.x * v.x + v.y * v.y;
+         }
+      fro[1] = s;
+      if (!(sqrt(fro[0] * fro[1]) < 1.0e6)) atomicOr(unsafe, 1);
+   }
+   // AinvT[c][r] = Ainv[r][c]
+   for (int e = t; e < Ns * Ns; e += 64)
+   {
+      const int r = e / Ns, c = e % Ns;
+      AinvT[(size_t)n * nm + (size_t)c * Ns + r] = A[r * W + Ns + c];
+   }
+   // M = Ainv Ahop[ifc], stored transposed: MT[ifc][l][k] = sum_j Ainv[k][j] Ahop[j][l]
+   for (int f = 0; f < 8; f++)
+   {
+      const double2 *H = Ahop + ((size_t)n * 8 + f) * nm;
+      for (int e = t; e < Ns * Ns; e += 64)
+      {
+         const int k = e / Ns, l = e % Ns;
+         double2 s = make_double2(0.0, 0.0);
+         for (int j = 0; j < Ns; j++) s = cadd(s, cmul(A[k * W + Ns + j], H[(size_t)j * Ns + l]));
+         MT[((size_t)n * 8 + f) * nm + (size_t)l * Ns + k] = s;
+         MT32[((size_t)n * 8 + f) * nm + (size_t)l * Ns + k] = make_float2((float)s.x, (float)s.y);
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// the even-odd preconditioned little operator Ahat = 1 - Aee^-1 Aeo Aoo^-1 Aoe
+// ---------------------------------------------------------------------------
+
+// out[b][k] = sign * sum_ifc sum_l M[b0+b][ifc][k][l] in[inn[b0+b][ifc] - i0][l]
+// (+ add[b][k] if add); one work-group of 64 lanes per block, lane k < Ns
+template <typename C>
+__global__ __launch_bounds__(64) void k_aw_hop(const C *MT, const int *inn, int Ns, int b0, int i0, const C *in,
+                                                const C *add, C *out, int neg)
+{
+   const int b = blockIdx.x, k = threadIdx.x;
+   if (k >= Ns) return;
+   const size_t nm = (size_t)Ns * Ns;
+   const int n = b0 + b;
+   typedef decltype(C::x) R;
+   R sx = 0, sy = 0;
+   for (int f = 0; f < 8; f++)
+   {
+      const C *M = MT + ((size_t)n * 8 + f) * nm;
+      const C *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
+      for (int l = 0; l < Ns; l++)
+      {
+         const C m = M[(size_t)l * Ns + k], x = v[l];
+         sx += m.x * x.x - m.y * x.y;
+         sy += m.x * x.y + m.y * x.x;
+      }
+   }
+   if (neg)
+   {
+      sx = -sx;
+      sy = -sy;
+   }
+   if (add)
+   {
+      sx += add[(size_t)b * Ns + k].x;
+      sy += add[(size_t)b * Ns + k].y;
+   }
+   out[(size_t)b * Ns + k] = cmk<C>(sx, sy);
+}
+
+// out[b] = AinvT^T of block b0+b times in[b] (+ add)
+__global__ __launch_bounds__(64) void k_aw_diag(const double2 *AinvT, int Ns, int b0, const double2 *in, double2 *out)
+{
+   const int b = blockIdx.x, k = threadIdx.x;
+   if (k >= Ns) return;
+   const size_t nm = (size_t)Ns * Ns;
+   const double2 *A = AinvT + (size_t)(b0 + b) * nm;
+   double2 s = make_double2(0.0, 0.0);
+   for (int l = 0; l < Ns; l++) s = cadd(s, cmul(A[(size_t)l * Ns + k], in[(size_t)b * Ns + l]));
+   out[(size_t)b * Ns + k] = s;
+}
+
+// Ahat v on the even blocks; t: work vector on the odd blocks
+template <typename C> void ahat(const C *MT, const C *v, C *t, C *w)
+{
+   Lib &L = lib();
+   k_aw_hop<C><<<D.nbh, 64, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
+   k_aw_hop<C><<<D.nbh, 64, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
+}
+
+// ---------------------------------------------------------------------------
+// little-vector linear algebra (contiguous complex vectors of n entries)
+// ---------------------------------------------------------------------------
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_vdot(const C *a, const C *b, size_t n, double *part)
+{
+   __shared__ double sh[3][256];
+   double x = 0.0, y = 0.0, z = 0.0;
+   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+   {
+      const double ax = a[i].x, ay = a[i].y, bx = b[i].x, by = b[i].y;
+      x += ax * bx + ay * by;
+      y += ax * by - ay * bx;
+      z += bx * bx + by * by;
+   }
+   sh[0][threadIdx.x] = x;
+   sh[1][threadIdx.x] = y;
+   sh[2][threadIdx.x] = z;
+   __syncthreads();
+   for (int m = 128; m > 0; m >>= 1)
+   {
+      if (threadIdx.x < m)
+         for (int q = 0; q < 3; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + m];
+      __syncthreads();
+   }
+   if (threadIdx.x == 0)
+      for (int q = 0; q < 3; q++) part[3 * blockIdx.x + q] = sh[q][0];
+}
+
+// (a, b) and |b|^2 in double, summed in a fixed order
+template <typename C> void vdot(const C *a, const C *b, size_t n, double *re, double *im, double *nb2)
+{
+   Lib &L = lib();
+   int nbk = (int)((n + 255) / 256);
+   if (nbk > 1024) nbk = 1024;
+   k_vdot<C><<<nbk, 256, 0, L.stream>>>(a, b, n, D.d_red);
+   HIP_OK(hipMemcpyAsync(D.h_red, D.d_red, sizeof(double) * 3 * nbk, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   double s[3] = {0.0, 0.0, 0.0};
+   for (int b = 0; b < nbk; b++)
+      for (int q = 0; q < 3; q++) s[q] += D.h_red[3 * b + q];
+   if (re) *re = s[0];
+   if (im) *im = s[1];
+   if (nb2) *nb2 = s[2];
+}
+
+// y = y + z x (complex z)
+template <typename C>
+__global__ __launch_bounds__(256) void k_vaxpy(C *y, const C *x, size_t n, double zr, double zi)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i >= n) return;
+   typedef decltype(C::x) R;
+   const R a = (R)zr, b = (R)zi;
+   y[i] = cmk<C>(y[i].x + (a * x[i].x - b * x[i].y), y[i].y + (a * x[i].y + b * x[i].x));
+}
+
+template <typename C> void vaxpy(C *y, const C *x, size_t n, double zr, double zi)
+{
+   k_vaxpy<C><<<(unsigned)((n + 255) / 256), 256, 0, lib().stream>>>(y, x, n, zr, zi);
+}
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_vscale(C *y, size_t n, double s)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i >= n) return;
+   typedef decltype(C::x) R;
+   y[i] = cmk<C>((R)s * y[i].x, (R)s * y[i].y);
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(256) void k_vconv(const A *a, B *b, size_t n)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i >= n) return;
+   typedef decltype(B::x) R;
+   b[i] = cmk<B>((R)a[i].x, (R)a[i].y);
+}
+
+template <typename A, typename B> void vconv(const A *a, B *b, size_t n)
+{
+   k_vconv<A, B><<<(unsigned)((n + 255) / 256), 256, 0, lib().stream>>>(a, b, n);
+}
+
+double2 *wvd(int k)
+{
+   while ((int)D.wvd.size() <= k) D.wvd.push_back((double2 *)dev_alloc(sizeof(double2) * (size_t)D.nb * D.Ns));
+   return D.wvd[k];
+}
+
+float2 *wv(int k)
+{
+   while ((int)D.wv.size() <= k) D.wv.push_back((float2 *)dev_alloc(sizeof(float2) * (size_t)D.nb * D.Ns));
+   return D.wv[k];
+}
+
+// single-pass GCR in single precision for Ahat x = b (linsolv/gcr4v.c):
+// returns the number of Krylov vectors (or -1 if res was not reached in nmx)
+int gcr4v(int nmx, double res, const float2 *b, float2 *x)
+{
+   const size_t nv = (size_t)D.nbh * D.Ns;
+   float2 *rho = wv(0), *t = wv(1);
+   std::vector<float2 *> phi(nmx), chi(nmx);
+   for (int k = 0; k < nmx; k++)
+   {
+      phi[k] = wv(2 + k);
+      chi[k] = wv(2 + nmx + k);
+   }
+   std::vector<double> bn(nmx);
+   std::vector<double> ar((size_t)nmx * nmx, 0.0), ai((size_t)nmx * nmx, 0.0), cr(nmx), ci(nmx);
+   Lib &L = lib();
+   HIP_OK(hipMemcpyAsync(rho, b, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
+   double r0;
+   vdot<float2>(rho, rho, nv, nullptr, nullptr, &r0);
+   r0 = sqrt(r0);
+   double rn = r0;
+   int n = 0, status = -1;
+   for (; n < nmx; n++)
+   {
+      ahat<float2>(D.d_MT32, rho, t, chi[n]);
+      HIP_OK(hipMemcpyAsync(phi[n], rho, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
+      // classical Gram-Schmidt of chi[n] against chi[0..n-1]
+      for (int l = 0; l < n; l++)
+      {
+         double re, im;
+         vdot<float2>(chi[l], chi[n], nv, &re, &im, nullptr);
+         ar[(size_t)l * nmx + n] = re;
+         ai[(size_t)l * nmx + n] = im;
+      }
+      for (int l = 0; l < n; l++)
+         vaxpy<float2>(chi[n], chi[l], nv, -(float)ar[(size_t)l * nmx + n], -(float)ai[(size_t)l * nmx + n]);
+      double nn;
+      vdot<float2>(chi[n], chi[n], nv, nullptr, nullptr, &nn);
+      bn[n] = sqrt(nn);
+      if (bn[n] == 0.0) break;
+      k_vscale<float2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(chi[n], nv, 1.0 / bn[n]);
+      double re, im;
+      vdot<float2>(chi[n], rho, nv, &re, &im, nullptr);
+      cr[n] = re;
+      ci[n] = im;
+      vaxpy<float2>(rho, chi[n], nv, -re, -im);
+      vdot<float2>(rho, rho, nv, nullptr, nullptr, &rn);
+      rn = sqrt(rn);
+      if (rn <= res * r0)
+      {
+         n++;
+         status = n;
+         break;
+      }
+   }
+   // back substitution (gcr4v.c set_psi): x = sum c'_l phi_l
+   const int m = n;
+   for (int l = m - 1; l >= 0; l--)
+   {
+      double zr = cr[l], zi = ci[l];
+      for (int i = l + 1; i < m; i++)
+      {
+         zr -= ar[(size_t)l * nmx + i] * cr[i] - ai[(size_t)l * nmx + i] * ci[i];
+         zi -= ar[(size_t)l * nmx + i] * ci[i] + ai[(size_t)l * nmx + i] * cr[i];
+      }
+      cr[l] = zr / bn[l];
+      ci[l] = zi / bn[l];
+   }
+   HIP_OK(hipMemsetAsync(x, 0, sizeof(float2) * nv, L.stream));
+   for (int l = 0; l < m; l++) vaxpy<float2>(x, phi[l], nv, cr[l], ci[l]);
+   return status;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI: parameters (flags/dfl_parms.c)
+// ---------------------------------------------------------------------------
+
+extern "C" dfl_parms_t set_dfl_parms(int *bs, int Ns)
+{
+   const char *where = "set_dfl_parms [dfl_parms.c]";
+   Lib &L = lib();
+   dfl_parms_t p;
+   memset(&p, 0, sizeof(p));
+   if (!require_geometry(where)) return p;
+   if (D.Ns > 0 && (memcmp(bs, D.bs, sizeof(D.bs)) != 0 || Ns != D.Ns))
+   {
+      fail(where, "bs[4] and Ns may be set only once");
+      return p;
+   }
+   int nblk = 1;
+   for (int mu = 0; mu < 4; mu++)
+   {
+      if (bs[mu] < 4 || bs[mu] > L.g.L[mu] || (bs[mu] % 2) || (L.g.L[mu] % bs[mu]) || ((L.g.L[mu] / bs[mu]) % 2))
+      {
+         fail(where, "Block sizes are out of range, odd, do not divide the lattice or give an odd number of blocks");
+         return p;
+      }
+      nblk *= L.g.L[mu] / bs[mu];
+   }
+   if (nblk % (2 * L.g.nthread))
+   {
+      fail(where, "The number of blocks in the local lattice must be an even multiple of NTHREAD");
+      return p;
+   }
+   if (Ns < 2 || (Ns & 1) || Ns > NS_MAX)
+   {
+      fail(where, "Improper value of Ns");
+      return p;
+   }
+   memcpy(D.bs, bs, sizeof(D.bs));
+   D.Ns = Ns;
+   memcpy(p.bs, bs, sizeof(p.bs));
+   p.Ns = Ns;
+   return p;
+}
+
+extern "C" dfl_parms_t dfl_parms(void)
+{
+   dfl_parms_t p;
+   memcpy(p.bs, D.bs, sizeof(p.bs));
+   p.Ns = D.Ns;
+   return p;
+}
+
+extern "C" dfl_pro_parms_t set_dfl_pro_parms(int nkv, int nmx, double res, int nmx_gcr, double res_gcr)
+{
+   dfl_pro_parms_t p;
+   memset(&p, 0, sizeof(p));
+   if (nkv < 1 || nmx < 1 || res <= DBL_EPSILON || nmx_gcr < 2 || res_gcr <= FLT_EPSILON)
+   {
+      fail("set_dfl_pro_parms [dfl_parms.c]", "Parameters are out of range");
+      return p;
+   }
+   D.nkv = nkv;
+   D.nmx = nmx;
+   D.res = res;
+   D.nmx_gcr = nmx_gcr;
+   D.res_gcr = res_gcr;
+   p.nkv = nkv;
+   p.nmx = nmx;
+   p.nmx_gcr = nmx_gcr;
+   p.res = res;
+   p.res_gcr = res_gcr;
+   return p;
+}
+
+extern "C" dfl_pro_parms_t dfl_pro_parms(void)
+{
+   dfl_pro_parms_t p;
+   p.nkv = D.nkv;
+   p.nmx = D.nmx;
+   p.nmx_gcr = D.nmx_gcr;
+   p.res = D.res;
+   p.res_gcr = D.res_gcr;
+   return p;
+}
+
+extern "C" dfl_gen_parms_t set_dfl_gen_parms(double kappa, double mu, int ninv, int nmr, int ncy)
+{
+   dfl_gen_parms_t p;
+   memset(&p, 0, sizeof(p));
+   if (ninv < 4 || nmr < 1 || ncy < 1 || kappa < 0.0)
+   {
+      fail("set_dfl_gen_parms [dfl_parms.c]", "Parameters are out of range");
+      return p;
+   }
+   D.ninv = ninv;
+   D.nmr = nmr;
+   D.ncy = ncy;
+   D.kappa = kappa;
+   D.mu = mu;
+   D.m0 = (kappa != 0.0) ? 1.0 / (2.0 * kappa) - 4.0 : DBL_MAX;
+   p.ninv = ninv;
+   p.nmr = nmr;
+   p.ncy = ncy;
+   p.kappa = kappa;
+   p.m0 = D.m0;
+   p.mu = mu;
+   return p;
+}
+
+extern "C" dfl_gen_parms_t dfl_gen_parms(void)
+{
+   dfl_gen_parms_t p;
+   p.ninv = D.ninv;
+   p.nmr = D.nmr;
+   p.ncy = D.ncy;
+   p.kappa = D.kappa;
+   p.m0 = D.m0;
+   p.mu = D.mu;
+   return p;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: the deflation subspace (dfl/dfl_subspace.c)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// device little vector <-> host array of nb * Ns complex_dble
+bool s2vd_dev(const float2 *s, double2 *vd)
+{
+   Lib &L = lib();
+   k_blk_proj<float2><<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, s, vd);
+   return true;
+}
+
+void vd2s_dev(const double2 *vd, float2 *s)
+{
+   Lib &L = lib();
+   k_blk_prolong<<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, vd, s);
+}
+
+bool ready(const char *where)
+{
+   if (!dfl_grid(where)) return false;
+   if (!D.basis_ok)
+   {
+      fail(where, "the deflation subspace is not initialised (dfl_subspace or dfl_modes)");
+      return false;
+   }
+   return true;
+}
+
+}  // namespace
+
+extern "C" void dfl_subspace(spinor **mds)
+{
+   const char *where = "dfl_subspace [dfl_subspace.c]";
+   Lib &L = lib();
+   if (!dfl_grid(where)) return;
+   HIP_OK(hipMemsetAsync(D.d_unsafe, 0, sizeof(int), L.stream));
+   for (int l = 0; l < D.Ns; l++)
+   {
+      Res r = resolve(mds[l], F32, L.g.V, 0, true);
+      if (!r.dev) return;
+      if (r.off)
+      {
+         fail(where, "the modes must be full fields");
+         return;
+      }
+      k_blk_gs<<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, (const float2 *)r.dev,
+                                           D.d_s64, l, D.d_vmds, D.d_unsafe);
+      finish(r, F32, L.g.V, false);
+   }
+   int degen = 0;
+   HIP_OK(hipMemcpyAsync(&degen, D.d_unsafe, sizeof(int), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   if (degen)
+   {
+      fail(where, "Degenerate block modes");
+      return;
+   }
+   D.basis_ok = true;
+   D.aw_ok = D.awhat_ok = false;
+}
+
+extern "C" void dfl_s2vd(spinor *s, complex_dble *vd)
+{
+   const char *where = "dfl_s2vd [dfl_subspace.c]";
+   Lib &L = lib();
+   if (!ready(where)) return;
+   Res r = resolve(s, F32, L.g.V, 0, true);
+   if (!r.dev) return;
+   double2 *w = wvd(0);
+   s2vd_dev((const float2 *)r.dev, w);
+   HIP_OK(hipMemcpyAsync(vd, w, sizeof(double2) * (size_t)D.nb * D.Ns, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   finish(r, F32, L.g.V, false);
+}
+
+extern "C" void dfl_vd2s(complex_dble *vd, spinor *s)
+{
+   const char *where = "dfl_vd2s [dfl_subspace.c]";
+   Lib &L = lib();
+   if (!ready(where)) return;
+   Res r = resolve(s, F32, L.g.V, 0, false);
+   if (!r.dev) return;
+   double2 *w = wvd(0);
+   HIP_OK(hipMemcpyAsync(w, vd, sizeof(double2) * (size_t)D.nb * D.Ns, hipMemcpyHostToDevice, L.stream));
+   vd2s_dev(w, (float2 *)r.dev);
+   finish(r, F32, L.g.V, true);
+}
+
+extern "C" void dfl_renormalize_modes(spinor **mds)
+{
+   Lib &L = lib();
+   const int vt = L.g.VT;
+   for (int l = 0; l < D.Ns; l++)
+   {
+      for (int j = 0; j < l; j++) project(vt, 3, mds[l], mds[j]);
+      (void)normalize(vt, 3, mds[l]);
+   }
+}
+
+// basis vector k as a host array in the reference's layout (test access)
+extern "C" void oqh_dfl_export_basis(int k, spinor *host)
+{
+   const char *where = "oqh_dfl_export_basis";
+   Lib &L = lib();
+   if (!ready(where)) return;
+   if (k < 0 || k >= D.Ns)
+   {
+      fail(where, "k out of range");
+      return;
+   }
+   oqh_export_s((spinor *)(D.d_basis + (size_t)k * D.fu), host, L.g.V);
+}
+
+// block structure: inn[nb][8] and the points of every block, bpts[nb][bvol]
+extern "C" int oqh_dfl_blocks(int *inn, int *bpts)
+{
+   if (!dfl_grid("oqh_dfl_blocks")) return 0;
+   if (inn) memcpy(inn, D.inn.data(), sizeof(int) * D.inn.size());
+   if (bpts) memcpy(bpts, D.bpts.data(), sizeof(int) * D.bpts.size());
+   return D.nb;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: the little Dirac operator (little/Aw_ops.c)
+// ---------------------------------------------------------------------------
+
+extern "C" void set_Aw(double mu)
+{
+   const char *where = "set_Aw [Aw_ops.c]";
+   Lib &L = lib();
+   if (!ready(where)) return;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return;
+   }
+   if (D.aw_ok && D.aw_mu == mu && D.aw_ud == L.ud_version && D.aw_swd == L.swd_epoch && D.aw_eo == L.eoflg &&
+       L.swd_valid && !L.swd_inv[0] && !L.swd_inv[1])
+      return;
+   ensure_swd(0, 0);
+   static int *d_blk = nullptr;
+   static int blk_n = 0;
+   if (!d_blk || blk_n != L.g.V)
+   {
+      dev_free(d_blk);
+      std::vector<int> blk(L.g.V);
+      for (int n = 0; n < D.nb; n++)
+         for (int i = 0; i < D.bvol; i++) blk[D.bpts[(size_t)n * D.bvol + i]] = n;
+      d_blk = (int *)dev_alloc(sizeof(int) * L.g.V);
+      if (!d_blk) return;
+      HIP_OK(hipMemcpy(d_blk, blk.data(), sizeof(int) * L.g.V, hipMemcpyHostToDevice));
+      blk_n = L.g.V;
+   }
+   AwArgs a;
+   memset(&a, 0, sizeof(a));
+   a.basis = D.d_basis;
+   a.fu = D.fu;
+   a.Ns = D.Ns;
+   a.bvol = D.bvol;
+   a.V = L.g.V;
+   a.h = L.g.V / 2;
+   a.masked = (L.bc != 3);
+   a.eoflg = L.eoflg;
+   a.bpts = D.d_bpts;
+   a.nbr = L.d_nbr;
+   a.bnd = L.d_bnd;
+   a.ud = L.d_ud;
+   a.swd = L.d_swd;
+   a.mu = mu;
+   a.Ablk = D.d_Ablk;
+   a.Ahop = D.d_Ahop;
+   for (int l = 0; l < D.Ns; l++) k_aw_column<<<D.nb, DWG, 0, L.stream>>>(a, d_blk, l);
+   HIP_OK(hipStreamSynchronize(L.stream));
+   D.aw_ok = true;
+   D.awhat_ok = false;
+   D.aw_mu = mu;
+   D.aw_ud = L.ud_version;
+   D.aw_swd = L.swd_epoch;
+   D.aw_eo = L.eoflg;
+}
+
+extern "C" int set_Awhat(double mu)
+{
+   Lib &L = lib();
+   set_Aw(mu);
+   if (!D.aw_ok) return 1;
+   if (D.awhat_ok) return 0;
+   const size_t lds = sizeof(double2) * (size_t)D.Ns * 2 * D.Ns;
+   HIP_OK(hipMemsetAsync(D.d_unsafe, 0, sizeof(int), L.stream));
+   k_aw_invert<<<D.nb, 64, lds, L.stream>>>(D.d_Ablk, D.d_Ahop, D.Ns, D.d_AinvT, D.d_MT, D.d_MT32, D.d_unsafe);
+   int unsafe = 0;
+   HIP_OK(hipMemcpyAsync(&unsafe, D.d_unsafe, sizeof(int), hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   D.awhat_ok = (unsafe == 0);
+   return unsafe ? 1 : 0;
+}
+
+// Ablk [nb][Ns][Ns] and Ahop [nb][8][Ns][Ns] (complex double, row-major) of
+// the last set_Aw (test access)
+extern "C" void oqh_dfl_export_Aw(complex_dble *ablk, complex_dble *ahop)
+{
+   Lib &L = lib();
+   if (!D.aw_ok)
+   {
+      fail("oqh_dfl_export_Aw", "the little Dirac operator is not set (set_Aw)");
+      return;
+   }
+   const size_t nm = (size_t)D.Ns * D.Ns;
+   if (ablk) HIP_OK(hipMemcpy(ablk, D.d_Ablk, sizeof(double2) * nm * D.nb, hipMemcpyDeviceToHost));
+   if (ahop) HIP_OK(hipMemcpy(ahop, D.d_Ahop, sizeof(double2) * nm * 8 * D.nb, hipMemcpyDeviceToHost));
+   (void)L;
+}
+
+// w = Ahat v on the even blocks (double precision), host arrays of
+// nb/2 * Ns complex numbers (little/Aw_dble.c Awhat_dble)
+extern "C" void Awhat_dble(complex_dble *v, complex_dble *w)
+{
+   Lib &L = lib();
+   if (!D.awhat_ok)
+   {
+      fail("Awhat_dble [Aw_dble.c]", "the even-odd little operator is not set (set_Awhat)");
+      return;
+   }
+   const size_t nv = (size_t)D.nbh * D.Ns;
+   double2 *a = wvd(1), *t = wvd(2), *b = wvd(3);
+   HIP_OK(hipMemcpyAsync(a, v, sizeof(double2) * nv, hipMemcpyHostToDevice, L.stream));
+   ahat<double2>(D.d_MT, a, t, b);
+   HIP_OK(hipMemcpyAsync(w, b, sizeof(double2) * nv, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+// ---------------------------------------------------------------------------
+// the little solver (dfl/ltl_gcr.c)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// psi = A^-1 eta (full little vectors on the device), FGCR on the even-odd
+// form; status[0]: outer Krylov vectors, status[1]: inner ones (sum)
+double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
+{
+   Lib &L = lib();
+   const size_t nv = (size_t)D.nbh * D.Ns;
+   ifail[0] = 0;
+   status[0] = status[1] = 0;
+   const int nkv = D.nkv;
+   double2 *u = wvd(4), *b = wvd(5), *x = wvd(6), *r = wvd(7), *t = wvd(8);
+   // u_o = Aoo^-1 eta_o, b = Aee^-1 eta_e - sum M_e u_o
+   k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, D.nbh, eta + nv, u);
+   k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, 0, eta, t);
+   k_aw_hop<double2><<<D.nbh, 64, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, 0, D.nbh, u, t, b, 1);
+   double bn2;
+   vdot<double2>(b, b, nv, nullptr, nullptr, &bn2);
+   const double rn0 = sqrt(bn2);
+   HIP_OK(hipMemsetAsync(x, 0, sizeof(double2) * nv, L.stream));
+   double rn = rn0;
+   if (rn0 > 0.0)
+   {
+      // FGCR (linsolv/fgcr4vd.c) with Mop = gcr4v in single precision
+      std::vector<double2 *> phi(nkv), chi(nkv);
+      for (int k = 0; k < nkv; k++)
+      {
+         phi[k] = wvd(9 + k);
+         chi[k] = wvd(9 + nkv + k);
+      }
+      float2 *b32 = wv(2 + 2 * D.nmx_gcr), *x32 = wv(3 + 2 * D.nmx_gcr);
+      std::vector<double> ar((size_t)nkv * nkv), ai((size_t)nkv * nkv), bn(nkv), cr(nkv), ci(nkv);
+      HIP_OK(hipMemcpyAsync(r, b, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
+      const double tol = D.res * rn0;
+      while (rn > tol)
+      {
+         int k;
+         for (k = 0;; k++)
+         {
+            // phi = (approximate Ahat^-1) r, chi = Ahat phi
+            vconv<double2, float2>(r, b32, nv);
+            const double rres = (rn * D.res_gcr < tol) ? tol / rn : D.res_gcr;
+            const int st = gcr4v(D.nmx_gcr, rres, b32, x32);
+            status[1] += (st < 0) ? D.nmx_gcr : st;
+            vconv<float2, double2>(x32, phi[k], nv);
+            ahat<double2>(D.d_MT, phi[k], t, chi[k]);
+            for (int l = 0; l < k; l++)
+            {
+               double re, im;
+               vdot<double2>(chi[l], chi[k], nv, &re, &im, nullptr);
+               ar[(size_t)l * nkv + k] = re;
+               ai[(size_t)l * nkv + k] = im;
+               vaxpy<double2>(chi[k], chi[l], nv, -re, -im);
+            }
+            double nn;
+            vdot<double2>(chi[k], chi[k], nv, nullptr, nullptr, &nn);
+            bn[k] = sqrt(nn);
+            if (bn[k] == 0.0) break;
+            k_vscale<double2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(chi[k], nv, 1.0 / bn[k]);
+            double re, im;
+            vdot<double2>(chi[k], r, nv, &re, &im, nullptr);
+            cr[k] = re;
+            ci[k] = im;
+            vaxpy<double2>(r, chi[k], nv, -re, -im);
+            double r2;
+            vdot<double2>(r, r, nv, nullptr, nullptr, &r2);
+            rn = sqrt(r2);
+            status[0] += 1;
+            if (rn <= tol || k == nkv - 1 || status[0] >= D.nmx) break;
+         }
+         const int m = (bn[k] == 0.0) ? k : k + 1;
+         for (int l = m - 1; l >= 0; l--)
+         {
+            double zr = cr[l], zi = ci[l];
+            for (int i = l + 1; i < m; i++)
+            {
+               zr -= ar[(size_t)l * nkv + i] * cr[i] - ai[(size_t)l * nkv + i] * ci[i];
+               zi -= ar[(size_t)l * nkv + i] * ci[i] + ai[(size_t)l * nkv + i] * cr[i];
+            }
+            cr[l] = zr / bn[l];
+            ci[l] = zi / bn[l];
+         }
+         for (int l = 0; l < m; l++) vaxpy<double2>(x, phi[l], nv, cr[l], ci[l]);
+         // recompute the residual r = b - Ahat x
+         ahat<double2>(D.d_MT, x, t, r);
+         k_vscale<double2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(r, nv, -1.0);
+         vaxpy<double2>(r, b, nv, 1.0, 0.0);
+         double r2;
+         vdot<double2>(r, r, nv, nullptr, nullptr, &r2);
+         rn = sqrt(r2);
+         if (status[0] >= D.nmx && rn > tol)
+         {
+            ifail[0] = -1;
+            break;
+         }
+         if (bn[k] == 0.0) break;
+      }
+   }
+   // psi_e = x, psi_o = u_o - sum M_o x_e
+   HIP_OK(hipMemcpyAsync(psi, x, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
+   k_aw_hop<double2><<<D.nbh, 64, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, D.nbh, 0, x, u, psi + nv, 1);
+   return rn;
+}
+
+}  // namespace
+
+extern "C" double ltl_gcr(double mu, complex_dble *eta, complex_dble *psi, int *ifail, int *status)
+{
+   const char *where = "ltl_gcr [ltl_gcr.c]";
+   Lib &L = lib();
+   if (!ready(where)) return 0.0;
+   if (D.nkv == 0)
+   {
+      fail(where, "Deflation projection parameters are not set");
+      return 0.0;
+   }
+   const size_t n = (size_t)D.nb * D.Ns;
+   if (set_Awhat(mu))
+   {
+      ifail[0] = -2;
+      status[0] = status[1] = 0;
+      return 0.0;
+   }
+   double2 *e = wvd(40 + 2 * D.nkv), *p = wvd(41 + 2 * D.nkv);
+   HIP_OK(hipMemcpyAsync(e, eta, sizeof(double2) * n, hipMemcpyHostToDevice, L.stream));
+   const double rho = ltl_solve(e, p, ifail, status);
+   HIP_OK(hipMemcpyAsync(psi, p, sizeof(double2) * n, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   return rho;
+}
+
+// ---------------------------------------------------------------------------
+// the deflated solver (dfl/dfl_sap_gcr.c) and the generation of the
+// subspace (dfl/dfl_modes.c)
+// ---------------------------------------------------------------------------
+
+namespace oqh {
+void random_uniform_s(float2 *f);  // rng.hip: uniform in [-1/2, 1/2) per component
+}
+
+namespace {
+
+struct DflRun
+{
+   double mud = 0.0;
+   float mus = 0.0f;
+   int isolv = 1, nmr = 4, ncy = 5;
+   int ltl[2] = {0, 0};
+   float2 *psi0 = nullptr;   // FP32 work field of the preconditioner
+   float2 *wtmp = nullptr;
+   double2 *vin = nullptr, *vout = nullptr;
+};
+
+DflRun R;
+
+float2 *work_field(float2 **p)
+{
+   if (!*p) *p = (float2 *)dev_alloc(sizeof(float2) * units_for(lib().g.V, 12));
+   return *p;
+}
+
+// the deflated preconditioner (dfl_sap_gcr.c Mop): psi0 = P_L solution of
+// the little equation, then ncy Schwarz cycles on the remaining residual;
+// phi = psi0 + SAP, chi = D phi
+void dfl_mop_dev(const float2 *rho, float2 *phi, float2 *chi)
+{
+   Lib &L = lib();
+   const int vt = L.g.VT;
+   float2 *p0 = work_field(&R.psi0);
+   if (!R.vin) R.vin = wvd(42 + 2 * D.nkv);
+   if (!R.vout) R.vout = wvd(43 + 2 * D.nkv);
+   s2vd_dev(rho, R.vin);
+   int ifail = 0, st[2] = {0, 0};
+   (void)ltl_solve(R.vin, R.vout, &ifail, st);
+   R.ltl[0] += st[0];
+   R.ltl[1] += st[1];
+   vd2s_dev(R.vout, p0);
+   Dw(R.mus, (spinor *)p0, (spinor *)chi);
+   diff_s2s(vt, 2, (spinor *)rho, (spinor *)chi);   // chi = rho - D psi0
+   if (L.sap_prec == 1 && sap_mixed_mop(R.mus, R.isolv, R.nmr, R.ncy, chi, phi, false))
+   {
+      // FP16 Schwarz cycles from psi = 0; chi = D (phi + psi0) in FP32
+      add_s2s(vt, 2, (spinor *)p0, (spinor *)phi);
+      Dw(R.mus, (spinor *)phi, (spinor *)chi);
+      return;
+   }
+   set_s2zero(vt, 2, (spinor *)phi);
+   for (int n = 0; n < R.ncy; n++) sap_device(R.mus, R.isolv, R.nmr, phi, chi);
+   diff_s2s(vt, 2, (spinor *)rho, (spinor *)chi);   // chi = D phi
+   add_s2s(vt, 2, (spinor *)p0, (spinor *)phi);
+   // chi = rho - (rho - D psi0 - D phi_sap) is D phi_sap; add D psi0
+   Dw(R.mus, (spinor *)p0, (spinor *)work_field(&R.wtmp));
+   add_s2s(vt, 2, (spinor *)R.wtmp, (spinor *)chi);
+}
+
+void dfl_Dop(spinor_dble *s, spinor_dble *r) { Dw_dble(R.mud, s, r); }
+
+void dfl_Mop(int k, spinor *rho, spinor *phi, spinor *chi)
+{
+   (void)k;
+   int o0 = 0, o1 = 0, o2 = 0;
+   float2 *r = (float2 *)device_field(rho, F32, &o0), *p = (float2 *)device_field(phi, F32, &o1),
+          *c = (float2 *)device_field(chi, F32, &o2);
+   if (!r || !p || !c || o0 || o1 || o2)
+   {
+      fail("dfl_sap_gcr [dfl_sap_gcr.c]", "the Krylov vectors must be full device fields");
+      return;
+   }
+   dfl_mop_dev(r, p, c);
+}
+
+}  // namespace
+
+extern "C" double dfl_sap_gcr(int nkv, int nmx, int istop, double res, double mu, spinor_dble *eta,
+                              spinor_dble *psi, int *ifail, int *status)
+{
+   const char *where = "dfl_sap_gcr [dfl_sap_gcr.c]";
+   Lib &L = lib();
+   ifail[0] = 0;
+   status[0] = status[1] = status[2] = 0;
+   if (!ready(where)) return 0.0;
+   if (L.sap_ncy == 0)
+   {
+      fail(where, "SAP parameters are not set");
+      return 0.0;
+   }
+   if (D.nkv == 0)
+   {
+      fail(where, "Deflation projection parameters are not set");
+      return 0.0;
+   }
+   const int vt = L.g.VT;
+   double rho = 0.0;
+   if (!sap_prepare(L.sap_isolv, where, true))
+      ifail[0] = -3;
+   else if (set_Awhat(mu))
+      ifail[0] = -2;
+   else
+   {
+      const double rho0 = istop ? unorm_dble(vt, 3, eta) : sqrt(norm_square_dble(vt, 3, eta).q[0]);
+      if (rho0 != 0.0)
+      {
+         alloc_ws(L.ws_used + 2 * nkv + 1);
+         alloc_wsd(L.wsd_used + 2);
+         spinor **ws = reserve_ws(2 * nkv + 1);
+         spinor_dble **wsd = reserve_wsd(2);
+         spinor_dble *rsd = wsd[1];
+         assign_sd2sd(vt, 2, eta, rsd);
+         scale_dble(vt, 2, 1.0 / rho0, rsd);
+         R.mud = mu;
+         R.mus = (float)mu;
+         R.isolv = L.sap_isolv;
+         R.nmr = L.sap_nmr;
+         R.ncy = L.sap_ncy;
+         R.ltl[0] = R.ltl[1] = 0;
+         rho = fgcr(vt, dfl_Dop, dfl_Mop, ws, wsd, nkv, nmx, istop, res, rsd, psi, status);
+         scale_dble(vt, 2, rho0, psi);
+         rho *= rho0;
+         if (status[0] < 0)
+         {
+            ifail[0] = -1;
+            status[0] = nmx;
+         }
+         if (status[0] > 0)
+         {
+            status[1] = (R.ltl[0] + status[0] / 2) / status[0];
+            status[2] = (R.ltl[1] + status[0] / 2) / status[0];
+         }
+         release_wsd();
+         release_ws();
+      }
+      else
+         set_sd2zero(vt, 2, psi);
+   }
+   if (ifail[0] <= -2)
+   {
+      if (psi != eta) set_sd2zero(vt, 2, psi);
+      rho = istop ? unorm_dble(vt, 3, eta) : sqrt(norm_square_dble(vt, 3, eta).q[0]);
+   }
+   return rho;
+}
+
+extern "C" void dfl_modes(int *ifail, int *status)
+{
+   const char *where = "dfl_modes [dfl_modes.c]";
+   Lib &L = lib();
+   ifail[0] = 0;
+   status[0] = status[1] = 0;
+   if (!dfl_grid(where)) return;
+   if (D.nkv == 0 || D.ninv == 0)
+   {
+      fail(where, "Deflation projection or subspace generation parameters are not set");
+      return;
+   }
+   if (L.sap_ncy == 0)
+   {
+      fail(where, "SAP parameters are not set");
+      return;
+   }
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return;
+   }
+   // set_flds: the generation mass and the twisted-mass flag
+   const double m0_keep = L.m0;
+   const int eo_keep = L.eoflg;
+   if (D.m0 != L.m0) oqh_set_sw_parms(D.m0, L.csw);
+   oqh_set_tm_parms(1);
+   if (!sap_prepare(1, where, true)) ifail[0] = -3;
+   const int vt = L.g.VT;
+   alloc_ws(L.ws_used + D.Ns + 2);
+   spinor **mds = reserve_ws(D.Ns);
+   spinor **ws = reserve_ws(2);
+   int off = 0;  // site offset of a device field (the fields are full)
+   for (int l = 0; l < D.Ns; l++)
+   {
+      random_uniform_s((float2 *)device_field(mds[l], F32, &off));
+      bnd_s2zero(ALL_PTS, mds[l]);
+   }
+   const float mus = (float)D.mu;
+   if (ifail[0] != 0)
+      dfl_subspace(mds);
+   else
+   {
+      int n;
+      for (n = 0; n < 3; n++)
+         for (int l = 0; l < D.Ns; l++)
+         {
+            // smooth_fields: n+1 Schwarz cycles on D psi = mds[l]
+            assign_s2s(vt, 2, mds[l], ws[0]);
+            set_s2zero(vt, 2, mds[l]);
+            float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
+            for (int j = 0; j <= n; j++) sap_device(mus, 1, D.nmr, p, r);
+         }
+      dfl_subspace(mds);
+      int nsolve = 0;
+      for (; n < D.ninv && ifail[0] == 0; n++)
+      {
+         if (set_Awhat(D.mu))
+         {
+            ifail[0] = -2;
+            break;
+         }
+         // dfl_smooth_fields: deflated inverse iteration of every mode
+         const size_t nw = (size_t)D.nb * D.Ns;
+         double2 *w = wvd(44 + 2 * D.nkv), *w2 = wvd(45 + 2 * D.nkv);
+         for (int l = 0; l < D.Ns; l++)
+         {
+            vconv<float2, double2>(D.d_vmds + (size_t)l * nw, w, nw);
+            int lf = 0, st[2] = {0, 0};
+            (void)ltl_solve(w, w2, &lf, st);
+            status[0] += st[0];
+            status[1] += st[1];
+            nsolve++;
+            float2 *p1 = (float2 *)device_field(ws[1], F32, &off);
+            vd2s_dev(w2, p1);
+            Dw(mus, ws[1], ws[0]);
+            diff_s2s(vt, 2, mds[l], ws[0]);   // ws0 = mds - D psi1
+            set_s2zero(vt, 2, mds[l]);
+            float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
+            for (int j = 0; j < D.ncy; j++) sap_device(mus, 1, D.nmr, p, r);
+            add_s2s(vt, 2, ws[1], mds[l]);
+         }
+         dfl_renormalize_modes(mds);
+         dfl_subspace(mds);
+      }
+      if (nsolve > 0)
+      {
+         status[0] = (status[0] + nsolve / 2) / nsolve;
+         status[1] = (status[1] + nsolve / 2) / nsolve;
+      }
+   }
+   release_ws();
+   release_ws();
+   // clean_up: restore the mass and the twisted-mass flag
+   if (L.m0 != m0_keep) oqh_set_sw_parms(m0_keep, L.csw);
+   oqh_set_tm_parms(eo_keep);
+   D.aw_ok = D.awhat_ok = false;
+}
+
+extern "C" void dfl_modes2(int *ifail, int *status)
+{
+   dfl_modes(ifail, status);
+   if (ifail[0] == -2)
+      dfl_modes(ifail + 1, status + 2);
+   else
+   {
+      ifail[1] = 0;
+      status[2] = status[3] = 0;
+   }
+}
+
+extern "C" double dfl_sap_gcr2(int nkv, int nmx, int istop, double res, double mu, spinor_dble *eta,
+                               spinor_dble *psi, int *ifail, int *status)
+{
+   Lib &L = lib();
+   const int vt = L.g.VT;
+   ifail[1] = 0;
+   for (int l = 3; l < 6; l++) status[l] = 0;
+   alloc_wsd(L.wsd_used + 1);
+   spinor_dble *rsd = reserve_wsd(1)[0];
+   assign_sd2sd(vt, 2, eta, rsd);
+   double rho = dfl_sap_gcr(nkv, nmx, istop, res, mu, rsd, psi, ifail, status);
+   if (ifail[0] == -1 || ifail[0] == -2)
+   {
+      dfl_modes(ifail + 1, status + 4);
+      if (ifail[1] == 0) rho = dfl_sap_gcr(nkv, nmx, istop, res, mu, rsd, psi, ifail + 1, status + 3);
+   }
+   release_wsd();
+   return rho;
+}
+
+// the whole deflation state, parameters included (oqh_finalize)
+void oqh::release_dfl()
+{
+   dfl_free_grid();
+   if (R.psi0) dev_free(R.psi0);
+   if (R.wtmp) dev_free(R.wtmp);
+   R = DflRun();
+   D = Dfl();
+}
+
+extern "C" void oqh_dfl_release(void) { release_dfl(); }

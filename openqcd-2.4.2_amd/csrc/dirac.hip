@@ -1083,6 +1083,37 @@ template <typename T> static void apply(int kind, T mu, void *sp, void *rp, cons
 
 using namespace oqh;
 
+// zero a field on the boundary time slices of the current boundary
+// conditions (time 0 and N0-1 for open bc), on the points of the set
+template <typename T> static void bnd_zero_set(ptset_t set, void *f, int prec, const char *where)
+{
+   Lib &L = lib();
+   if (!require_geometry(where)) return;
+   const int V = L.g.V, h = V / 2;
+   Res r = resolve(f, prec, V, 0, true);
+   if (!r.dev) return;
+   if (r.off)
+   {
+      fail(where, "the field must be a full field");
+      return;
+   }
+   typedef typename cplx_of<T>::t C;
+   if (set == ALL_PTS || set == EVEN_PTS) zero_bnd<T>((C *)r.dev, 0, h);
+   if (set == ALL_PTS || set == ODD_PTS) zero_bnd<T>((C *)r.dev, h, h);
+   finish(r, prec, V, true);
+}
+
+// lattice/bcnds.c:750 (bnd_sd2zero), bcnds.c:709 (bnd_s2zero)
+extern "C" void bnd_sd2zero(ptset_t set, spinor_dble *sd)
+{
+   bnd_zero_set<double>(set, sd, F64, "bnd_sd2zero [bcnds.c]");
+}
+
+extern "C" void bnd_s2zero(ptset_t set, spinor *s)
+{
+   bnd_zero_set<float>(set, s, F32, "bnd_s2zero [bcnds.c]");
+}
+
 // whether the operators rebuild row 2 of the links of the current gauge
 // field, and the largest deviation of a stored row 2 from the rebuilt one
 extern "C" int oqh_link_rebuild(double *dev)

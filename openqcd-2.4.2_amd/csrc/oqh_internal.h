@@ -152,6 +152,7 @@ void sap_release();  // frees the SAP block grid (made for the current geometry)
 void release_flow(); // frees the force field of the Wilson flow
 void release_nn();   // frees the network of the learned preconditioner
 void release_rng();  // frees the device random number streams
+void release_dfl();  // frees the deflation subspace and resets its parameters
 bool nn_ready();     // network created and enabled in sap_gcr
 bool nn_correct(const float2 *r, float2 *phi);  // phi += NN(r)
 

@@ -233,9 +233,34 @@ __global__ __launch_bounds__(RWG) void k_random_ud(RlxState *streams, int nthr, 
    store_lane(streams + k, R);
 }
 
-// random_sd: lane k fills points [k chunk, (k+1) chunk) with sigma times
-// Gaussian deviates (24 per point, in the spinor's component order)
-__global__ __launch_bounds__(RWG) void k_random_sd(RlxState *streams, int nthr, double2 *sd, int chunk, double sigma)
+// random_sd: lane k (stream k0 + k) fills the points [x0 + k chunk,
+// x0 + (k+1) chunk) with sigma times Gaussian deviates (24 per point, in the
+// spinor's component order)
+template <typename C>
+__global__ __launch_bounds__(RWG) void k_random_sf(RlxState *streams, int k0, int nthr, C *sd, int x0, int chunk,
+                                                   double sigma)
+{
+   __shared__ long long W[RWG][13][4];
+   __shared__ double RD[RWG][48];
+   const int k = blockIdx.x * RWG + threadIdx.x;
+   if (k >= nthr) return;
+   LaneRng R;
+   load_lane(streams + k0 + k, W[threadIdx.x], RD[threadIdx.x], R);
+   double r[24];
+   for (int x = x0 + k * chunk; x < x0 + (k + 1) * chunk; x++)
+   {
+      R.gauss(r, 24);
+      C *p = sd + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+      for (int c = 0; c < 12; c++) p[c * SB] = cmk<C>(sigma * r[2 * c], sigma * r[2 * c + 1]);
+   }
+   store_lane(streams + k0 + k, R);
+}
+
+// random source of main/ms4.c:383-406: lane k walks the ipt indices of
+// reference thread k in order and draws a Gaussian spinor (random_sd(1,0,..)
+// with its stream) at the points of time slice t0, zero elsewhere
+__global__ __launch_bounds__(RWG) void k_random_source(RlxState *streams, int nthr, double2 *sd, const int *x0c,
+                                                       int vt, int t0)
 {
    __shared__ long long W[RWG][13][4];
    __shared__ double RD[RWG][48];
@@ -244,11 +269,39 @@ __global__ __launch_bounds__(RWG) void k_random_sd(RlxState *streams, int nthr, 
    LaneRng R;
    load_lane(streams + k, W[threadIdx.x], RD[threadIdx.x], R);
    double r[24];
-   for (int x = k * chunk; x < (k + 1) * chunk; x++)
+   for (int ix = k * vt; ix < (k + 1) * vt; ix++)
    {
-      R.gauss(r, 24);
-      double2 *p = sd + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
-      for (int c = 0; c < 12; c++) p[c * SB] = make_double2(sigma * r[2 * c], sigma * r[2 * c + 1]);
+      // thread k of ms4.c walks the ipt indices [k VOLUME_TRD, (k+1) VOLUME_TRD)
+      double2 *p = sd + (size_t)(ix >> 6) * (12 * SB) + (ix & (SB - 1));
+      if (x0c[ix] == t0)
+      {
+         R.gauss(r, 24);
+         for (int c = 0; c < 12; c++) p[c * SB] = make_double2(r[2 * c], r[2 * c + 1]);
+      }
+      else
+         for (int c = 0; c < 12; c++) p[c * SB] = make_double2(0.0, 0.0);
+   }
+   store_lane(streams + k, R);
+}
+
+// uniform numbers in [-1/2, 1/2) on every real component of the points of
+// reference thread k (its VOLUME_TRD ipt indices), from stream k
+__global__ __launch_bounds__(RWG) void k_random_uniform(RlxState *streams, int nthr, float2 *f, int vt)
+{
+   __shared__ long long W[RWG][13][4];
+   __shared__ double RD[RWG][48];
+   const int k = blockIdx.x * RWG + threadIdx.x;
+   if (k >= nthr) return;
+   LaneRng R;
+   load_lane(streams + k, W[threadIdx.x], RD[threadIdx.x], R);
+   for (int x = k * vt; x < (k + 1) * vt; x++)
+   {
+      float2 *p = f + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+      for (int c = 0; c < 12; c++)
+      {
+         const double a = R.next(), b = R.next();
+         p[c * SB] = make_float2((float)a - 0.5f, (float)b - 0.5f);
+      }
    }
    store_lane(streams + k, R);
 }
@@ -278,6 +331,15 @@ bool streams_ready(const char *where)
 }
 
 }  // namespace
+
+// random start fields of the deflation subspace generation
+void random_uniform_s(float2 *f)
+{
+   Lib &L = lib();
+   if (!streams_ready("dfl_modes [dfl_modes.c]") || !f) return;
+   const int nthr = L.g.nthread;
+   k_random_uniform<<<(nthr + RWG - 1) / RWG, RWG, 0, L.stream>>>(d_streams, nthr, f, L.g.VT);
+}
 
 void release_rng()
 {
@@ -335,27 +397,54 @@ extern "C" void random_ud(void)
    L.swd_valid = L.sw_valid = false;
 }
 
-extern "C" void random_sd(int vol, spinor_dble *sd, double sigma)
+// random_sd (sflds/sflds.c:1254-1268): icom & 2: every
+// reference thread k fills vol points from s + k vol with its own stream;
+// otherwise the calling (master) thread fills vol points from s with stream 0
+template <typename C>
+static void random_field(int vol, int icom, void *s, double sigma, int prec, const char *where)
 {
-   const char *where = "random_sd [sflds.c]";
    Lib &L = lib();
    if (!streams_ready(where)) return;
-   const int nthr = L.g.nthread;
-   if (vol <= 0 || vol % nthr)
+   const int nthr = (icom & 0x2) ? L.g.nthread : 1;
+   const int n = vol * nthr;
+   if (vol <= 0)
    {
-      fail(where, "the volume must be a positive multiple of the number of threads");
+      fail(where, "Parameters are out of range");
       return;
    }
-   Res r = resolve(sd, F64, vol, 0, false);
+   Res r = resolve(s, prec, n, 0, false);
    if (!r.dev) return;
-   double2 *base = (double2 *)r.dev;
+   k_random_sf<C><<<(nthr + RWG - 1) / RWG, RWG, 0, L.stream>>>(d_streams, 0, nthr, (C *)r.dev, r.off, vol, sigma);
+   finish(r, prec, n, true);
+}
+
+extern "C" void random_sd(int vol, int icom, spinor_dble *sd, double sigma)
+{
+   random_field<double2>(vol, icom, sd, sigma, F64, "random_sd [sflds.c]");
+}
+
+// the random source of main/ms4.c:383-406 on time slice t0 (full field)
+extern "C" void oqh_random_source(int t0, spinor_dble *sd)
+{
+   const char *where = "oqh_random_source";
+   Lib &L = lib();
+   if (!streams_ready(where)) return;
+   if (t0 < 0 || t0 >= L.g.L[0])
+   {
+      fail(where, "the time slice is out of range");
+      return;
+   }
+   Res r = resolve(sd, F64, L.g.V, 0, false);
+   if (!r.dev) return;
    if (r.off)
    {
-      fail(where, "the field must start at a site block boundary");
+      fail(where, "the field must be a full field");
       return;
    }
-   k_random_sd<<<(nthr + RWG - 1) / RWG, RWG, 0, L.stream>>>(d_streams, nthr, base, vol / nthr, sigma);
-   finish(r, F64, vol, true);
+   const int nthr = L.g.nthread;
+   k_random_source<<<(nthr + RWG - 1) / RWG, RWG, 0, L.stream>>>(d_streams, nthr, (double2 *)r.dev, L.d_x0, L.g.VT,
+                                                                t0);
+   finish(r, F64, L.g.V, true);
 }
 
 extern "C" void ranlxd(double *r, int n)

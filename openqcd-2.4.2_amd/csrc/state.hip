@@ -471,6 +471,7 @@ extern "C" void oqh_finalize(void)
    L.ud_rec = false;
    release_nn();
    release_rng();
+   release_dfl();
    L.nn_prec = 0;
    L.sap_graph = 1;
    for (auto &kv : L.reg) dev_free(kv.second.base);

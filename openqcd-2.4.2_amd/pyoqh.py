@@ -67,7 +67,7 @@ _SIG = {
     "oqh_set_sap_graph": (None, [_I]),
     "oqh_sap_autotune": (_D, [_I, _I, _I, _I, P]),
     "start_ranlux": (None, [_I, _I]), "random_ud": (None, []),
-    "random_sd": (None, [_I, P, _D]), "ranlxd": (None, [P, _I]),
+    "random_sd": (None, [_I, _I, P, _D]), "oqh_random_source": (None, [_I, P]), "ranlxd": (None, [P, _I]),
     "sap_gcr_multi": (None, [_I, _I, _I, _I, _D, _D, P, P, P, P, P]),
     "oqh_nn_create": (_I, [_I, P]), "oqh_nn_nweights": (_I, []),
     "oqh_nn_set_weights": (None, [P]), "oqh_nn_get_weights": (None, [P]),
@@ -141,6 +141,7 @@ class Lattice:
         if self.lib.oqh_geometry(_p(a), _p(b)):
             raise RuntimeError("oqh_geometry failed")
         self.V = self.lib.oqh_volume()
+        self.VT = int(np.prod(self.Ltrd))   # VOLUME_TRD: points per reference thread
         self.lib.oqh_set_bc_parms(bc, cF[0], cF[1])
         self.lib.oqh_set_sw_parms(m0, csw)
         self.lib.oqh_set_tm_parms(eoflg)

@@ -50,6 +50,9 @@ CONFIGS = {
     "16x16x16x16_t8": ((16, 16, 16, 16), (8, 8, 8, 16), False),
     "12x12x12x12_t4": ((12, 12, 12, 12), (6, 6, 12, 12), False),
     "64x32x32x32_avx": ((64, 32, 32, 32), (16, 16, 16, 32), True),
+    # SAP+GCR CPU baselines: 8^4 (4^4 blocks need NTHREAD <= 8) and 48^4
+    "8x8x8x8_avx": ((8, 8, 8, 8), (8, 4, 4, 4), True),
+    "48x48x48x48_avx": ((48, 48, 48, 48), (24, 24, 24, 24), True),
 }
 
 

@@ -538,6 +538,37 @@ void oc_random_sd(oc_ctx *c, double *sd, double sigma)
       }
 }
 
+/* random_sd(vol, icom=0, sd, sigma) at the master thread: stream 0 fills
+   vol points from sd (sflds/sflds.c:1254-1268) */
+void oc_random_sd_master(oc_ctx *c, int vol, double *sd, double sigma)
+{
+   double r[24];
+   for (int ix = 0; ix < vol; ix++)
+   {
+      oc_gauss_internal(c, 0, r, 24);
+      for (int i = 0; i < 24; i++) sd[(size_t)24 * ix + i] = sigma * r[i];
+   }
+}
+
+/* random_source of main/ms4.c:383-406: thread k walks the ipt indices
+   [k VOLUME_TRD, (k+1) VOLUME_TRD) and draws random_sd(1,0,..) with its
+   stream at the points of global time t0, zero elsewhere */
+void oc_random_source(oc_ctx *c, int t0, double *sd)
+{
+   double r[24];
+   for (int k = 0; k < c->nthread; k++)
+      for (int ix = k * c->VT; ix < (k + 1) * c->VT; ix++)
+      {
+         if (c->x0[ix] == t0)
+         {
+            oc_gauss_internal(c, k, r, 24);
+            for (int i = 0; i < 24; i++) sd[(size_t)24 * ix + i] = r[i];
+         }
+         else
+            for (int i = 0; i < 24; i++) sd[(size_t)24 * ix + i] = 0.0;
+      }
+}
+
 void oc_random_s(oc_ctx *c, float *s, float sigma)
 {
    float r[24];

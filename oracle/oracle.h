@@ -71,6 +71,8 @@ double oc_plaq_sum(const oc_ctx *c, int iw);        /* plaq_sum_dble (iw=0), pla
 /* ---- spinor fields (sflds/sflds.c, sflds/unorm.c, lattice/bcnds.c) ---- */
 void oc_random_sd(oc_ctx *c, double *sd, double sigma);
 void oc_random_s(oc_ctx *c, float *s, float sigma);
+void oc_random_sd_master(oc_ctx *c, int vol, double *sd, double sigma);
+void oc_random_source(oc_ctx *c, int t0, double *sd);   /* main/ms4.c:383-406 */
 void oc_bnd_sd2zero(oc_ctx *c, int set, double *sd);
 void oc_bnd_s2zero(oc_ctx *c, int set, float *s);
 double oc_unorm_dble(int vol, const double *sd);

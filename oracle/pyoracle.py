@@ -42,6 +42,7 @@ def lib():
             "oc_udfld": (P, [P]), "oc_set_ud": (None, [P, P]),
             "oc_assign_ud2u": (None, [P]), "oc_ufld": (P, [P]),
             "oc_random_sd": (None, [P, P, d]), "oc_random_s": (None, [P, P, f]),
+            "oc_random_sd_master": (None, [P, i, P, d]), "oc_random_source": (None, [P, i, P]),
             "oc_bnd_sd2zero": (None, [P, i, P]), "oc_bnd_s2zero": (None, [P, i, P]),
             "oc_unorm_dble": (d, [i, P]), "oc_unorm": (f, [i, P]),
             "oc_ftensor": (None, [P, P]), "oc_sw_term": (i, [P, i]),
@@ -173,6 +174,18 @@ class Oracle:
     def random_sd(self, sigma=1.0):
         a = np.empty((self.V, 24), np.float64)
         lib().oc_random_sd(self.h, ptr(a), sigma)
+        return a
+
+    def random_sd_master(self, vol, sigma=1.0):
+        """random_sd(vol, 0, sd, sigma) called by the master thread."""
+        a = np.empty((vol, 24), np.float64)
+        lib().oc_random_sd_master(self.h, vol, ptr(a), sigma)
+        return a
+
+    def random_source(self, t0):
+        """The random time-slice source of main/ms4.c (random_source)."""
+        a = np.empty((self.V, 24), np.float64)
+        lib().oc_random_source(self.h, t0, ptr(a))
         return a
 
     def random_s(self, sigma=1.0):

@@ -10,6 +10,8 @@
  *   ref_<tag> bench [options]          time Dw_dble/Dw/Dwhat(_dble), print JSON
  *   ref_<tag> cnfg IN OUT [options]    import_cnfg(IN), print the plaquette
  *                                      sums, export_cnfg(OUT)
+ *   ref_<tag> gcr [options]            time sap_gcr on the reference's own
+ *                                      field and source, print JSON
  * Options: -bc T -cF a b -csw c -m0 m -seed s -mu mu -eoflg e
  *          -bs b0 b1 b2 b3 -isolv i -nmr n -ncy n -nkv n -nmx n -res r
  *          -secs t (bench: target seconds per operator)
@@ -71,7 +73,14 @@ int main(int argc, char *argv[])
    if (argc < 2) { fprintf(stderr, "usage: %s dump OUTDIR|bench [options]\n", argv[0]); return 1; }
    i = 2;
    const char *cnfg_in = NULL, *cnfg_out = NULL;
-   if (strcmp(argv[1], "dump") == 0)
+   int gcr_mode = 0;
+   if (strcmp(argv[1], "gcr") == 0)
+   {
+      /* SAP grid parameters are set below as in dump mode */
+      gcr_mode = 1;
+      dump = 0;
+   }
+   else if (strcmp(argv[1], "dump") == 0)
    {
       if (argc < 3) return 1;
       snprintf(outdir, sizeof(outdir), "%s", argv[2]);
@@ -114,7 +123,7 @@ int main(int argc, char *argv[])
    geometry();
    set_sw_parms(m0);
    set_tm_parms(eoflg);
-   if (dump && (((L0 / bs[0]) * (L1 / bs[1]) * (L2 / bs[2]) * (L3 / bs[3])) % (2 * NTHREAD)) == 0)
+   if ((dump || gcr_mode) && (((L0 / bs[0]) * (L1 / bs[1]) * (L2 / bs[2]) * (L3 / bs[3])) % (2 * NTHREAD)) == 0)
       set_sap_parms(bs, isolv, nmr, ncy);
 
    alloc_wsd(10);
@@ -133,6 +142,36 @@ int main(int argc, char *argv[])
    }
 
    random_ud();
+   if (gcr_mode)
+   {
+      /* ---------------- SAP+GCR timing (devel/sap/check3.c style) ------- */
+      double t1, t2, rho, rq;
+      int ifail = 0, status = 0;
+      qflt q0, q1;
+      if (nflow > 0) fwd_euler(nflow, eflow);
+      set_ud_phase();
+      if ((((L0 / bs[0]) * (L1 / bs[1]) * (L2 / bs[2]) * (L3 / bs[3])) % (2 * NTHREAD)) != 0)
+      {
+         printf("{\"error\": \"SAP block grid not allowed for NTHREAD=%d\"}\n", NTHREAD);
+         MPI_Finalize();
+         return 1;
+      }
+      random_sd(VOLUME_TRD, 2, wsd[0], 1.0);
+      bnd_sd2zero(ALL_PTS, wsd[0]);
+      t1 = wtime();
+      rho = sap_gcr(nkv, nmx, 0, res, mu, wsd[0], wsd[1], &ifail, &status);
+      t2 = wtime();
+      Dw_dble(mu, wsd[1], wsd[2]);
+      mulr_spinor_add_dble(VOLUME_TRD, 2, wsd[2], wsd[0], -1.0);
+      q1 = norm_square_dble(VOLUME_TRD, 3, wsd[2]);
+      q0 = norm_square_dble(VOLUME_TRD, 3, wsd[0]);
+      rq = sqrt(q1.q[0] / q0.q[0]);
+      printf("{\"nthread\": %d, \"volume\": %d, \"gcr_status\": %d, \"gcr_ifail\": %d, "
+             "\"gcr_rho\": %.6e, \"true_residual\": %.6e, \"gcr_sec\": %.6e}\n",
+             NTHREAD, VOLUME, status, ifail, rho, rq, t2 - t1);
+      MPI_Finalize();
+      return 0;
+   }
    if (dump && nflow > 0)
    {
       plaq_frc();

@@ -43,7 +43,7 @@ int main(int argc, char *argv[])
 
    alloc_wsd(3);
    spinor_dble **wsd = reserve_wsd(3);
-   random_sd(vol, wsd[0], 1.0);
+   random_sd(vol, 0, wsd[0], 1.0);
    double rho = sap_gcr(16, 256, 0, 1e-10, 0.0, wsd[0], wsd[1], &ifail, &status);
    Dw_dble(0.0, wsd[1], wsd[2]);
    mulr_spinor_add_dble(vol, 0, wsd[2], wsd[0], -1.0);

@@ -1,0 +1,75 @@
+"""Several right-hand sides at the configured size: the 32^3x64 operator
+workload's field -- open bc (type 0, cF=1), csw=1.77, m0=-0.3, the
+reference's random field (start_ranlux(0, 2), random_ud) smoothed on the
+device by 100 Wilson-flow steps of 0.01 -- with the 12 spin-colour point
+sources and 4 Gaussian sources (random_sd, zeroed on the boundary time
+slices) solved together by one sap_gcr_multi call (FP32 SAP on 4^4 blocks,
+isolv=1, nmr=4, ncy=5; FP64 FGCR, nkv=24, relative residual 1e-10).
+
+Open bc make the time slices 0 and N0-1 inactive (lattice/bcnds.c:41-54),
+so the point sources sit at x = (N0/2, 0, 0, 0), the middle of the time
+extent, rather than at the origin.  Every solution is judged like the
+reference's solver (devel/sap/check3.c): the oracle builds the SW term and
+recomputes |eta - Dw_dble psi| / |eta| on the exported field."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pyoqh
+import pyoracle as po
+
+P = C.c_void_p
+
+
+@pytest.mark.gpu
+def test_multi_rhs_32x32x32x64_open():
+    L, T = (64, 32, 32, 32), (8, 8, 8, 8)
+    lat = pyoqh.Lattice(L, T, bc=0, m0=-0.3, csw=1.77)
+    try:
+        lib = lat.lib
+        V = lat.V
+        print("device: random_ud of seed 2, 100 flow steps", flush=True)
+        lib.start_ranlux(0, 2)
+        lib.random_ud()
+        lib.fwd_euler(100, 0.01)
+        lib.bnd_sd2zero.argtypes = [C.c_int, P]
+        bs = np.array((4, 4, 4, 4), np.int32)
+        lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        n = 16
+        es, ps = [lat.alloc_sd() for _ in range(n)], [lat.alloc_sd() for _ in range(n)]
+        ix = int(lat.ipt()[(L[0] // 2) * L[1] * L[2] * L[3]])
+        srcs = []
+        for k in range(12):
+            e = np.zeros((V, 24))
+            e[ix, 2 * k] = 1.0
+            lat.put_sd(es[k], e)
+            srcs.append(e)
+        for k in range(12, 16):
+            lib.random_sd(lat.VT, 2, es[k], 1.0)
+            lib.bnd_sd2zero(0, es[k])
+            srcs.append(lat.get_sd(es[k]))
+        rho = np.zeros(n)
+        ifail = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        print("device: sap_gcr_multi, 16 sources", flush=True)
+        lib.sap_gcr_multi(n, 24, 1024, 0, 1e-10, 0.0, (P * n)(*es), (P * n)(*ps), rho.ctypes.data_as(P),
+                          ifail.ctypes.data_as(P), status.ctypes.data_as(P))
+        print("status", status.tolist(), "ifail", ifail.tolist(), flush=True)
+        assert (ifail == 0).all()
+        sols = [lat.get_sd(p) for p in ps]
+        ud = lat.export_ud()
+    finally:
+        lat.close()
+    print("oracle: SW term and Dw_dble at 32^3x64", flush=True)
+    o = po.Oracle(L, T, bc=0, m0=-0.3, csw=1.77)
+    o.set_ud(ud)
+    del ud
+    assert o.sw_term(3) == 0
+    worst = 0.0
+    for k, (eta, psi) in enumerate(zip(srcs, sols)):
+        res = np.linalg.norm(eta - o.Dw_dble(0.0, psi)) / np.linalg.norm(eta)
+        worst = max(worst, res)
+        print("source", k, "status", int(status[k]), "true residual", res, flush=True)
+        assert res <= 1.01e-10
+    print("largest true residual", worst)

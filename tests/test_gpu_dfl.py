@@ -1,0 +1,214 @@
+"""Deflation (dfl/*, little/*) on the GPU against the oracle's restatement
+(oracle/dfl.py, pinned by tests/test_oracle_dfl.py) on the reference's
+seeded 16x8^3 field with 4^4 deflation blocks (nb = 32) and Ns = 4 modes:
+
+* dfl_subspace: the basis equals the oracle's block Gram-Schmidt of the same
+  modes (FP32 storage: 2e-6 of the block norm);
+* dfl_s2vd / dfl_vd2s against the oracle's projections (1e-5 relative);
+* set_Aw: Ablk and Ahop against the little operator by its definition (the
+  oracle's Dw_dble on every basis vector extended by zero), 1e-11 relative;
+  faces that meet the same neighbouring block (two blocks per direction)
+  are compared as their sum;
+* set_Awhat / Awhat_dble against the dense even-odd operator (1e-11);
+* ltl_gcr: residual of the little equation (dense operator) at the
+  requested 1e-10, and the solution against a dense solve;
+* dfl_modes + dfl_sap_gcr (FP32 and FP16-storage Schwarz cycles) and
+  dfl_sap_gcr2: the double-precision true residual recomputed by the
+  oracle reaches 1e-10 (devel/sap/check3.c judges the reference so), with
+  fewer Krylov vectors than SAP+GCR without deflation."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import dfl as od
+import pyoqh
+from helpers import make_oracle
+
+P = C.c_void_p
+L, T, BS, NS = (16, 8, 8, 8), (8, 8, 8, 8), (4, 4, 4, 4), 4
+
+
+class DflParms(C.Structure):
+    _fields_ = [("bs", C.c_int * 4), ("Ns", C.c_int)]
+
+
+class DflProParms(C.Structure):
+    _fields_ = [("nkv", C.c_int), ("nmx", C.c_int), ("nmx_gcr", C.c_int), ("res", C.c_double),
+                ("res_gcr", C.c_double)]
+
+
+class DflGenParms(C.Structure):
+    _fields_ = [("ninv", C.c_int), ("nmr", C.c_int), ("ncy", C.c_int), ("kappa", C.c_double),
+                ("m0", C.c_double), ("mu", C.c_double)]
+
+
+def _sig(lib):
+    lib.set_dfl_parms.restype = DflParms
+    lib.set_dfl_parms.argtypes = [P, C.c_int]
+    lib.set_dfl_pro_parms.restype = DflProParms
+    lib.set_dfl_pro_parms.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double]
+    lib.set_dfl_gen_parms.restype = DflGenParms
+    lib.set_dfl_gen_parms.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    lib.dfl_subspace.argtypes = [P]
+    lib.dfl_s2vd.argtypes = [P, P]
+    lib.dfl_vd2s.argtypes = [P, P]
+    lib.set_Aw.argtypes = [C.c_double]
+    lib.set_Awhat.restype = C.c_int
+    lib.set_Awhat.argtypes = [C.c_double]
+    lib.Awhat_dble.argtypes = [P, P]
+    lib.ltl_gcr.restype = C.c_double
+    lib.ltl_gcr.argtypes = [C.c_double, P, P, P, P]
+    lib.dfl_modes.argtypes = [P, P]
+    for nm in ("dfl_sap_gcr", "dfl_sap_gcr2"):
+        getattr(lib, nm).restype = C.c_double
+        getattr(lib, nm).argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P, P, P]
+    lib.sap_gcr.restype = C.c_double
+    lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.oqh_dfl_export_basis.argtypes = [C.c_int, P]
+    lib.oqh_dfl_blocks.restype = C.c_int
+    lib.oqh_dfl_blocks.argtypes = [P, P]
+    lib.oqh_dfl_export_Aw.argtypes = [P, P]
+
+
+def _cz(a):
+    return np.ascontiguousarray(a, np.complex128)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    o, eta, s32 = make_oracle(L, T, bc=3, m0=-0.3, csw=1.77, seed=4)
+    o.sw_term(3)
+    lat = pyoqh.Lattice(L, T, bc=3, m0=-0.3, csw=1.77)
+    _sig(lat.lib)
+    lat.import_ud(o.ud())
+    assert lat.sw_term(3) == 0
+    yield o, eta, s32, lat
+    lat.lib.oqh_dfl_release()
+    lat.close()
+
+
+@pytest.mark.gpu
+def test_subspace_and_little_operator(setup):
+    o, eta, s32, lat = setup
+    lib, V = lat.lib, lat.V
+    bs = np.array(BS, np.int32)
+    p = lib.set_dfl_parms(bs.ctypes.data_as(P), NS)
+    assert list(p.bs) == list(BS) and p.Ns == NS
+    q = lib.set_dfl_pro_parms(16, 256, 1e-10, 24, 1e-2)
+    assert q.nkv == 16 and q.nmx_gcr == 24
+    rng = np.random.default_rng(12)
+    mds = [rng.standard_normal((V, 24)).astype(np.float32) for _ in range(NS)]
+    fm = [lat.alloc_s() for _ in range(NS)]
+    for f, m in zip(fm, mds):
+        lat.put_s(f, m)
+    lib.dfl_subspace((P * NS)(*fm))
+    nb = lib.oqh_dfl_blocks(None, None)
+    assert nb == 32
+    inn = np.empty((nb, 8), np.int32)
+    bpts = np.empty((nb, 256), np.int32)
+    lib.oqh_dfl_blocks(inn.ctypes.data_as(P), bpts.ctypes.data_as(P))
+    # the oracle's block Gram-Schmidt of the same modes
+    basis_o, vm_o = od.block_gram_schmidt([m.astype(np.float64) for m in mds], list(bpts))
+    basis_g = np.zeros((NS, V, 12), complex)
+    for k in range(NS):
+        b = np.empty((V, 24), np.float32)
+        lib.oqh_dfl_export_basis(k, b.ctypes.data_as(P))
+        basis_g[k] = od.cplx(b)
+    dev = np.abs(basis_g - basis_o).max()
+    print("basis max deviation", dev, flush=True)
+    assert dev < 2e-6 * np.abs(basis_o).max()
+    # projections
+    w = np.empty((nb, NS), np.complex128)
+    s = lat.alloc_s()
+    lat.put_s(s, s32)
+    lib.dfl_s2vd(s, w.ctypes.data_as(P))
+    w_o = od.s2vd(basis_g, list(bpts), s32.astype(np.float64))
+    assert np.abs(w - w_o).max() < 1e-5 * np.abs(w_o).max()
+    lib.dfl_vd2s(_cz(w_o).ctypes.data_as(P), s)
+    p_o = od.vd2s(basis_g, list(bpts), w_o)
+    assert np.abs(lat.get_s(s) - p_o).max() < 1e-5 * np.abs(p_o).max()
+    # the little operator against its definition
+    mu = 0.05
+    lib.set_Aw(mu)
+    nm = NS * NS
+    ablk = np.empty((nb, NS, NS), np.complex128)
+    ahop = np.empty((nb, 8, NS, NS), np.complex128)
+    lib.oqh_dfl_export_Aw(ablk.ctypes.data_as(P), ahop.ctypes.data_as(P))
+    A = od.little_operator(o, basis_g, list(bpts), mu)
+    scale = np.abs(A).max()
+    worst = 0.0
+    for n in range(nb):
+        worst = max(worst, np.abs(ablk[n] - A[n * NS:(n + 1) * NS, n * NS:(n + 1) * NS]).max())
+        for mu_ in range(4):
+            m0, m1 = inn[n, 2 * mu_], inn[n, 2 * mu_ + 1]
+            ref0 = A[n * NS:(n + 1) * NS, m0 * NS:(m0 + 1) * NS]
+            if m0 == m1:
+                worst = max(worst, np.abs(ahop[n, 2 * mu_] + ahop[n, 2 * mu_ + 1] - ref0).max())
+            else:
+                ref1 = A[n * NS:(n + 1) * NS, m1 * NS:(m1 + 1) * NS]
+                worst = max(worst, np.abs(ahop[n, 2 * mu_] - ref0).max(), np.abs(ahop[n, 2 * mu_ + 1] - ref1).max())
+    print("little operator max deviation / scale", worst / scale, flush=True)
+    assert worst < 1e-11 * scale
+    # even-odd form
+    assert lib.set_Awhat(mu) == 0
+    h = (nb // 2) * NS
+    Ah = od.ahat_dense(A, nb, NS)
+    v = _cz(rng.standard_normal(h) + 1j * rng.standard_normal(h))
+    out = np.empty(h, np.complex128)
+    lib.Awhat_dble(v.ctypes.data_as(P), out.ctypes.data_as(P))
+    assert np.abs(out - Ah @ v).max() < 1e-11 * np.abs(Ah @ v).max()
+    # the little solver
+    e = _cz(rng.standard_normal(nb * NS) + 1j * rng.standard_normal(nb * NS))
+    x = np.empty(nb * NS, np.complex128)
+    ifail = np.zeros(1, np.int32)
+    st = np.zeros(2, np.int32)
+    rho = lib.ltl_gcr(mu, e.ctypes.data_as(P), x.ctypes.data_as(P), ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    res = np.linalg.norm(A @ x - e) / np.linalg.norm(e)
+    xd = np.linalg.solve(A, e)
+    print("ltl_gcr ifail", ifail[0], "status", st.tolist(), "rho", rho, "residual", res,
+          "error", np.linalg.norm(x - xd) / np.linalg.norm(xd), flush=True)
+    assert ifail[0] == 0 and res < 1e-8
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    for f in fm + [s]:
+        lat.free(f)
+
+
+@pytest.mark.gpu
+def test_deflated_solver(setup):
+    o, eta, s32, lat = setup
+    lib, V = lat.lib, lat.V
+    bs = np.array((4, 4, 4, 4), np.int32)
+    lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+    kappa = 1.0 / (2.0 * (-0.3 + 4.0))
+    g = lib.set_dfl_gen_parms(kappa, 0.0, 6, 4, 5)
+    assert abs(g.m0 - (-0.3)) < 1e-12
+    ifail = np.zeros(2, np.int32)
+    st = np.zeros(6, np.int32)
+    lib.dfl_modes(ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    print("dfl_modes ifail", ifail[0], "status", st[:2].tolist(), flush=True)
+    assert ifail[0] == 0
+    e, p = lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(e, eta)
+    plain_if, plain_st = C.c_int(), C.c_int()
+    lib.sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, C.byref(plain_if), C.byref(plain_st))
+    counts = {}
+    for prec in (0, 1):
+        lib.oqh_set_sap_prec(prec)
+        lib.dfl_sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+        res = np.linalg.norm(eta - o.Dw_dble(0.0, lat.get_sd(p).copy())) / np.linalg.norm(eta)
+        print("prec", prec, "dfl_sap_gcr ifail", ifail[0], "status", st[:3].tolist(), "plain", plain_st.value,
+              "true residual", res, flush=True)
+        assert ifail[0] == 0 and res <= 1.01e-10
+        counts[prec] = int(st[0])
+    lib.oqh_set_sap_prec(0)
+    assert counts[0] < plain_st.value
+    assert np.array_equal(lat.get_sd(e), eta)
+    # dfl_sap_gcr2, in place (psi == eta)
+    lib.dfl_sap_gcr2(16, 256, 0, 1e-10, 0.0, e, e, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    res = np.linalg.norm(eta - o.Dw_dble(0.0, lat.get_sd(e).copy())) / np.linalg.norm(eta)
+    print("dfl_sap_gcr2 ifail", ifail.tolist(), "status", st.tolist(), "true residual", res, flush=True)
+    assert ifail[0] == 0 and ifail[1] == 0 and res <= 1.01e-10
+    lat.free(e)
+    lat.free(p)

@@ -71,7 +71,7 @@ def test_sap_gcr_48x48x48x48_true_residual():
         bs = np.array((4, 4, 4, 4), np.int32)
         lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
         e, p = lat.alloc_sd(), lat.alloc_sd()
-        lib.random_sd(V, e, 1.0)
+        lib.random_sd(lat.VT, 2, e, 1.0)
         eta = lat.get_sd(e)
         sols = {}
         for prec in (0, 1):

@@ -1,8 +1,9 @@
 """Batched Dirac operator (oqh_Dw_dble_multi, oqh_Dw_multi): each of the n
-fields gets exactly what Dw_dble / Dw gives it alone (the per-field arithmetic
+fields gets what Dw_dble / Dw gives it alone (the per-field arithmetic
 is the same; only the tile size differs, which changes the order in which the
 hopping terms of a point are added), for every batch size and both boundary
-conditions, on identical seeded fields."""
+conditions, on identical seeded fields; and the batch of 12 agrees with the
+oracle's Dw_dble and Dw point by point (1e-13, 1e-6)."""
 import ctypes as C
 
 import numpy as np
@@ -47,6 +48,19 @@ def test_multi_rhs_matches_single(bc):
                     single(0.05, ins[k], ref)
                     rel, absz = site_rel(get(outs[k]), get(ref))
                     assert rel < tol and absz == 0.0, (prec, n, k, rel)
-            print("bc", bc, "multi-RHS agrees with single-field operator")
+            # the batch of 12 against the oracle's Dw_dble / Dw per field
+            # (1e-13 and 1e-6 per point, as the single-field operators)
+            o.sw_term(3)
+            if not prec:
+                o.assign_ud2u()
+                o.assign_swd2sw()
+            for k in range(12):
+                if prec:
+                    oref = o.Dw_dble(0.05, src[k].copy())
+                else:
+                    oref = o.Dw(np.float32(0.05), src[k].astype(np.float32))
+                rel, absz = site_rel(get(outs[k]), oref)
+                assert rel < (1e-13 if prec else 1e-6) and absz == 0.0, (prec, k, rel)
+            print("bc", bc, "multi-RHS agrees with single-field operator and the oracle")
     finally:
         lat.close()

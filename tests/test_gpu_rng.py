@@ -58,10 +58,22 @@ def test_random_fields(c):
         print("random_ud max deviation", dev)
         assert dev < 1e-14
         s = lat.alloc_sd()
-        lat.lib.random_sd(lat.V, s, 1.0)
+        lat.lib.random_sd(lat.VT, 2, s, 1.0)
         dev = np.abs(lat.get_sd(s) - sd_o).max()
         print("random_sd max deviation", dev)
         assert dev < 1e-13
+        # the next draws: the random source of main/ms4.c on time slice 3,
+        # then random_sd(vol, 0, ..) of the master thread on 100 points
+        src_o = o.random_source(3)
+        lat.lib.oqh_random_source(3, s)
+        got = lat.get_sd(s)
+        assert np.abs(got - src_o).max() < 1e-13
+        lex = np.empty(lat.V, np.int64)
+        lex[o.ipt()] = np.arange(lat.V)
+        assert not got[lex // (c["L"][1] * c["L"][2] * c["L"][3]) != 3].any()
+        m_o = o.random_sd_master(100, 0.5)
+        lat.lib.random_sd(100, 0, s, 0.5)
+        assert np.abs(lat.get_sd(s)[:100] - m_o).max() < 1e-13
         # the operators accept the generated field (links SU(3))
         z = (ud_g[:, 0::2] + 1j * ud_g[:, 1::2]).reshape(-1, 3, 3)
         nz = np.abs(z).max(axis=(1, 2)) > 0
