@@ -229,19 +229,22 @@ def learned_held_out(wts):
 
 
 def multi_rhs():
-    """Several right-hand sides on one 32^3x64 field (periodic bc with the
-    quark phase, csw=1.77, m0=-0.35, random_ud links of seed 2 flowed by 50 steps
-    of 0.01): the 12 spin-colour point sources at the origin and 4 Gaussian
-    sources, solved together by sap_gcr_multi (FP32 SAP on 4^4 blocks,
-    isolv=1, nmr=4, ncy=5; FGCR nkv=16 to 1e-10), against one sap_gcr call
-    per source for two of them."""
+    """Several right-hand sides on the 32^3x64 operator workload's field
+    (open bc, cF=1, csw=1.77, random_ud links of seed 2 flowed by 100 steps
+    of 0.01) at m0=0.0 (below that the solver does not converge on this
+    field): the 12 spin-colour point sources at x = (32, 0, 0, 0) (open bc
+    make the time slices 0 and 63 inactive) and 4 Gaussian sources, solved
+    together by sap_gcr_multi (FP32 SAP on 4^4 blocks, isolv=1, nmr=4,
+    ncy=5; FGCR nkv=16 to 1e-10), against one sap_gcr call per source for
+    two of them."""
     import ctypes as C
     import numpy as np
     import pyoqh
     P = C.c_void_p
-    lat = pyoqh.Lattice((64, 32, 32, 32), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
-    rec = {"workload": "32^3x64 periodic, csw=1.77, m0=-0.35, flow 50 x 0.01, 12 point + 4 Gaussian "
-                       "sources, FP32 SAP 4^4 isolv=1 nmr=4 ncy=5, nkv=16, res 1e-10"}
+    lat = pyoqh.Lattice((64, 32, 32, 32), (8, 8, 8, 8), bc=0, m0=0.0, csw=1.77)
+    rec = {"workload": "32^3x64 open bc (cF=1), csw=1.77, m0=0.0, random_ud seed 2 flowed 100 x 0.01, "
+                       "12 point sources at (32,0,0,0) + 4 Gaussian sources, FP32 SAP 4^4 isolv=1 nmr=4 ncy=5, "
+                       "nkv=16, res 1e-10"}
     try:
         lib = lat.lib
         lib.sap_gcr.restype = C.c_double
@@ -250,18 +253,20 @@ def multi_rhs():
         V = lat.V
         lib.start_ranlux(0, 2)
         lib.random_ud()
-        lib.fwd_euler(50, 0.01)
-        lib.set_ud_phase()
+        lib.fwd_euler(100, 0.01)
         bs = np.array((4, 4, 4, 4), np.int32)
         lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+        lib.bnd_sd2zero.argtypes = [C.c_int, P]
         n = 16
         es, ps = [lat.alloc_sd() for _ in range(n)], [lat.alloc_sd() for _ in range(n)]
+        ix = int(lat.ipt()[32 * 32 * 32 * 32])
         for k in range(12):
             e = np.zeros((V, 24))
-            e[0, 2 * k] = 1.0
+            e[ix, 2 * k] = 1.0
             lat.put_sd(es[k], e)
         for k in range(12, 16):
             lib.random_sd(lat.VT, 2, es[k], 1.0)
+            lib.bnd_sd2zero(0, es[k])
         rho = np.zeros(n)
         ifail = np.zeros(n, np.int32)
         status = np.zeros(n, np.int32)

@@ -52,7 +52,7 @@ using namespace oqh;
 namespace {
 
 constexpr int DWG = 256;     // work-group of the block kernels
-constexpr int NS_MAX = 64;   // largest Ns (one lane per basis vector)
+constexpr int NS_MAX = 40;   // largest Ns (one lane per basis vector; Gauss-Jordan of Ns x 2Ns in LDS)
 
 struct Dfl
 {
@@ -1387,14 +1387,16 @@ float2 *work_field(float2 **p)
    return *p;
 }
 
-// the deflated preconditioner (dfl_sap_gcr.c Mop): psi0 = P_L solution of
-// the little equation, then ncy Schwarz cycles on the remaining residual;
-// phi = psi0 + SAP, chi = D phi
+// the deflated preconditioner (dfl_sap_gcr.c Mop): psi0 = the solution of
+// the little equation for the components of rho along the subspace, then ncy
+// Schwarz cycles on the residual r1 = rho - D psi0; phi = psi0 + SAP(r1),
+// chi = D phi.  r1 lives in a fixed work field, so that the FP16 variant's
+// captured Schwarz cycles are replayed for every Krylov vector.
 void dfl_mop_dev(const float2 *rho, float2 *phi, float2 *chi)
 {
    Lib &L = lib();
    const int vt = L.g.VT;
-   float2 *p0 = work_field(&R.psi0);
+   float2 *p0 = work_field(&R.psi0), *r1 = work_field(&R.wtmp);
    if (!R.vin) R.vin = wvd(42 + 2 * D.nkv);
    if (!R.vout) R.vout = wvd(43 + 2 * D.nkv);
    s2vd_dev(rho, R.vin);
@@ -1403,9 +1405,9 @@ void dfl_mop_dev(const float2 *rho, float2 *phi, float2 *chi)
    R.ltl[0] += st[0];
    R.ltl[1] += st[1];
    vd2s_dev(R.vout, p0);
-   Dw(R.mus, (spinor *)p0, (spinor *)chi);
-   diff_s2s(vt, 2, (spinor *)rho, (spinor *)chi);   // chi = rho - D psi0
-   if (L.sap_prec == 1 && sap_mixed_mop(R.mus, R.isolv, R.nmr, R.ncy, chi, phi, false))
+   Dw(R.mus, (spinor *)p0, (spinor *)r1);
+   diff_s2s(vt, 2, (spinor *)rho, (spinor *)r1);   // r1 = rho - D psi0
+   if (L.sap_prec == 1 && sap_mixed_mop(R.mus, R.isolv, R.nmr, R.ncy, r1, phi, false))
    {
       // FP16 Schwarz cycles from psi = 0; chi = D (phi + psi0) in FP32
       add_s2s(vt, 2, (spinor *)p0, (spinor *)phi);
@@ -1413,12 +1415,11 @@ void dfl_mop_dev(const float2 *rho, float2 *phi, float2 *chi)
       return;
    }
    set_s2zero(vt, 2, (spinor *)phi);
-   for (int n = 0; n < R.ncy; n++) sap_device(R.mus, R.isolv, R.nmr, phi, chi);
-   diff_s2s(vt, 2, (spinor *)rho, (spinor *)chi);   // chi = D phi
+   for (int n = 0; n < R.ncy; n++) sap_device(R.mus, R.isolv, R.nmr, phi, r1);
+   // r1 = rho - D psi0 - D phi_sap, so chi = rho - r1 = D (psi0 + phi_sap)
+   assign_s2s(vt, 2, (spinor *)r1, (spinor *)chi);
+   diff_s2s(vt, 2, (spinor *)rho, (spinor *)chi);
    add_s2s(vt, 2, (spinor *)p0, (spinor *)phi);
-   // chi = rho - (rho - D psi0 - D phi_sap) is D phi_sap; add D psi0
-   Dw(R.mus, (spinor *)p0, (spinor *)work_field(&R.wtmp));
-   add_s2s(vt, 2, (spinor *)R.wtmp, (spinor *)chi);
 }
 
 void dfl_Dop(spinor_dble *s, spinor_dble *r) { Dw_dble(R.mud, s, r); }

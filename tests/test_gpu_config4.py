@@ -1,14 +1,17 @@
 """Several right-hand sides at the configured size: the 32^3x64 operator
-workload's field -- open bc (type 0, cF=1), csw=1.77, m0=-0.3, the
-reference's random field (start_ranlux(0, 2), random_ud) smoothed on the
-device by 100 Wilson-flow steps of 0.01 -- with the 12 spin-colour point
+workload's field -- open bc (type 0, cF=1), csw=1.77, the reference's random
+field (start_ranlux(0, 2), random_ud) smoothed on the device by 100
+Wilson-flow steps of 0.01 -- with the 12 spin-colour point
 sources and 4 Gaussian sources (random_sd, zeroed on the boundary time
 slices) solved together by one sap_gcr_multi call (FP32 SAP on 4^4 blocks,
 isolv=1, nmr=4, ncy=5; FP64 FGCR, nkv=24, relative residual 1e-10).
 
 Open bc make the time slices 0 and N0-1 inactive (lattice/bcnds.c:41-54),
 so the point sources sit at x = (N0/2, 0, 0, 0), the middle of the time
-extent, rather than at the origin.  Every solution is judged like the
+extent, rather than at the origin.  The solves use m0 = 0.0: on this field
+the operator workload's m0 = -0.3 is below the point where SAP+GCR converges
+(the reference build on the same field at 16^4 fails to reach 1e-10 in 512
+Krylov vectors for m0 = -0.3, -0.2 and -0.1 and needs 22 at m0 = 0.0).  Every solution is judged like the
 reference's solver (devel/sap/check3.c): the oracle builds the SW term and
 recomputes |eta - Dw_dble psi| / |eta| on the exported field."""
 import ctypes as C
@@ -25,7 +28,7 @@ P = C.c_void_p
 @pytest.mark.gpu
 def test_multi_rhs_32x32x32x64_open():
     L, T = (64, 32, 32, 32), (8, 8, 8, 8)
-    lat = pyoqh.Lattice(L, T, bc=0, m0=-0.3, csw=1.77)
+    lat = pyoqh.Lattice(L, T, bc=0, m0=0.0, csw=1.77)
     try:
         lib = lat.lib
         V = lat.V
@@ -62,7 +65,7 @@ def test_multi_rhs_32x32x32x64_open():
     finally:
         lat.close()
     print("oracle: SW term and Dw_dble at 32^3x64", flush=True)
-    o = po.Oracle(L, T, bc=0, m0=-0.3, csw=1.77)
+    o = po.Oracle(L, T, bc=0, m0=0.0, csw=1.77)
     o.set_ud(ud)
     del ud
     assert o.sw_term(3) == 0
