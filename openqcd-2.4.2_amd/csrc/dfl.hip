@@ -1101,7 +1101,13 @@ extern "C" void oqh_dfl_export_basis(int k, spinor *host)
       fail(where, "k out of range");
       return;
    }
-   oqh_export_s((spinor *)(D.d_basis + (size_t)k * D.fu), host, L.g.V);
+   // the basis fields are internal: convert and copy them here
+   const size_t bytes = (size_t)L.g.V * 12 * sizeof(float2);
+   float2 *aos = (float2 *)tmp_buffer(bytes);
+   if (!aos) return;
+   blocks_to_aos_f32(D.d_basis + (size_t)k * D.fu, aos, L.g.V, 12, L.stream);
+   HIP_OK(hipMemcpyAsync(host, aos, bytes, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
 }
 
 // block structure: inn[nb][8] and the points of every block, bpts[nb][bvol]
@@ -1381,9 +1387,11 @@ struct DflRun
 
 DflRun R;
 
+// a field of the library's registry, so that the operators and the BLAS of
+// the C API accept it as a device field
 float2 *work_field(float2 **p)
 {
-   if (!*p) *p = (float2 *)dev_alloc(sizeof(float2) * units_for(lib().g.V, 12));
+   if (!*p) *p = (float2 *)oqh_alloc_s(lib().g.V);
    return *p;
 }
 
@@ -1644,8 +1652,8 @@ extern "C" double dfl_sap_gcr2(int nkv, int nmx, int istop, double res, double m
 void oqh::release_dfl()
 {
    dfl_free_grid();
-   if (R.psi0) dev_free(R.psi0);
-   if (R.wtmp) dev_free(R.wtmp);
+   if (R.psi0) oqh_free(R.psi0);
+   if (R.wtmp) oqh_free(R.wtmp);
    R = DflRun();
    D = Dfl();
 }
