@@ -93,21 +93,8 @@ struct Dfl
 
 Dfl D;
 
-void dfl_free_grid()
-{
-   dev_free(D.d_bpts); dev_free(D.d_inn); dev_free(D.d_basis); dev_free(D.d_vmds); dev_free(D.d_s64);
-   dev_free(D.d_Ablk); dev_free(D.d_Ahop); dev_free(D.d_AinvT); dev_free(D.d_MT); dev_free(D.d_MT32);
-   dev_free(D.d_unsafe); dev_free(D.d_red);
-   for (auto p : D.wvd) dev_free(p);
-   for (auto p : D.wv) dev_free(p);
-   if (D.h_red) (void)hipHostFree(D.h_red);
-   D.d_bpts = D.d_inn = nullptr;
-   D.d_basis = nullptr; D.d_vmds = nullptr; D.d_s64 = nullptr;
-   D.d_Ablk = D.d_Ahop = D.d_AinvT = D.d_MT = nullptr;
-   D.d_MT32 = nullptr; D.d_unsafe = nullptr; D.d_red = nullptr; D.h_red = nullptr;
-   D.wvd.clear(); D.wv.clear();
-   D.grid = D.basis_ok = D.aw_ok = D.awhat_ok = false;
-}
+void dfl_free_grid();
+
 
 // ---------------------------------------------------------------------------
 // the DFL block grid (dfl/dfl_geometry.c, block/blk_grid.c)
@@ -541,6 +528,7 @@ __global__ __launch_bounds__(64) void k_aw_invert(const double2 *Ablk, const dou
    double2 *A = lds;              // [Ns][2 Ns] augmented
    __shared__ int piv;
    __shared__ double fro[2];
+   __shared__ double2 colc[NS_MAX];   // column c of the rows being eliminated
    const int n = blockIdx.x, t = threadIdx.x;
    const size_t nm = (size_t)Ns * Ns;
    const int W = 2 * Ns;
@@ -593,13 +581,15 @@ __global__ __launch_bounds__(64) void k_aw_invert(const double2 *Ablk, const dou
       const double2 ip = make_double2(pv.x / pm, -pv.y / pm);
       __syncthreads();
       for (int e = t; e < W; e += 64) A[c * W + e] = cmul(ip, A[c * W + e]);
+      // the factors are read before any row is changed: a row of 2 Ns
+      // entries may span two passes of the lanes over the matrix
+      for (int r = t; r < Ns; r += 64) colc[r] = A[r * W + c];
       __syncthreads();
       for (int e = t; e < Ns * W; e += 64)
       {
          const int r = e / W, col = e % W;
          if (r == c) continue;
-         const double2 f = A[r * W + c];
-         A[e] = csub(A[e], cmul(f, A[c * W + col]));
+         A[e] = csub(A[e], cmul(colc[r], A[c * W + col]));
       }
    }
    __syncthreads();
@@ -641,27 +631,43 @@ __global__ __launch_bounds__(64) void k_aw_invert(const double2 *Ablk, const dou
 // ---------------------------------------------------------------------------
 
 // out[b][k] = sign * sum_ifc sum_l M[b0+b][ifc][k][l] in[inn[b0+b][ifc] - i0][l]
-// (+ add[b][k] if add); one work-group of 64 lanes per block, lane k < Ns
+// (+ add[b][k] if add); one work-group of 256 lanes per block: lane group g
+// (32 lanes for Ns <= 32, else 64) takes the faces g, g + 256/G, ..., lane k
+// of a group the output component k; the face sums are added in LDS in a
+// fixed order
+constexpr int AWH = 256;
 template <typename C>
-__global__ __launch_bounds__(64) void k_aw_hop(const C *MT, const int *inn, int Ns, int b0, int i0, const C *in,
-                                                const C *add, C *out, int neg)
+__global__ __launch_bounds__(AWH) void k_aw_hop(const C *MT, const int *inn, int Ns, int b0, int i0, const C *in,
+                                                 const C *add, C *out, int neg)
 {
-   const int b = blockIdx.x, k = threadIdx.x;
-   if (k >= Ns) return;
+   typedef decltype(C::x) R;
+   __shared__ C part[AWH];
+   const int G = (Ns <= 32) ? 32 : 64, nf = AWH / G;
+   const int b = blockIdx.x, g = threadIdx.x / G, k = threadIdx.x % G;
    const size_t nm = (size_t)Ns * Ns;
    const int n = b0 + b;
-   typedef decltype(C::x) R;
    R sx = 0, sy = 0;
-   for (int f = 0; f < 8; f++)
-   {
-      const C *M = MT + ((size_t)n * 8 + f) * nm;
-      const C *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
-      for (int l = 0; l < Ns; l++)
+   if (k < Ns)
+      for (int f = g; f < 8; f += nf)
       {
-         const C m = M[(size_t)l * Ns + k], x = v[l];
-         sx += m.x * x.x - m.y * x.y;
-         sy += m.x * x.y + m.y * x.x;
+         const C *M = MT + ((size_t)n * 8 + f) * nm;
+         const C *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
+         for (int l = 0; l < Ns; l++)
+         {
+            const C m = M[(size_t)l * Ns + k], x = v[l];
+            sx += m.x * x.x - m.y * x.y;
+            sy += m.x * x.y + m.y * x.x;
+         }
       }
+   part[threadIdx.x] = cmk<C>(sx, sy);
+   __syncthreads();
+   if (threadIdx.x >= Ns) return;
+   sx = 0;
+   sy = 0;
+   for (int q = 0; q < nf; q++)
+   {
+      sx += part[q * G + k].x;
+      sy += part[q * G + k].y;
    }
    if (neg)
    {
@@ -692,8 +698,8 @@ __global__ __launch_bounds__(64) void k_aw_diag(const double2 *AinvT, int Ns, in
 template <typename C> void ahat(const C *MT, const C *v, C *t, C *w)
 {
    Lib &L = lib();
-   k_aw_hop<C><<<D.nbh, 64, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
-   k_aw_hop<C><<<D.nbh, 64, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
+   k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
+   k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -794,20 +800,320 @@ float2 *wv(int k)
    return D.wv[k];
 }
 
+// ---------------------------------------------------------------------------
+// Gram-Schmidt of the little solvers with the coefficients kept on the device
+// ---------------------------------------------------------------------------
+//
+// The Krylov vectors of a little solver are stored contiguously, phi_0..
+// phi_{nmx-1} then chi_0..chi_{nmx-1}, ld complex numbers apart.  One step n
+// of the classical Gram-Schmidt process of gcr4v.c / fgcr4vd.c runs as
+//   k_lmdot + k_lsum:      a[n][l] = (chi_l, chi_n), l < n
+//   k_lmaxpy + k_lsum_bc:  chi_n -= sum_l a[n][l] chi_l; b_n = |chi_n|,
+//                          c_n = (chi_n, rho) / b_n
+//   k_lscale + k_lsum_bc:  chi_n /= b_n; rho -= c_n chi_n; |rho|^2
+// and the host reads the four numbers of step n once.  The sums are taken in
+// double precision in a fixed order (per lane, per work-group, then over the
+// work-groups), so the results do not depend on the launch timing.
+
+constexpr int LRB = 256;     // lanes of the little-vector kernels
+constexpr int LNBK = 512;    // largest number of work-groups of a reduction
+
+int lnbk(size_t nv)
+{
+   size_t n = (nv + 4 * LRB - 1) / (4 * LRB);
+   return (int)std::max<size_t>(1, std::min<size_t>(n, LNBK));
+}
+
+__device__ inline double2 wave_sum2(double2 v)
+{
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      v.x += __shfl_xor(v.x, m);
+      v.y += __shfl_xor(v.y, m);
+   }
+   return v;
+}
+
+// part[l * gridDim.x + blockIdx.x] = this work-group's share of (chi_l, chi_n)
+template <typename C>
+__global__ __launch_bounds__(LRB) void k_lmdot(const C *chi, size_t ld, int n, size_t nv, double2 *part)
+{
+   __shared__ double2 sh[LRB / 64];
+   const int l = blockIdx.y;
+   const C *cl = chi + (size_t)l * ld, *cn = chi + (size_t)n * ld;
+   double x = 0.0, y = 0.0;
+   for (size_t i = (size_t)blockIdx.x * LRB + threadIdx.x; i < nv; i += (size_t)gridDim.x * LRB)
+   {
+      const double ax = cl[i].x, ay = cl[i].y, bx = cn[i].x, by = cn[i].y;
+      x += ax * bx + ay * by;
+      y += ax * by - ay * bx;
+   }
+   const double2 s = wg_sum(make_double2(x, y), sh);
+   if (threadIdx.x == 0) part[(size_t)l * gridDim.x + blockIdx.x] = s;
+}
+
+// out[q] = sum_b part[q * nbk + b] for q < nq; one wave per q, fixed order
+__global__ __launch_bounds__(LRB) void k_lsum(const double2 *part, int nbk, int nq, double2 *out)
+{
+   const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+   for (int q = w; q < nq; q += LRB / 64)
+   {
+      double2 v = make_double2(0.0, 0.0);
+      for (int b = t; b < nbk; b += 64)
+      {
+         v.x += part[(size_t)q * nbk + b].x;
+         v.y += part[(size_t)q * nbk + b].y;
+      }
+      v = wave_sum2(v);
+      if (t == 0) out[q] = v;
+   }
+}
+
+// mode 0: bc[0] = |chi_n|, bc[1] + i bc[2] = (chi_n, rho) / |chi_n| from the
+// columns (chi_n, rho) and (|chi_n|^2, 0); mode 1: bc[3] = |rho|^2
+__global__ __launch_bounds__(64) void k_lsum_bc(const double2 *part, int nbk, int mode, double *bc)
+{
+   const int t = threadIdx.x;
+   double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+   for (int b = t; b < nbk; b += 64)
+   {
+      v0.x += part[b].x;
+      v0.y += part[b].y;
+      if (mode == 0) v1.x += part[nbk + b].x;
+   }
+   v0 = wave_sum2(v0);
+   v1 = wave_sum2(v1);
+   if (t != 0) return;
+   if (mode == 0)
+   {
+      const double bn = sqrt(v1.x);
+      bc[0] = bn;
+      bc[1] = (bn > 0.0) ? v0.x / bn : 0.0;
+      bc[2] = (bn > 0.0) ? v0.y / bn : 0.0;
+   }
+   else
+      bc[3] = v0.x;
+}
+
+// chi_n -= sum_{l<n} a[l] chi_l (in the precision of C), then this
+// work-group's shares of (chi_n, rho) and |chi_n|^2
+template <typename C>
+__global__ __launch_bounds__(LRB) void k_lmaxpy(C *chi, size_t ld, int n, size_t nv, const double2 *a, const C *rho,
+                                                double2 *part)
+{
+   typedef decltype(C::x) R;
+   __shared__ double2 sh[LRB / 64];
+   C *cn = chi + (size_t)n * ld;
+   double x = 0.0, y = 0.0, z = 0.0;
+   for (size_t i = (size_t)blockIdx.x * LRB + threadIdx.x; i < nv; i += (size_t)gridDim.x * LRB)
+   {
+      R vx = cn[i].x, vy = cn[i].y;
+      for (int l = 0; l < n; l++)
+      {
+         const R ar = (R)a[l].x, ai = (R)a[l].y;
+         const C w = chi[(size_t)l * ld + i];
+         vx -= ar * w.x - ai * w.y;
+         vy -= ar * w.y + ai * w.x;
+      }
+      cn[i] = cmk<C>(vx, vy);
+      const double px = vx, py = vy, rx = rho[i].x, ry = rho[i].y;
+      x += px * rx + py * ry;
+      y += px * ry - py * rx;
+      z += px * px + py * py;
+   }
+   const double2 s0 = wg_sum(make_double2(x, y), sh);
+   const double2 s1 = wg_sum(make_double2(z, 0.0), sh);
+   if (threadIdx.x == 0)
+   {
+      part[blockIdx.x] = s0;
+      part[gridDim.x + blockIdx.x] = s1;
+   }
+}
+
+// chi_n /= b_n, rho -= c_n chi_n (nothing if b_n = 0), shares of |rho|^2
+template <typename C>
+__global__ __launch_bounds__(LRB) void k_lscale(C *cn, C *rho, size_t nv, const double *bc, double2 *part)
+{
+   typedef decltype(C::x) R;
+   __shared__ double2 sh[LRB / 64];
+   const double bn = bc[0];
+   const R sc = (bn > 0.0) ? (R)(1.0 / bn) : (R)0, cr = (R)bc[1], ci = (R)bc[2];
+   double z = 0.0;
+   for (size_t i = (size_t)blockIdx.x * LRB + threadIdx.x; i < nv; i += (size_t)gridDim.x * LRB)
+   {
+      R rx = rho[i].x, ry = rho[i].y;
+      if (bn > 0.0)
+      {
+         const R vx = sc * cn[i].x, vy = sc * cn[i].y;
+         cn[i] = cmk<C>(vx, vy);
+         rx -= cr * vx - ci * vy;
+         ry -= cr * vy + ci * vx;
+         rho[i] = cmk<C>(rx, ry);
+      }
+      const double px = rx, py = ry;
+      z += px * px + py * py;
+   }
+   const double2 s = wg_sum(make_double2(z, 0.0), sh);
+   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x = (add ? x : 0) + sum_{l<m} c[l] phi_l
+template <typename C>
+__global__ __launch_bounds__(LRB) void k_lcomb(C *x, const C *phi, size_t ld, int m, size_t nv, const double2 *c,
+                                               int add)
+{
+   typedef decltype(C::x) R;
+   const size_t i = (size_t)blockIdx.x * LRB + threadIdx.x;
+   if (i >= nv) return;
+   R vx = 0, vy = 0;
+   if (add)
+   {
+      vx = x[i].x;
+      vy = x[i].y;
+   }
+   for (int l = 0; l < m; l++)
+   {
+      const R cr = (R)c[l].x, ci = (R)c[l].y;
+      const C w = phi[(size_t)l * ld + i];
+      vx += cr * w.x - ci * w.y;
+      vy += cr * w.y + ci * w.x;
+   }
+   x[i] = cmk<C>(vx, vy);
+}
+
+// one little solver's Krylov space and Gram-Schmidt state
+template <typename C> struct LKry
+{
+   C *v = nullptr;          // [2 nmx][ld]: phi_0.., then chi_0..
+   int nmx = 0;
+   size_t ld = 0, nv = 0;
+   double2 *a = nullptr;    // [nmx][nmx]: a[n * nmx + l] = (chi_l, chi_n)
+   double *bc = nullptr;    // [nmx][4]: |chi_n|, c_n, |rho|^2 after step n
+   double2 *c = nullptr;    // [nmx] coefficients of the solution
+   double2 *part = nullptr; // [nmx][LNBK] work-group shares
+   double *h = nullptr;     // pinned host copy of a, bc, then c
+   C *phi(int k) const { return v + (size_t)k * ld; }
+   C *chi(int k) const { return v + (size_t)(nmx + k) * ld; }
+};
+
+LKry<float2> K32;
+LKry<double2> K64;
+
+template <typename C> void lkry_free(LKry<C> &K)
+{
+   dev_free(K.v); dev_free(K.a); dev_free(K.bc); dev_free(K.c); dev_free(K.part);
+   if (K.h) (void)hipHostFree(K.h);
+   K = LKry<C>();
+}
+
+template <typename C> bool lkry_alloc(LKry<C> &K, int nmx, size_t nv)
+{
+   const size_t ld = (nv + 63) & ~(size_t)63;
+   if (K.v && K.nmx == nmx && K.ld == ld) return true;
+   lkry_free(K);
+   const int nm = std::max(nmx, 2);
+   K.v = (C *)dev_alloc(sizeof(C) * 2 * (size_t)nmx * ld);
+   K.a = (double2 *)dev_alloc(sizeof(double2) * (size_t)nmx * nmx);
+   K.bc = (double *)dev_alloc(sizeof(double) * 4 * (size_t)nmx);
+   K.c = (double2 *)dev_alloc(sizeof(double2) * (size_t)nmx);
+   K.part = (double2 *)dev_alloc(sizeof(double2) * (size_t)nm * LNBK);
+   HIP_OK(hipHostMalloc((void **)&K.h, sizeof(double) * (2 * (size_t)nmx * nmx + 6 * (size_t)nmx), 0));
+   if (!K.v || !K.a || !K.bc || !K.c || !K.part || !K.h)
+   {
+      lkry_free(K);
+      return false;
+   }
+   K.nmx = nmx;
+   K.ld = ld;
+   K.nv = nv;
+   return true;
+}
+
+void dfl_free_grid()
+{
+   dev_free(D.d_bpts); dev_free(D.d_inn); dev_free(D.d_basis); dev_free(D.d_vmds); dev_free(D.d_s64);
+   dev_free(D.d_Ablk); dev_free(D.d_Ahop); dev_free(D.d_AinvT); dev_free(D.d_MT); dev_free(D.d_MT32);
+   dev_free(D.d_unsafe); dev_free(D.d_red);
+   for (auto p : D.wvd) dev_free(p);
+   for (auto p : D.wv) dev_free(p);
+   if (D.h_red) (void)hipHostFree(D.h_red);
+   D.d_bpts = D.d_inn = nullptr;
+   D.d_basis = nullptr; D.d_vmds = nullptr; D.d_s64 = nullptr;
+   D.d_Ablk = D.d_Ahop = D.d_AinvT = D.d_MT = nullptr;
+   D.d_MT32 = nullptr; D.d_unsafe = nullptr; D.d_red = nullptr; D.h_red = nullptr;
+   D.wvd.clear(); D.wv.clear();
+   D.grid = D.basis_ok = D.aw_ok = D.awhat_ok = false;
+   lkry_free(K32);
+   lkry_free(K64);
+}
+
+// step n of the Gram-Schmidt process on chi_n (see above); returns false if
+// |chi_n| = 0, else the residual norm |rho| after the step in *rn
+template <typename C> bool lgs_step(LKry<C> &K, int n, C *rho, double *rn)
+{
+   Lib &L = lib();
+   const size_t nv = K.nv;
+   const int nbk = lnbk(nv);
+   double2 *acol = K.a + (size_t)n * K.nmx;
+   double *bcn = K.bc + 4 * (size_t)n;
+   if (n > 0)
+   {
+      k_lmdot<C><<<dim3(nbk, n), LRB, 0, L.stream>>>(K.chi(0), K.ld, n, nv, K.part);
+      k_lsum<<<1, LRB, 0, L.stream>>>(K.part, nbk, n, acol);
+   }
+   k_lmaxpy<C><<<nbk, LRB, 0, L.stream>>>(K.chi(0), K.ld, n, nv, acol, rho, K.part);
+   k_lsum_bc<<<1, 64, 0, L.stream>>>(K.part, nbk, 0, bcn);
+   k_lscale<C><<<nbk, LRB, 0, L.stream>>>(K.chi(n), rho, nv, bcn, K.part);
+   k_lsum_bc<<<1, 64, 0, L.stream>>>(K.part, nbk, 1, bcn);
+   HIP_OK(hipMemcpyAsync(K.h, bcn, sizeof(double) * 4, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   *rn = sqrt(K.h[3]);
+   return K.h[0] > 0.0;
+}
+
+// x = (add ? x : 0) + sum_{l<m} c'_l phi_l with c' from the back
+// substitution of gcr4v.c / fgcr4vd.c (update_psi)
+template <typename C> void lgs_solution(LKry<C> &K, int m, C *x, int add)
+{
+   Lib &L = lib();
+   if (m <= 0)
+   {
+      if (!add) HIP_OK(hipMemsetAsync(x, 0, sizeof(C) * K.nv, L.stream));
+      return;
+   }
+   const int nmx = K.nmx;
+   double *ha = K.h, *hb = K.h + 2 * (size_t)nmx * nmx, *hc = hb + 4 * (size_t)nmx;
+   HIP_OK(hipMemcpyAsync(ha, K.a, sizeof(double2) * (size_t)m * nmx, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipMemcpyAsync(hb, K.bc, sizeof(double) * 4 * (size_t)m, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   for (int l = m - 1; l >= 0; l--)
+   {
+      double zr = hb[4 * l + 1], zi = hb[4 * l + 2];
+      for (int i = l + 1; i < m; i++)
+      {
+         // (chi_l, chi_i) is stored in column i
+         const double ar = ha[2 * ((size_t)i * nmx + l)], ai = ha[2 * ((size_t)i * nmx + l) + 1];
+         const double cr = hc[2 * i], ci = hc[2 * i + 1];
+         zr -= ar * cr - ai * ci;
+         zi -= ar * ci + ai * cr;
+      }
+      hc[2 * l] = zr / hb[4 * l];
+      hc[2 * l + 1] = zi / hb[4 * l];
+   }
+   HIP_OK(hipMemcpyAsync(K.c, hc, sizeof(double2) * (size_t)m, hipMemcpyHostToDevice, L.stream));
+   k_lcomb<C><<<(unsigned)((K.nv + LRB - 1) / LRB), LRB, 0, L.stream>>>(x, K.phi(0), K.ld, m, K.nv, K.c, add);
+   // the pinned coefficients must stay until the copy has run
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
 // single-pass GCR in single precision for Ahat x = b (linsolv/gcr4v.c):
 // returns the number of Krylov vectors (or -1 if res was not reached in nmx)
 int gcr4v(int nmx, double res, const float2 *b, float2 *x)
 {
    const size_t nv = (size_t)D.nbh * D.Ns;
+   if (!lkry_alloc(K32, nmx, nv)) return -1;
    float2 *rho = wv(0), *t = wv(1);
-   std::vector<float2 *> phi(nmx), chi(nmx);
-   for (int k = 0; k < nmx; k++)
-   {
-      phi[k] = wv(2 + k);
-      chi[k] = wv(2 + nmx + k);
-   }
-   std::vector<double> bn(nmx);
-   std::vector<double> ar((size_t)nmx * nmx, 0.0), ai((size_t)nmx * nmx, 0.0), cr(nmx), ci(nmx);
    Lib &L = lib();
    HIP_OK(hipMemcpyAsync(rho, b, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
    double r0;
@@ -815,54 +1121,22 @@ int gcr4v(int nmx, double res, const float2 *b, float2 *x)
    r0 = sqrt(r0);
    double rn = r0;
    int n = 0, status = -1;
-   for (; n < nmx; n++)
-   {
-      ahat<float2>(D.d_MT32, rho, t, chi[n]);
-      HIP_OK(hipMemcpyAsync(phi[n], rho, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
-      // classical Gram-Schmidt of chi[n] against chi[0..n-1]
-      for (int l = 0; l < n; l++)
+   if (r0 > 0.0)
+      for (; n < nmx; n++)
       {
-         double re, im;
-         vdot<float2>(chi[l], chi[n], nv, &re, &im, nullptr);
-         ar[(size_t)l * nmx + n] = re;
-         ai[(size_t)l * nmx + n] = im;
+         ahat<float2>(D.d_MT32, rho, t, K32.chi(n));
+         HIP_OK(hipMemcpyAsync(K32.phi(n), rho, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
+         if (!lgs_step(K32, n, rho, &rn)) break;
+         if (rn <= res * r0)
+         {
+            n++;
+            status = n;
+            break;
+         }
       }
-      for (int l = 0; l < n; l++)
-         vaxpy<float2>(chi[n], chi[l], nv, -(float)ar[(size_t)l * nmx + n], -(float)ai[(size_t)l * nmx + n]);
-      double nn;
-      vdot<float2>(chi[n], chi[n], nv, nullptr, nullptr, &nn);
-      bn[n] = sqrt(nn);
-      if (bn[n] == 0.0) break;
-      k_vscale<float2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(chi[n], nv, 1.0 / bn[n]);
-      double re, im;
-      vdot<float2>(chi[n], rho, nv, &re, &im, nullptr);
-      cr[n] = re;
-      ci[n] = im;
-      vaxpy<float2>(rho, chi[n], nv, -re, -im);
-      vdot<float2>(rho, rho, nv, nullptr, nullptr, &rn);
-      rn = sqrt(rn);
-      if (rn <= res * r0)
-      {
-         n++;
-         status = n;
-         break;
-      }
-   }
-   // back substitution (gcr4v.c set_psi): x = sum c'_l phi_l
-   const int m = n;
-   for (int l = m - 1; l >= 0; l--)
-   {
-      double zr = cr[l], zi = ci[l];
-      for (int i = l + 1; i < m; i++)
-      {
-         zr -= ar[(size_t)l * nmx + i] * cr[i] - ai[(size_t)l * nmx + i] * ci[i];
-         zi -= ar[(size_t)l * nmx + i] * ci[i] + ai[(size_t)l * nmx + i] * cr[i];
-      }
-      cr[l] = zr / bn[l];
-      ci[l] = zi / bn[l];
-   }
-   HIP_OK(hipMemsetAsync(x, 0, sizeof(float2) * nv, L.stream));
-   for (int l = 0; l < m; l++) vaxpy<float2>(x, phi[l], nv, cr[l], ci[l]);
+   else
+      status = 0;
+   lgs_solution(K32, n, x, 0);
    return status;
 }
 
@@ -1247,7 +1521,7 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
    // u_o = Aoo^-1 eta_o, b = Aee^-1 eta_e - sum M_e u_o
    k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, D.nbh, eta + nv, u);
    k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, 0, eta, t);
-   k_aw_hop<double2><<<D.nbh, 64, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, 0, D.nbh, u, t, b, 1);
+   k_aw_hop<double2><<<D.nbh, AWH, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, 0, D.nbh, u, t, b, 1);
    double bn2;
    vdot<double2>(b, b, nv, nullptr, nullptr, &bn2);
    const double rn0 = sqrt(bn2);
@@ -1256,19 +1530,18 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
    if (rn0 > 0.0)
    {
       // FGCR (linsolv/fgcr4vd.c) with Mop = gcr4v in single precision
-      std::vector<double2 *> phi(nkv), chi(nkv);
-      for (int k = 0; k < nkv; k++)
+      if (!lkry_alloc(K64, nkv, nv))
       {
-         phi[k] = wvd(9 + k);
-         chi[k] = wvd(9 + nkv + k);
+         ifail[0] = -1;
+         return rn;
       }
-      float2 *b32 = wv(2 + 2 * D.nmx_gcr), *x32 = wv(3 + 2 * D.nmx_gcr);
-      std::vector<double> ar((size_t)nkv * nkv), ai((size_t)nkv * nkv), bn(nkv), cr(nkv), ci(nkv);
+      float2 *b32 = wv(2), *x32 = wv(3);
       HIP_OK(hipMemcpyAsync(r, b, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
       const double tol = D.res * rn0;
       while (rn > tol)
       {
          int k;
+         bool zero = false;
          for (k = 0;; k++)
          {
             // phi = (approximate Ahat^-1) r, chi = Ahat phi
@@ -1276,45 +1549,17 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
             const double rres = (rn * D.res_gcr < tol) ? tol / rn : D.res_gcr;
             const int st = gcr4v(D.nmx_gcr, rres, b32, x32);
             status[1] += (st < 0) ? D.nmx_gcr : st;
-            vconv<float2, double2>(x32, phi[k], nv);
-            ahat<double2>(D.d_MT, phi[k], t, chi[k]);
-            for (int l = 0; l < k; l++)
+            vconv<float2, double2>(x32, K64.phi(k), nv);
+            ahat<double2>(D.d_MT, K64.phi(k), t, K64.chi(k));
+            if (!lgs_step(K64, k, r, &rn))
             {
-               double re, im;
-               vdot<double2>(chi[l], chi[k], nv, &re, &im, nullptr);
-               ar[(size_t)l * nkv + k] = re;
-               ai[(size_t)l * nkv + k] = im;
-               vaxpy<double2>(chi[k], chi[l], nv, -re, -im);
+               zero = true;
+               break;
             }
-            double nn;
-            vdot<double2>(chi[k], chi[k], nv, nullptr, nullptr, &nn);
-            bn[k] = sqrt(nn);
-            if (bn[k] == 0.0) break;
-            k_vscale<double2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(chi[k], nv, 1.0 / bn[k]);
-            double re, im;
-            vdot<double2>(chi[k], r, nv, &re, &im, nullptr);
-            cr[k] = re;
-            ci[k] = im;
-            vaxpy<double2>(r, chi[k], nv, -re, -im);
-            double r2;
-            vdot<double2>(r, r, nv, nullptr, nullptr, &r2);
-            rn = sqrt(r2);
             status[0] += 1;
             if (rn <= tol || k == nkv - 1 || status[0] >= D.nmx) break;
          }
-         const int m = (bn[k] == 0.0) ? k : k + 1;
-         for (int l = m - 1; l >= 0; l--)
-         {
-            double zr = cr[l], zi = ci[l];
-            for (int i = l + 1; i < m; i++)
-            {
-               zr -= ar[(size_t)l * nkv + i] * cr[i] - ai[(size_t)l * nkv + i] * ci[i];
-               zi -= ar[(size_t)l * nkv + i] * ci[i] + ai[(size_t)l * nkv + i] * cr[i];
-            }
-            cr[l] = zr / bn[l];
-            ci[l] = zi / bn[l];
-         }
-         for (int l = 0; l < m; l++) vaxpy<double2>(x, phi[l], nv, cr[l], ci[l]);
+         lgs_solution(K64, zero ? k : k + 1, x, 1);
          // recompute the residual r = b - Ahat x
          ahat<double2>(D.d_MT, x, t, r);
          k_vscale<double2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(r, nv, -1.0);
@@ -1327,12 +1572,12 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
             ifail[0] = -1;
             break;
          }
-         if (bn[k] == 0.0) break;
+         if (zero) break;
       }
    }
    // psi_e = x, psi_o = u_o - sum M_o x_e
    HIP_OK(hipMemcpyAsync(psi, x, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
-   k_aw_hop<double2><<<D.nbh, 64, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, D.nbh, 0, x, u, psi + nv, 1);
+   k_aw_hop<double2><<<D.nbh, AWH, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, D.nbh, 0, x, u, psi + nv, 1);
    return rn;
 }
 

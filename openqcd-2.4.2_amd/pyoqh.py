@@ -42,6 +42,20 @@ class ComplexQflt(C.Structure):
     _fields_ = [("re", Qflt), ("im", Qflt)]
 
 
+class DflParms(C.Structure):
+    _fields_ = [("bs", C.c_int * 4), ("Ns", C.c_int)]
+
+
+class DflProParms(C.Structure):
+    _fields_ = [("nkv", C.c_int), ("nmx", C.c_int), ("nmx_gcr", C.c_int), ("res", C.c_double),
+                ("res_gcr", C.c_double)]
+
+
+class DflGenParms(C.Structure):
+    _fields_ = [("ninv", C.c_int), ("nmr", C.c_int), ("ncy", C.c_int), ("kappa", C.c_double),
+                ("m0", C.c_double), ("mu", C.c_double)]
+
+
 _SIG = {
     "oqh_init": (_I, [_I]), "oqh_finalize": (None, []),
     "oqh_set_error_mode": (None, [_I]), "oqh_last_error": (_I, [C.c_char_p, _I]),
@@ -76,6 +90,13 @@ _SIG = {
     "oqh_nn_free": (None, []),
     "oqh_Dw_dble_multi": (None, [_D, _I, P, P]), "oqh_Dw_multi": (None, [_F, _I, P, P]),
     "oqh_time_dirac_multi": (_D, [_I, _D, _I, P, P, _I]),
+    "set_dfl_parms": (DflParms, [P, _I]), "dfl_parms": (DflParms, []),
+    "set_dfl_pro_parms": (DflProParms, [_I, _I, _D, _I, _D]), "dfl_pro_parms": (DflProParms, []),
+    "set_dfl_gen_parms": (DflGenParms, [_D, _D, _I, _I, _I]), "dfl_gen_parms": (DflGenParms, []),
+    "dfl_modes": (None, [P, P]), "dfl_modes2": (None, [P, P]),
+    "dfl_sap_gcr": (_D, [_I, _I, _I, _D, _D, P, P, P, P]),
+    "dfl_sap_gcr2": (_D, [_I, _I, _I, _D, _D, P, P, P, P]),
+    "oqh_dfl_release": (None, []),
 }
 for _n in ("Dw_dble", "Dwee_dble", "Dwoo_dble", "Dwhat_dble"):
     _SIG[_n] = (None, [_D, P, P])

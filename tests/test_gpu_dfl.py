@@ -1,6 +1,9 @@
 """Deflation (dfl/*, little/*) on the GPU against the oracle's restatement
 (oracle/dfl.py, pinned by tests/test_oracle_dfl.py) on the reference's
-seeded 16x8^3 field with 4^4 deflation blocks (nb = 32) and Ns = 4 modes:
+seeded 16x8^3 field (50 flow steps, m0 = -0.5) with 4^4 deflation blocks
+(nb = 32) and Ns = 6 modes (Ns = 10 for the solver; with these values a
+row of the 2 Ns wide Gauss-Jordan matrix of set_Awhat spans two passes of
+the 64 lanes, the case that a wrong ordering of the elimination breaks):
 
 * dfl_subspace: the basis equals the oracle's block Gram-Schmidt of the same
   modes (FP32 storage: 2e-6 of the block norm);
@@ -23,10 +26,10 @@ import pytest
 
 import dfl as od
 import pyoqh
-from helpers import make_oracle
+import pyoracle as po
 
 P = C.c_void_p
-L, T, BS, NS = (16, 8, 8, 8), (8, 8, 8, 8), (4, 4, 4, 4), 4
+L, T, BS, NS, M0 = (16, 8, 8, 8), (8, 8, 8, 8), (4, 4, 4, 4), 6, -0.5
 
 
 class DflParms(C.Structure):
@@ -78,9 +81,18 @@ def _cz(a):
 
 @pytest.fixture(scope="module")
 def setup():
-    o, eta, s32 = make_oracle(L, T, bc=3, m0=-0.3, csw=1.77, seed=4)
+    # the reference's random field of seed 4 smoothed by 50 flow steps; at
+    # m0 = -0.5 the oracle's SAP+GCR (4^4 blocks, nkv 16) needs 68 Krylov
+    # vectors for 1e-10, so that the deflation has low modes to remove
+    o = po.Oracle(L, T, bc=3, m0=M0, csw=1.77)
+    o.start_ranlux(0, 4)
+    o.random_ud()
+    o.fwd_euler(50, 0.01)
+    o.set_ud_phase()
+    eta = o.random_sd()
+    s32 = o.random_s()
     o.sw_term(3)
-    lat = pyoqh.Lattice(L, T, bc=3, m0=-0.3, csw=1.77)
+    lat = pyoqh.Lattice(L, T, bc=3, m0=M0, csw=1.77)
     _sig(lat.lib)
     lat.import_ud(o.ud())
     assert lat.sw_term(3) == 0
@@ -181,11 +193,18 @@ def test_deflated_solver(setup):
     lib, V = lat.lib, lat.V
     bs = np.array((4, 4, 4, 4), np.int32)
     lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
-    kappa = 1.0 / (2.0 * (-0.3 + 4.0))
+    # a new subspace of 10 modes per block (the parameters are set once, so
+    # the deflation state is released first)
+    lib.oqh_dfl_release()
+    bsd = np.array(BS, np.int32)
+    assert lib.set_dfl_parms(bsd.ctypes.data_as(P), 10).Ns == 10
+    lib.set_dfl_pro_parms(16, 256, 1e-2, 24, 1e-2)
+    kappa = 1.0 / (2.0 * (M0 + 4.0))
     g = lib.set_dfl_gen_parms(kappa, 0.0, 6, 4, 5)
-    assert abs(g.m0 - (-0.3)) < 1e-12
+    assert abs(g.m0 - M0) < 1e-12
     ifail = np.zeros(2, np.int32)
     st = np.zeros(6, np.int32)
+    lib.start_ranlux(0, 1234)
     lib.dfl_modes(ifail.ctypes.data_as(P), st.ctypes.data_as(P))
     print("dfl_modes ifail", ifail[0], "status", st[:2].tolist(), flush=True)
     assert ifail[0] == 0
