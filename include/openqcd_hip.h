@@ -451,9 +451,12 @@ double fgcr(int vol, void (*Dop)(spinor_dble *s, spinor_dble *r),
    block was not safe, Frobenius condition >= 1e6); Awhat_dble: the even-odd
    operator on host arrays of nb/2 * Ns entries.  ltl_gcr (dfl/ltl_gcr.c):
    the little Dirac equation, FGCR in double precision preconditioned by
-   single-pass GCR in single precision (without the little-mode deflation of
-   little/ltl_modes.c).  dfl_modes / dfl_modes2 (dfl/dfl_modes.c): the
-   subspace by inverse iteration from random fields.  dfl_sap_gcr /
+   single-pass GCR in single precision, both deflated by the little modes
+   (little/ltl_modes.c, set by set_Awhat).  dfl_modes / dfl_modes2
+   (dfl/dfl_modes.c:423, 583): the subspace by inverse iteration from random
+   fields; dfl_update / dfl_update2 (dfl_modes.c:504, 604): nsm further
+   deflated steps on the modes restored by dfl_restore_modes
+   (dfl_subspace.c:279).  dfl_sap_gcr /
    dfl_sap_gcr2 (dfl/dfl_sap_gcr.c): the deflated SAP+GCR solver, status[0]
    the Krylov count, status[1], status[2] the average outer and inner counts
    of the little solver (dfl_sap_gcr2: ifail[2], status[6]).  With
@@ -475,6 +478,9 @@ void Awhat_dble(complex_dble *v, complex_dble *w);
 double ltl_gcr(double mu, complex_dble *eta, complex_dble *psi, int *ifail, int *status);
 void dfl_modes(int *ifail, int *status);
 void dfl_modes2(int *ifail, int *status);
+void dfl_restore_modes(spinor **mds);
+void dfl_update(int nsm, int *ifail, int *status);
+void dfl_update2(int nsm, int *ifail, int *status);
 double dfl_sap_gcr(int nkv, int nmx, int istop, double res, double mu,
                    spinor_dble *eta, spinor_dble *psi, int *ifail, int *status);
 double dfl_sap_gcr2(int nkv, int nmx, int istop, double res, double mu,

@@ -25,13 +25,12 @@
 // ([k] fastest) so that the 64 lanes of a matrix-vector product read rows
 // as whole lines; an FP32 copy serves the inner single-precision solver.
 //
-// Differences to the reference (documented where they matter): the little
-// solver is FGCR in double precision preconditioned by single-pass GCR in
-// single precision, as ltl_gcr.c, but without the additional deflation of the
-// little operator by the global modes (little/ltl_modes.c); the random start
-// fields of the subspace generation come from the double-precision RANLUX
-// streams (uniform in [-1/2, 1/2) like ranlxs minus 1/2) rather than from the
-// single-precision generator.
+// The little solver is FGCR in double precision preconditioned by
+// single-pass GCR in single precision, both deflated by the little modes, as
+// ltl_gcr.c with little/ltl_modes.c.  Difference to the reference: the random
+// start fields of the subspace generation come from the double-precision
+// RANLUX streams (uniform in [-1/2, 1/2) like ranlxs minus 1/2) rather than
+// from the single-precision generator.
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -89,6 +88,18 @@ struct Dfl
    std::vector<double2 *> wvd;
    std::vector<float2 *> wv;
    double *d_red = nullptr, *h_red = nullptr;  // reductions
+   // little modes (little/ltl_modes.c): Ns orthonormal vectors w_l on the
+   // even blocks, Ahat w_l, both in FP64 and FP32, and M = (w, Ahat w)^-1
+   bool lm_ok = false;
+   size_t ldv = 0;
+   double2 *d_lw = nullptr, *d_law = nullptr;
+   float2 *d_lw32 = nullptr, *d_law32 = nullptr;
+   double2 *d_lM = nullptr;     // [Ns][Ns] row-major
+   double2 *d_lc = nullptr;     // [Ns] products, then [2][Ns] coefficients d, -d
+   double2 *d_ld = nullptr;
+   double2 *d_lB = nullptr;     // [Ns][Ns] (w_l, Ahat w_j) at column j
+   double2 *d_lpart = nullptr;  // [Ns][LNBK] work-group shares
+   double *d_lbc = nullptr;     // [Ns][4] Gram-Schmidt norms
 };
 
 Dfl D;
@@ -835,13 +846,13 @@ __device__ inline double2 wave_sum2(double2 v)
    return v;
 }
 
-// part[l * gridDim.x + blockIdx.x] = this work-group's share of (chi_l, chi_n)
+// part[l * gridDim.x + blockIdx.x] = this work-group's share of (X_l, y)
 template <typename C>
-__global__ __launch_bounds__(LRB) void k_lmdot(const C *chi, size_t ld, int n, size_t nv, double2 *part)
+__global__ __launch_bounds__(LRB) void k_lmdot(const C *X, size_t ld, const C *yv, size_t nv, double2 *part)
 {
    __shared__ double2 sh[LRB / 64];
    const int l = blockIdx.y;
-   const C *cl = chi + (size_t)l * ld, *cn = chi + (size_t)n * ld;
+   const C *cl = X + (size_t)l * ld, *cn = yv;
    double x = 0.0, y = 0.0;
    for (size_t i = (size_t)blockIdx.x * LRB + threadIdx.x; i < nv; i += (size_t)gridDim.x * LRB)
    {
@@ -1046,6 +1057,11 @@ void dfl_free_grid()
    D.grid = D.basis_ok = D.aw_ok = D.awhat_ok = false;
    lkry_free(K32);
    lkry_free(K64);
+   dev_free(D.d_lw); dev_free(D.d_law); dev_free(D.d_lw32); dev_free(D.d_law32); dev_free(D.d_lM);
+   dev_free(D.d_lc); dev_free(D.d_ld); dev_free(D.d_lB); dev_free(D.d_lpart); dev_free(D.d_lbc);
+   D.d_lw = D.d_law = nullptr; D.d_lw32 = D.d_law32 = nullptr; D.d_lM = D.d_lc = D.d_ld = D.d_lB = nullptr;
+   D.d_lpart = nullptr; D.d_lbc = nullptr;
+   D.lm_ok = false;
 }
 
 // step n of the Gram-Schmidt process on chi_n (see above); returns false if
@@ -1059,7 +1075,7 @@ template <typename C> bool lgs_step(LKry<C> &K, int n, C *rho, double *rn)
    double *bcn = K.bc + 4 * (size_t)n;
    if (n > 0)
    {
-      k_lmdot<C><<<dim3(nbk, n), LRB, 0, L.stream>>>(K.chi(0), K.ld, n, nv, K.part);
+      k_lmdot<C><<<dim3(nbk, n), LRB, 0, L.stream>>>(K.chi(0), K.ld, K.chi(n), nv, K.part);
       k_lsum<<<1, LRB, 0, L.stream>>>(K.part, nbk, n, acol);
    }
    k_lmaxpy<C><<<nbk, LRB, 0, L.stream>>>(K.chi(0), K.ld, n, nv, acol, rho, K.part);
@@ -1107,6 +1123,201 @@ template <typename C> void lgs_solution(LKry<C> &K, int m, C *x, int add)
    HIP_OK(hipStreamSynchronize(L.stream));
 }
 
+// ---------------------------------------------------------------------------
+// deflation of the little equation by the little modes (little/ltl_modes.c)
+// ---------------------------------------------------------------------------
+//
+// The little modes w_l are the even-block parts of the coordinates of the Ns
+// generating modes in the subspace basis (dfl_subspace), orthonormalised by
+// classical Gram-Schmidt.  With P0 the projector to their span and
+// M = (P0 Ahat P0)^-1 (the Ns x Ns matrix (w_l, Ahat w_j)^-1), the little
+// solver works with P_L = 1 - Ahat P0 M P0, which removes the lowest modes of
+// Ahat from the Krylov iteration.  All coefficients stay on the device.
+
+// d[k] = sign * sum_l M[k][l] c[l], and d[Ns + k] = -d[k]
+__global__ __launch_bounds__(64) void k_lmatvec(const double2 *M, int Ns, const double2 *c, double sign, double2 *d)
+{
+   const int k = threadIdx.x;
+   if (k >= Ns) return;
+   double2 s = make_double2(0.0, 0.0);
+   for (int l = 0; l < Ns; l++) s = cadd(s, cmul(M[(size_t)k * Ns + l], c[l]));
+   d[k] = make_double2(sign * s.x, sign * s.y);
+   d[Ns + k] = make_double2(-sign * s.x, -sign * s.y);
+}
+
+// v /= bc[0] (bc[0] > 0)
+template <typename C> __global__ __launch_bounds__(LRB) void k_lnorm(C *v, size_t nv, const double *bc)
+{
+   typedef decltype(C::x) R;
+   const size_t i = (size_t)blockIdx.x * LRB + threadIdx.x;
+   if (i >= nv || !(bc[0] > 0.0)) return;
+   const R f = (R)(1.0 / bc[0]);
+   v[i] = cmk<C>(f * v[i].x, f * v[i].y);
+}
+
+// D.d_ld = sign * M (X, v), and its negative at D.d_ld + Ns
+template <typename C> void lm_coef(const C *X, const C *v, double sign)
+{
+   Lib &L = lib();
+   const size_t nv = (size_t)D.nbh * D.Ns;
+   const int nbk = lnbk(nv);
+   k_lmdot<C><<<dim3(nbk, D.Ns), LRB, 0, L.stream>>>(X, D.ldv, v, nv, D.d_lpart);
+   k_lsum<<<1, LRB, 0, L.stream>>>(D.d_lpart, nbk, D.Ns, D.d_lc);
+   k_lmatvec<<<1, 64, 0, L.stream>>>(D.d_lM, D.Ns, D.d_lc, sign, D.d_ld);
+}
+
+template <typename C> void lm_comb(C *x, const C *Y, const double2 *c, int add)
+{
+   const size_t nv = (size_t)D.nbh * D.Ns;
+   k_lcomb<C><<<(unsigned)((nv + LRB - 1) / LRB), LRB, 0, lib().stream>>>(x, Y, D.ldv, D.Ns, nv, c, add);
+}
+
+// vd = P_L vd (dfl_Lvd)
+void dfl_Lvd_dev(double2 *vd)
+{
+   lm_coef<double2>(D.d_lw, vd, 1.0);
+   lm_comb<double2>(vd, D.d_law, D.d_ld + D.Ns, 1);
+}
+
+// wd = P0 M P0 vd, vd = P_L vd (dfl_LRvd)
+void dfl_LRvd_dev(double2 *vd, double2 *wd)
+{
+   lm_coef<double2>(D.d_lw, vd, 1.0);
+   lm_comb<double2>(wd, D.d_lw, D.d_ld, 0);
+   lm_comb<double2>(vd, D.d_law, D.d_ld + D.Ns, 1);
+}
+
+// vd -= P0 M P0 wd, wd = P_L wd (dfl_RLvd)
+void dfl_RLvd_dev(double2 *vd, double2 *wd)
+{
+   lm_coef<double2>(D.d_lw, wd, -1.0);
+   lm_comb<double2>(wd, D.d_law, D.d_ld, 1);
+   lm_comb<double2>(vd, D.d_lw, D.d_ld, 1);
+}
+
+// v = P_L v in single precision (dfl_Lv)
+void dfl_Lv_dev(float2 *v)
+{
+   lm_coef<float2>(D.d_lw32, v, 1.0);
+   lm_comb<float2>(v, D.d_law32, D.d_ld + D.Ns, 1);
+}
+
+inline double2 hmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+inline double2 hsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// the little modes, Ahat on them and M (set_ltl_modes); returns 1 if the
+// inversion of (w, Ahat w) is not safe (Frobenius condition number >= 1e6)
+// or the modes are degenerate, else 0
+int set_ltl_modes()
+{
+   Lib &L = lib();
+   const int Ns = D.Ns;
+   const size_t nv = (size_t)D.nbh * Ns;
+   const size_t ldv = (nv + 63) & ~(size_t)63;
+   const int nbk = lnbk(nv);
+   if (!D.d_lw || D.ldv != ldv)
+   {
+      dev_free(D.d_lw); dev_free(D.d_law); dev_free(D.d_lw32); dev_free(D.d_law32); dev_free(D.d_lM);
+      dev_free(D.d_lc); dev_free(D.d_ld); dev_free(D.d_lB); dev_free(D.d_lpart); dev_free(D.d_lbc);
+      D.ldv = ldv;
+      D.d_lw = (double2 *)dev_alloc(sizeof(double2) * ldv * Ns);
+      D.d_law = (double2 *)dev_alloc(sizeof(double2) * ldv * Ns);
+      D.d_lw32 = (float2 *)dev_alloc(sizeof(float2) * ldv * Ns);
+      D.d_law32 = (float2 *)dev_alloc(sizeof(float2) * ldv * Ns);
+      D.d_lM = (double2 *)dev_alloc(sizeof(double2) * Ns * Ns);
+      D.d_lc = (double2 *)dev_alloc(sizeof(double2) * Ns);
+      D.d_ld = (double2 *)dev_alloc(sizeof(double2) * 2 * Ns);
+      D.d_lB = (double2 *)dev_alloc(sizeof(double2) * Ns * Ns);
+      D.d_lpart = (double2 *)dev_alloc(sizeof(double2) * (size_t)std::max(Ns, 2) * LNBK);
+      D.d_lbc = (double *)dev_alloc(sizeof(double) * 4 * Ns);
+      if (!D.d_lw || !D.d_law || !D.d_lw32 || !D.d_law32 || !D.d_lM || !D.d_lc || !D.d_ld || !D.d_lB ||
+          !D.d_lpart || !D.d_lbc)
+         return 1;
+   }
+   D.lm_ok = false;
+   // the modes' coordinates on the even blocks, Gram-Schmidt (vs2vds)
+   double2 *acol = D.d_lB;
+   for (int l = 0; l < Ns; l++)
+   {
+      double2 *wl = D.d_lw + (size_t)l * ldv;
+      vconv<float2, double2>(D.d_vmds + (size_t)l * D.nb * Ns, wl, nv);
+      if (l > 0)
+      {
+         k_lmdot<double2><<<dim3(nbk, l), LRB, 0, L.stream>>>(D.d_lw, ldv, wl, nv, D.d_lpart);
+         k_lsum<<<1, LRB, 0, L.stream>>>(D.d_lpart, nbk, l, acol);
+      }
+      k_lmaxpy<double2><<<nbk, LRB, 0, L.stream>>>(D.d_lw, ldv, l, nv, acol, wl, D.d_lpart);
+      k_lsum_bc<<<1, 64, 0, L.stream>>>(D.d_lpart, nbk, 0, D.d_lbc + 4 * l);
+      k_lnorm<double2><<<(unsigned)((nv + LRB - 1) / LRB), LRB, 0, L.stream>>>(wl, nv, D.d_lbc + 4 * l);
+   }
+   // Ahat w_l (vds2Avds) and the FP32 copies
+   double2 *t = wvd(8);
+   for (int l = 0; l < Ns; l++)
+   {
+      ahat<double2>(D.d_MT, D.d_lw + (size_t)l * ldv, t, D.d_law + (size_t)l * ldv);
+      vconv<double2, float2>(D.d_lw + (size_t)l * ldv, D.d_lw32 + (size_t)l * ldv, nv);
+      vconv<double2, float2>(D.d_law + (size_t)l * ldv, D.d_law32 + (size_t)l * ldv, nv);
+   }
+   // B[l][j] = (w_l, Ahat w_j), stored by columns (set_Ads)
+   for (int j = 0; j < Ns; j++)
+   {
+      k_lmdot<double2><<<dim3(nbk, Ns), LRB, 0, L.stream>>>(D.d_lw, ldv, D.d_law + (size_t)j * ldv, nv, D.d_lpart);
+      k_lsum<<<1, LRB, 0, L.stream>>>(D.d_lpart, nbk, Ns, D.d_lB + (size_t)j * Ns);
+   }
+   std::vector<double2> Bc((size_t)Ns * Ns);
+   std::vector<double> bc(4 * (size_t)Ns);
+   HIP_OK(hipMemcpyAsync(Bc.data(), D.d_lB, sizeof(double2) * Ns * Ns, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipMemcpyAsync(bc.data(), D.d_lbc, sizeof(double) * 4 * Ns, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   for (int l = 0; l < Ns; l++)
+      if (!(bc[4 * l] > 0.0)) return 1;
+   // M = B^-1: Gauss-Jordan with partial pivoting in double precision
+   const int W = 2 * Ns;
+   std::vector<double2> A((size_t)Ns * W);
+   double f0 = 0.0, f1 = 0.0;
+   for (int r = 0; r < Ns; r++)
+      for (int c = 0; c < W; c++)
+      {
+         A[(size_t)r * W + c] = (c < Ns) ? Bc[(size_t)c * Ns + r] : make_double2(c - Ns == r ? 1.0 : 0.0, 0.0);
+         if (c < Ns) f0 += A[(size_t)r * W + c].x * A[(size_t)r * W + c].x + A[(size_t)r * W + c].y * A[(size_t)r * W + c].y;
+      }
+   for (int c = 0; c < Ns; c++)
+   {
+      int p = c;
+      double best = -1.0;
+      for (int r = c; r < Ns; r++)
+      {
+         const double m = A[(size_t)r * W + c].x * A[(size_t)r * W + c].x + A[(size_t)r * W + c].y * A[(size_t)r * W + c].y;
+         if (m > best) { best = m; p = r; }
+      }
+      if (!(best > 0.0)) return 1;
+      if (p != c)
+         for (int e = 0; e < W; e++) std::swap(A[(size_t)c * W + e], A[(size_t)p * W + e]);
+      const double2 pv = A[(size_t)c * W + c];
+      const double pm = pv.x * pv.x + pv.y * pv.y;
+      const double2 ip = make_double2(pv.x / pm, -pv.y / pm);
+      for (int e = 0; e < W; e++) A[(size_t)c * W + e] = hmul(ip, A[(size_t)c * W + e]);
+      for (int r = 0; r < Ns; r++)
+      {
+         if (r == c) continue;
+         const double2 fr = A[(size_t)r * W + c];
+         for (int e = 0; e < W; e++) A[(size_t)r * W + e] = hsub(A[(size_t)r * W + e], hmul(fr, A[(size_t)c * W + e]));
+      }
+   }
+   std::vector<double2> Mi((size_t)Ns * Ns);
+   for (int r = 0; r < Ns; r++)
+      for (int c = 0; c < Ns; c++)
+      {
+         Mi[(size_t)r * Ns + c] = A[(size_t)r * W + Ns + c];
+         f1 += Mi[(size_t)r * Ns + c].x * Mi[(size_t)r * Ns + c].x + Mi[(size_t)r * Ns + c].y * Mi[(size_t)r * Ns + c].y;
+      }
+   if (!(sqrt(f0 * f1) < 1.0e6)) return 1;
+   HIP_OK(hipMemcpyAsync(D.d_lM, Mi.data(), sizeof(double2) * Ns * Ns, hipMemcpyHostToDevice, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
+   D.lm_ok = true;
+   return 0;
+}
+
 // single-pass GCR in single precision for Ahat x = b (linsolv/gcr4v.c):
 // returns the number of Krylov vectors (or -1 if res was not reached in nmx)
 int gcr4v(int nmx, double res, const float2 *b, float2 *x)
@@ -1125,6 +1336,7 @@ int gcr4v(int nmx, double res, const float2 *b, float2 *x)
       for (; n < nmx; n++)
       {
          ahat<float2>(D.d_MT32, rho, t, K32.chi(n));
+         dfl_Lv_dev(K32.chi(n));
          HIP_OK(hipMemcpyAsync(K32.phi(n), rho, sizeof(float2) * nv, hipMemcpyDeviceToDevice, L.stream));
          if (!lgs_step(K32, n, rho, &rn)) break;
          if (rn <= res * r0)
@@ -1365,6 +1577,30 @@ extern "C" void dfl_renormalize_modes(spinor **mds)
 }
 
 // basis vector k as a host array in the reference's layout (test access)
+// mds[l] = the generating mode l rebuilt from its coordinates in the
+// current subspace (dfl/dfl_subspace.c dfl_restore_modes)
+extern "C" void dfl_restore_modes(spinor **mds)
+{
+   const char *where = "dfl_restore_modes [dfl_subspace.c]";
+   Lib &L = lib();
+   if (!ready(where)) return;
+   const size_t nw = (size_t)D.nb * D.Ns;
+   double2 *w = wvd(0);
+   for (int l = 0; l < D.Ns; l++)
+   {
+      int off = 0;
+      float2 *f = (float2 *)device_field(mds[l], F32, &off);
+      if (!f || off)
+      {
+         fail(where, "the modes must be full device fields");
+         return;
+      }
+      vconv<float2, double2>(D.d_vmds + (size_t)l * nw, w, nw);
+      vd2s_dev(w, f);
+   }
+   (void)L;
+}
+
 extern "C" void oqh_dfl_export_basis(int k, spinor *host)
 {
    const char *where = "oqh_dfl_export_basis";
@@ -1464,6 +1700,7 @@ extern "C" int set_Awhat(double mu)
    int unsafe = 0;
    HIP_OK(hipMemcpyAsync(&unsafe, D.d_unsafe, sizeof(int), hipMemcpyDeviceToHost, L.stream));
    HIP_OK(hipStreamSynchronize(L.stream));
+   if (!unsafe) unsafe = set_ltl_modes();
    D.awhat_ok = (unsafe == 0);
    return unsafe ? 1 : 0;
 }
@@ -1526,7 +1763,16 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
    vdot<double2>(b, b, nv, nullptr, nullptr, &bn2);
    const double rn0 = sqrt(bn2);
    HIP_OK(hipMemsetAsync(x, 0, sizeof(double2) * nv, L.stream));
+   // the part along the little modes, x0 = P0 M P0 b, is solved exactly;
+   // FGCR then solves P_L Ahat y = P_L b (ltl_gcr.c with ltl_modes.c)
+   double2 *x0 = wvd(9);
    double rn = rn0;
+   if (D.lm_ok && rn0 > 0.0)
+   {
+      dfl_LRvd_dev(b, x0);
+      vdot<double2>(b, b, nv, nullptr, nullptr, &bn2);
+      rn = sqrt(bn2);
+   }
    if (rn0 > 0.0)
    {
       // FGCR (linsolv/fgcr4vd.c) with Mop = gcr4v in single precision
@@ -1551,6 +1797,7 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
             status[1] += (st < 0) ? D.nmx_gcr : st;
             vconv<float2, double2>(x32, K64.phi(k), nv);
             ahat<double2>(D.d_MT, K64.phi(k), t, K64.chi(k));
+            if (D.lm_ok) dfl_RLvd_dev(K64.phi(k), K64.chi(k));
             if (!lgs_step(K64, k, r, &rn))
             {
                zero = true;
@@ -1560,8 +1807,9 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
             if (rn <= tol || k == nkv - 1 || status[0] >= D.nmx) break;
          }
          lgs_solution(K64, zero ? k : k + 1, x, 1);
-         // recompute the residual r = b - Ahat x
+         // recompute the residual r = P_L (b - Ahat x)
          ahat<double2>(D.d_MT, x, t, r);
+         if (D.lm_ok) dfl_Lvd_dev(r);
          k_vscale<double2><<<(unsigned)((nv + 255) / 256), 256, 0, L.stream>>>(r, nv, -1.0);
          vaxpy<double2>(r, b, nv, 1.0, 0.0);
          double r2;
@@ -1575,6 +1823,7 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
          if (zero) break;
       }
    }
+   if (D.lm_ok && rn0 > 0.0) vaxpy<double2>(x, x0, nv, 1.0, 0.0);
    // psi_e = x, psi_o = u_o - sum M_o x_e
    HIP_OK(hipMemcpyAsync(psi, x, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
    k_aw_hop<double2><<<D.nbh, AWH, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, D.nbh, 0, x, u, psi + nv, 1);
@@ -1600,7 +1849,7 @@ extern "C" double ltl_gcr(double mu, complex_dble *eta, complex_dble *psi, int *
       status[0] = status[1] = 0;
       return 0.0;
    }
-   double2 *e = wvd(40 + 2 * D.nkv), *p = wvd(41 + 2 * D.nkv);
+   double2 *e = wvd(10), *p = wvd(11);
    HIP_OK(hipMemcpyAsync(e, eta, sizeof(double2) * n, hipMemcpyHostToDevice, L.stream));
    const double rho = ltl_solve(e, p, ifail, status);
    HIP_OK(hipMemcpyAsync(psi, p, sizeof(double2) * n, hipMemcpyDeviceToHost, L.stream));
@@ -1650,8 +1899,8 @@ void dfl_mop_dev(const float2 *rho, float2 *phi, float2 *chi)
    Lib &L = lib();
    const int vt = L.g.VT;
    float2 *p0 = work_field(&R.psi0), *r1 = work_field(&R.wtmp);
-   if (!R.vin) R.vin = wvd(42 + 2 * D.nkv);
-   if (!R.vout) R.vout = wvd(43 + 2 * D.nkv);
+   if (!R.vin) R.vin = wvd(12);
+   if (!R.vout) R.vout = wvd(13);
    s2vd_dev(rho, R.vin);
    int ifail = 0, st[2] = {0, 0};
    (void)ltl_solve(R.vin, R.vout, &ifail, st);
@@ -1762,34 +2011,121 @@ extern "C" double dfl_sap_gcr(int nkv, int nmx, int istop, double res, double mu
    return rho;
 }
 
+namespace {
+
+// checks and set_flds of dfl_modes.c: the generation mass and the
+// twisted-mass flag (restored by gen_clean_up); false if a check failed
+struct GenState
+{
+   double m0_keep = 0.0;
+   int eo_keep = 0;
+};
+
+bool gen_prepare(const char *where, GenState &g, int *ifail)
+{
+   Lib &L = lib();
+   if (!dfl_grid(where)) return false;
+   if (D.nkv == 0 || D.ninv == 0)
+   {
+      fail(where, "Deflation projection or subspace generation parameters are not set");
+      return false;
+   }
+   if (L.sap_ncy == 0)
+   {
+      fail(where, "SAP parameters are not set");
+      return false;
+   }
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return false;
+   }
+   g.m0_keep = L.m0;
+   g.eo_keep = L.eoflg;
+   if (D.m0 != L.m0) oqh_set_sw_parms(D.m0, L.csw);
+   oqh_set_tm_parms(1);
+   if (!sap_prepare(1, where, true)) ifail[0] = -3;
+   return true;
+}
+
+void gen_clean_up(const GenState &g)
+{
+   Lib &L = lib();
+   if (L.m0 != g.m0_keep) oqh_set_sw_parms(g.m0_keep, L.csw);
+   oqh_set_tm_parms(g.eo_keep);
+   D.aw_ok = D.awhat_ok = false;
+}
+
+// one deflated inverse-iteration step of every mode (dfl_smooth_fields):
+// mds_l <- psi1 + ncy Schwarz cycles on mds_l - D psi1, psi1 the prolongated
+// little solution of the mode's coordinates; adds the little solver's
+// Krylov counts to st
+void dfl_smooth_fields(spinor **mds, spinor **ws, int *st)
+{
+   Lib &L = lib();
+   const int vt = L.g.VT;
+   const float mus = (float)D.mu;
+   const size_t nw = (size_t)D.nb * D.Ns;
+   double2 *w = wvd(14), *w2 = wvd(15);
+   int off = 0;
+   for (int l = 0; l < D.Ns; l++)
+   {
+      vconv<float2, double2>(D.d_vmds + (size_t)l * nw, w, nw);
+      int lf = 0, s2[2] = {0, 0};
+      (void)ltl_solve(w, w2, &lf, s2);
+      st[0] += s2[0];
+      st[1] += s2[1];
+      float2 *p1 = (float2 *)device_field(ws[1], F32, &off);
+      vd2s_dev(w2, p1);
+      Dw(mus, ws[1], ws[0]);
+      diff_s2s(vt, 2, mds[l], ws[0]);   // ws0 = mds - D psi1
+      set_s2zero(vt, 2, mds[l]);
+      float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
+      for (int j = 0; j < D.ncy; j++) sap_device(mus, 1, D.nmr, p, r);
+      add_s2s(vt, 2, ws[1], mds[l]);
+   }
+}
+
+// n deflated steps (set_Awhat, dfl_smooth_fields, renormalisation, new
+// subspace) while ifail[0] == 0; returns the number of steps done
+int dfl_steps(int n, spinor **mds, spinor **ws, int *ifail, int *status)
+{
+   int k = 0;
+   for (; k < n && ifail[0] == 0; k++)
+   {
+      if (set_Awhat(D.mu))
+      {
+         ifail[0] = -2;
+         break;
+      }
+      dfl_smooth_fields(mds, ws, status);
+      dfl_renormalize_modes(mds);
+      dfl_subspace(mds);
+   }
+   return k;
+}
+
+void average_status(int nsolve, int *status)
+{
+   if (nsolve > 0)
+   {
+      status[0] = (status[0] + nsolve / 2) / nsolve;
+      status[1] = (status[1] + nsolve / 2) / nsolve;
+   }
+}
+
+bool gen_done = false;   // a subspace was generated by dfl_modes
+
+}  // namespace
+
 extern "C" void dfl_modes(int *ifail, int *status)
 {
    const char *where = "dfl_modes [dfl_modes.c]";
    Lib &L = lib();
    ifail[0] = 0;
    status[0] = status[1] = 0;
-   if (!dfl_grid(where)) return;
-   if (D.nkv == 0 || D.ninv == 0)
-   {
-      fail(where, "Deflation projection or subspace generation parameters are not set");
-      return;
-   }
-   if (L.sap_ncy == 0)
-   {
-      fail(where, "SAP parameters are not set");
-      return;
-   }
-   if (!L.ud_set)
-   {
-      fail(where, "gauge field is not set");
-      return;
-   }
-   // set_flds: the generation mass and the twisted-mass flag
-   const double m0_keep = L.m0;
-   const int eo_keep = L.eoflg;
-   if (D.m0 != L.m0) oqh_set_sw_parms(D.m0, L.csw);
-   oqh_set_tm_parms(1);
-   if (!sap_prepare(1, where, true)) ifail[0] = -3;
+   GenState g;
+   if (!gen_prepare(where, g, ifail)) return;
    const int vt = L.g.VT;
    alloc_ws(L.ws_used + D.Ns + 2);
    spinor **mds = reserve_ws(D.Ns);
@@ -1805,60 +2141,64 @@ extern "C" void dfl_modes(int *ifail, int *status)
       dfl_subspace(mds);
    else
    {
-      int n;
-      for (n = 0; n < 3; n++)
+      // smooth_fields: n+1 Schwarz cycles on D psi = mds[l], n = 0, 1, 2
+      for (int n = 0; n < 3; n++)
          for (int l = 0; l < D.Ns; l++)
          {
-            // smooth_fields: n+1 Schwarz cycles on D psi = mds[l]
             assign_s2s(vt, 2, mds[l], ws[0]);
             set_s2zero(vt, 2, mds[l]);
             float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
             for (int j = 0; j <= n; j++) sap_device(mus, 1, D.nmr, p, r);
          }
       dfl_subspace(mds);
-      int nsolve = 0;
-      for (; n < D.ninv && ifail[0] == 0; n++)
-      {
-         if (set_Awhat(D.mu))
-         {
-            ifail[0] = -2;
-            break;
-         }
-         // dfl_smooth_fields: deflated inverse iteration of every mode
-         const size_t nw = (size_t)D.nb * D.Ns;
-         double2 *w = wvd(44 + 2 * D.nkv), *w2 = wvd(45 + 2 * D.nkv);
-         for (int l = 0; l < D.Ns; l++)
-         {
-            vconv<float2, double2>(D.d_vmds + (size_t)l * nw, w, nw);
-            int lf = 0, st[2] = {0, 0};
-            (void)ltl_solve(w, w2, &lf, st);
-            status[0] += st[0];
-            status[1] += st[1];
-            nsolve++;
-            float2 *p1 = (float2 *)device_field(ws[1], F32, &off);
-            vd2s_dev(w2, p1);
-            Dw(mus, ws[1], ws[0]);
-            diff_s2s(vt, 2, mds[l], ws[0]);   // ws0 = mds - D psi1
-            set_s2zero(vt, 2, mds[l]);
-            float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
-            for (int j = 0; j < D.ncy; j++) sap_device(mus, 1, D.nmr, p, r);
-            add_s2s(vt, 2, ws[1], mds[l]);
-         }
-         dfl_renormalize_modes(mds);
-         dfl_subspace(mds);
-      }
-      if (nsolve > 0)
-      {
-         status[0] = (status[0] + nsolve / 2) / nsolve;
-         status[1] = (status[1] + nsolve / 2) / nsolve;
-      }
+      const int nd = dfl_steps(D.ninv - 3, mds, ws, ifail, status);
+      average_status(D.Ns * nd, status);
+      gen_done = true;
    }
    release_ws();
    release_ws();
-   // clean_up: restore the mass and the twisted-mass flag
-   if (L.m0 != m0_keep) oqh_set_sw_parms(m0_keep, L.csw);
-   oqh_set_tm_parms(eo_keep);
-   D.aw_ok = D.awhat_ok = false;
+   gen_clean_up(g);
+}
+
+// nsm further deflated inverse-iteration steps on the modes restored from
+// the current subspace (dfl/dfl_modes.c dfl_update)
+extern "C" void dfl_update(int nsm, int *ifail, int *status)
+{
+   const char *where = "dfl_update [dfl_modes.c]";
+   Lib &L = lib();
+   ifail[0] = 0;
+   status[0] = status[1] = 0;
+   if (!gen_done || !D.basis_ok)
+   {
+      fail(where, "No previous call of dfl_modes()");
+      return;
+   }
+   GenState g;
+   if (!gen_prepare(where, g, ifail)) return;
+   if (ifail[0] == 0)
+   {
+      alloc_ws(L.ws_used + D.Ns + 2);
+      spinor **mds = reserve_ws(D.Ns);
+      spinor **ws = reserve_ws(2);
+      dfl_restore_modes(mds);
+      const int nd = dfl_steps(nsm, mds, ws, ifail, status);
+      average_status(D.Ns * nd, status);
+      release_ws();
+      release_ws();
+   }
+   gen_clean_up(g);
+}
+
+extern "C" void dfl_update2(int nsm, int *ifail, int *status)
+{
+   dfl_update(nsm, ifail, status);
+   if (ifail[0] == -2)
+      dfl_modes(ifail + 1, status + 2);
+   else
+   {
+      ifail[1] = 0;
+      status[2] = status[3] = 0;
+   }
 }
 
 extern "C" void dfl_modes2(int *ifail, int *status)

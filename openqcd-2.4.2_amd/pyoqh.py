@@ -94,6 +94,7 @@ _SIG = {
     "set_dfl_pro_parms": (DflProParms, [_I, _I, _D, _I, _D]), "dfl_pro_parms": (DflProParms, []),
     "set_dfl_gen_parms": (DflGenParms, [_D, _D, _I, _I, _I]), "dfl_gen_parms": (DflGenParms, []),
     "dfl_modes": (None, [P, P]), "dfl_modes2": (None, [P, P]),
+    "dfl_update": (None, [_I, P, P]), "dfl_update2": (None, [_I, P, P]), "dfl_restore_modes": (None, [P]),
     "dfl_sap_gcr": (_D, [_I, _I, _I, _D, _D, P, P, P, P]),
     "dfl_sap_gcr2": (_D, [_I, _I, _I, _D, _D, P, P, P, P]),
     "oqh_dfl_release": (None, []),

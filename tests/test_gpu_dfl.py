@@ -94,6 +94,8 @@ def setup():
     o.sw_term(3)
     lat = pyoqh.Lattice(L, T, bc=3, m0=M0, csw=1.77)
     _sig(lat.lib)
+    lat.lib.dfl_restore_modes.argtypes = [P]
+    lat.lib.dfl_update.argtypes = [C.c_int, P, P]
     lat.import_ud(o.ud())
     assert lat.sw_term(3) == 0
     yield o, eta, s32, lat
@@ -131,6 +133,12 @@ def test_subspace_and_little_operator(setup):
     dev = np.abs(basis_g - basis_o).max()
     print("basis max deviation", dev, flush=True)
     assert dev < 2e-6 * np.abs(basis_o).max()
+    # dfl_restore_modes rebuilds the modes from their coordinates
+    rest = [lat.alloc_s() for _ in range(NS)]
+    lib.dfl_restore_modes((P * NS)(*rest))
+    for f, m in zip(rest, mds):
+        assert np.abs(lat.get_s(f) - m).max() < 2e-6 * np.abs(m).max()
+        lat.free(f)
     # projections
     w = np.empty((nb, NS), np.complex128)
     s = lat.alloc_s()
@@ -224,6 +232,14 @@ def test_deflated_solver(setup):
     lib.oqh_set_sap_prec(0)
     assert counts[0] < plain_st.value
     assert np.array_equal(lat.get_sd(e), eta)
+    # one more deflated inverse-iteration step on the restored modes
+    lib.dfl_update(1, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    print("dfl_update ifail", ifail[0], "status", st[:2].tolist(), flush=True)
+    assert ifail[0] == 0
+    lib.dfl_sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    res = np.linalg.norm(eta - o.Dw_dble(0.0, lat.get_sd(p).copy())) / np.linalg.norm(eta)
+    print("after dfl_update: status", st[:3].tolist(), "true residual", res, flush=True)
+    assert ifail[0] == 0 and res <= 1.01e-10 and st[0] <= counts[0] + 2
     # dfl_sap_gcr2, in place (psi == eta)
     lib.dfl_sap_gcr2(16, 256, 0, 1e-10, 0.0, e, e, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
     res = np.linalg.norm(eta - o.Dw_dble(0.0, lat.get_sd(e).copy())) / np.linalg.norm(eta)
