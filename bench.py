@@ -167,49 +167,8 @@ def solve_times(args):
             log("bench: %s %.3f s status %d residual %.2e error %.2e" % (name, dt, status.value, res, err))
         lib.oqh_set_sap_prec(0)
         lib.oqh_set_nn_prec(0)
-        # deflated SAP+GCR (dfl_sap_gcr) with the subspace parameters of the
-        # reference's ms4.in: 4^4 blocks, Ns = 28, generated at the solve's mass
-        dfl = {"subspace": "bs 4^4, Ns 28; projection nkv 16 nmx 64 res 1e-2 nmx_gcr 24 res_gcr 1e-2; "
-                           "generation kappa 1/(2(4+m0)) mu 0 ninv 10 nmr 4 ncy 4",
-               "setup": "dfl_modes + set_Awhat(0): once per gauge field, shared by all solves on it"}
-        bsd = np.array((4, 4, 4, 4), np.int32)
-        lib.set_dfl_parms(bsd.ctypes.data_as(P), 28)
-        lib.set_dfl_pro_parms(16, 64, 1e-2, 24, 1e-2)
-        lib.set_dfl_gen_parms(1.0 / (2.0 * (4.0 - 0.35)), 0.0, 10, 4, 4)
-        lib.start_ranlux(0, 7)
-        dif = np.zeros(2, np.int32)
-        dst = np.zeros(6, np.int32)
-        lat.sync()
-        t0 = time.perf_counter()
-        lib.dfl_modes(dif.ctypes.data_as(P), dst.ctypes.data_as(P))
-        # the little operator of the solve (eoflg 0, mu 0) with its little modes
-        lib.set_Awhat.restype = C.c_int
-        lib.set_Awhat.argtypes = [C.c_double]
-        if dif[0] == 0 and lib.set_Awhat(0.0) != 0:
-            dif[0] = -2
-        lat.sync()
-        dfl["setup_seconds"] = round(time.perf_counter() - t0, 3)
-        dfl["setup_ifail"] = int(dif[0])
-        log("bench: dfl_modes %.3f s ifail %d status %s" % (dfl["setup_seconds"], dif[0], dst[:2].tolist()))
-        if dif[0] == 0:
-            for name, prec in (("fp32_sap", 0), ("fp16_storage_sap", 1)):
-                lib.oqh_set_sap_prec(prec)
-                lat.sync()
-                t0 = time.perf_counter()
-                lib.dfl_sap_gcr(24, 1024, 0, 1e-10, 0.0, e, p, dif.ctypes.data_as(P), dst.ctypes.data_as(P))
-                lat.sync()
-                dt = time.perf_counter() - t0
-                lib.Dw_dble(0.0, p, r)
-                lib.mulr_spinor_add_dble(V, 0, r, e, -1.0)
-                res = (nsq(r) / nsq(e)) ** 0.5
-                lib.mulr_spinor_add_dble(V, 0, p, pref, -1.0)
-                err = (nsq(p) / nref) ** 0.5
-                dfl[name] = {"seconds": round(dt, 3), "status": [int(x) for x in dst[:3]], "ifail": int(dif[0]),
-                             "true_residual": float("%.3e" % res), "error_vs_reference": float("%.3e" % err)}
-                log("bench: deflated %s %.3f s status %s residual %.2e error %.2e" %
-                    (name, dt, dst[:3].tolist(), res, err))
-            lib.oqh_set_sap_prec(0)
-        out["deflated"] = dfl
+        out["deflated"] = deflated_solves(lat, e, p, r, pref)
+        dfl = out["deflated"]
         for name in ("fp16_storage_sap", "fp64_sap", "fp16_sap_learned"):
             if out["plain_fp32_sap"]["ifail"] == 0 and out[name]["ifail"] == 0:
                 out["speedup_%s_vs_plain" % name] = round(out["plain_fp32_sap"]["seconds"] /
@@ -222,6 +181,95 @@ def solve_times(args):
         lat.close()
     out["learned_held_out"] = learned_held_out(wts)
     return out
+
+
+def deflated_solves(lat, e, p, r, pref=None):
+    """Deflated SAP+GCR (dfl_sap_gcr) on the 48^4 solver workload (field,
+    source and SAP parameters set by the caller) with the subspace
+    parameters of the reference's ms4.in: 4^4 DFL blocks, Ns = 28, generated
+    at the solve's mass (dfl_modes, then set_Awhat for the solve), then one
+    solve with FP32 and one with FP16-storage Schwarz cycles; the true
+    residual and, given a reference solution pref, the error against it."""
+    import ctypes as C
+    import numpy as np
+    P = C.c_void_p
+    lib, V = lat.lib, lat.V
+
+    def nsq(f):
+        q = lib.norm_square_dble(V, 0, f)
+        return q.q[0] + q.q[1]
+
+    dfl = {"subspace": "bs 4^4, Ns 28; projection nkv 16 nmx 64 res 1e-2 nmx_gcr 24 res_gcr 1e-2; "
+                       "generation kappa 1/(2(4+m0)) mu 0 ninv 10 nmr 4 ncy 4",
+           "setup": "dfl_modes + set_Awhat(0): once per gauge field, shared by all solves on it"}
+    bsd = np.array((4, 4, 4, 4), np.int32)
+    lib.set_dfl_parms(bsd.ctypes.data_as(P), 28)
+    lib.set_dfl_pro_parms(16, 64, 1e-2, 24, 1e-2)
+    lib.set_dfl_gen_parms(1.0 / (2.0 * (4.0 - 0.35)), 0.0, 10, 4, 4)
+    lib.set_Awhat.restype = C.c_int
+    lib.set_Awhat.argtypes = [C.c_double]
+    lib.start_ranlux(0, 7)
+    dif = np.zeros(2, np.int32)
+    dst = np.zeros(6, np.int32)
+    lat.sync()
+    t0 = time.perf_counter()
+    lib.dfl_modes(dif.ctypes.data_as(P), dst.ctypes.data_as(P))
+    # the little operator of the solve (eoflg 0, mu 0) with its little modes
+    if dif[0] == 0 and lib.set_Awhat(0.0) != 0:
+        dif[0] = -2
+    lat.sync()
+    dfl["setup_seconds"] = round(time.perf_counter() - t0, 3)
+    dfl["setup_ifail"] = int(dif[0])
+    dfl["setup_status"] = [int(x) for x in dst[:2]]
+    log("bench: dfl_modes + set_Awhat %.3f s ifail %d status %s" % (dfl["setup_seconds"], dif[0], dst[:2].tolist()))
+    if dif[0] == 0:
+        nref = nsq(pref) if pref is not None else None
+        for name, prec in (("fp32_sap", 0), ("fp16_storage_sap", 1)):
+            lib.oqh_set_sap_prec(prec)
+            lat.sync()
+            t0 = time.perf_counter()
+            lib.dfl_sap_gcr(24, 1024, 0, 1e-10, 0.0, e, p, dif.ctypes.data_as(P), dst.ctypes.data_as(P))
+            lat.sync()
+            dt = time.perf_counter() - t0
+            lib.Dw_dble(0.0, p, r)
+            lib.mulr_spinor_add_dble(V, 0, r, e, -1.0)
+            res = (nsq(r) / nsq(e)) ** 0.5
+            rec = {"seconds": round(dt, 3), "status": [int(x) for x in dst[:3]], "ifail": int(dif[0]),
+                   "true_residual": float("%.3e" % res)}
+            if pref is not None:
+                lib.mulr_spinor_add_dble(V, 0, p, pref, -1.0)
+                rec["error_vs_reference"] = float("%.3e" % ((nsq(p) / nref) ** 0.5))
+            dfl[name] = rec
+            log("bench: deflated %s %.3f s status %s residual %.2e" % (name, dt, dst[:3].tolist(), res))
+        lib.oqh_set_sap_prec(0)
+    return dfl
+
+
+def solver_lattice():
+    """The 48^4 solver workload's lattice, field, SAP parameters and a
+    Gaussian source (see solve_times)."""
+    import ctypes as C
+    import numpy as np
+    import pyoqh
+    P = C.c_void_p
+    lat = pyoqh.Lattice((48, 48, 48, 48), (8, 8, 8, 8), bc=3, m0=-0.35, csw=1.77)
+    lib = lat.lib
+    lib.start_ranlux(0, 3)
+    lib.random_ud()
+    lib.fwd_euler(50, 0.01)
+    lib.set_ud_phase()
+    lib.oqh_set_sap_parms.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    lib.sap_gcr.restype = C.c_double
+    lib.sap_gcr.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, P, P,
+                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.norm_square_dble.restype = pyoqh.Qflt
+    lib.norm_square_dble.argtypes = [C.c_int, C.c_int, P]
+    lib.mulr_spinor_add_dble.argtypes = [C.c_int, C.c_int, P, P, C.c_double]
+    bs = np.array((4, 4, 4, 4), np.int32)
+    lib.oqh_set_sap_parms(bs.ctypes.data_as(P), 1, 4, 5)
+    e, p, r = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
+    lib.random_sd(lat.VT, 2, e, 1.0)
+    return lat, e, p, r
 
 
 def learned_held_out(wts):
@@ -485,6 +533,8 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="only the solver section (training, reference solve, variants): "
                          "prints its JSON object")
+    ap.add_argument("--deflated-only", action="store_true",
+                    help="only the deflated solves of the 48^4 solver workload: prints their JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -493,6 +543,14 @@ def main():
 
     import numpy as np
     import pyoqh
+
+    if args.deflated_only:
+        lat, e, p, r = solver_lattice()
+        try:
+            print(json.dumps({"deflated": deflated_solves(lat, e, p, r)}))
+        finally:
+            lat.close()
+        return
 
     if args.solve_only:
         print(json.dumps({"solve": solve_times(args), "uneven_batch": uneven_batch(),
