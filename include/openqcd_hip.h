@@ -162,6 +162,12 @@ void random_sd(int vol, int icom, spinor_dble *sd, double sigma);
    ipt indices) on the points of time slice t0, zero elsewhere; full field */
 void oqh_random_source(int t0, spinor_dble *sd);
 void ranlxd(double *r, int n);
+/* one start field of dfl_modes (replaces the ranlxs loop of
+   dfl/dfl_modes.c:248-271, random_fields): component 4 j + i of point x
+   (ipt index) is word i of Philox4x32-10 (Salmon et al., SC11 2011) at
+   counter (x, j, 0, 0), j < 6, as (w >> 8) 2^-24 - 1/2, keyed by two ranlxd
+   numbers of stream 0 times 2^32; exported as test access */
+void oqh_random_start_field(spinor *s);
 
 /* learned preconditioner (no counterpart in the reference): a linear
    gauge-equivariant network of parallel-transport layers,

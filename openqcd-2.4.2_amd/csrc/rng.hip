@@ -284,28 +284,6 @@ __global__ __launch_bounds__(RWG) void k_random_source(RlxState *streams, int nt
    store_lane(streams + k, R);
 }
 
-// uniform numbers in [-1/2, 1/2) on every real component of the points of
-// reference thread k (its VOLUME_TRD ipt indices), from stream k
-__global__ __launch_bounds__(RWG) void k_random_uniform(RlxState *streams, int nthr, float2 *f, int vt)
-{
-   __shared__ long long W[RWG][13][4];
-   __shared__ double RD[RWG][48];
-   const int k = blockIdx.x * RWG + threadIdx.x;
-   if (k >= nthr) return;
-   LaneRng R;
-   load_lane(streams + k, W[threadIdx.x], RD[threadIdx.x], R);
-   for (int x = k * vt; x < (k + 1) * vt; x++)
-   {
-      float2 *p = f + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
-      for (int c = 0; c < 12; c++)
-      {
-         const double a = R.next(), b = R.next();
-         p[c * SB] = make_float2((float)a - 0.5f, (float)b - 0.5f);
-      }
-   }
-   store_lane(streams + k, R);
-}
-
 // n numbers of stream 0 (the master thread's ranlxd)
 __global__ void k_ranlxd(RlxState *streams, double *out, int n)
 {
@@ -333,18 +311,60 @@ bool streams_ready(const char *where)
 }  // namespace
 
 // random start fields of the deflation subspace generation
+// Philox4x32-10 (J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC11, 2011): ten rounds of
+// two 32x32->64-bit products with the key bumped by the Weyl constants
+__device__ inline uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+   for (int r = 0; r < 10; r++)
+   {
+      const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+      c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0);
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+   }
+   return c;
+}
+
+// start fields of the subspace generation: component 4 j + i of point x
+// (ipt index) is word i of Philox4x32-10 at counter (x, j, 0, 0), j < 6, as
+// (w >> 8) 2^-24 - 1/2; the key is two ranlxd numbers of stream 0 scaled
+// by 2^32 (key[0], key[1] on the device)
+__global__ __launch_bounds__(256) void k_random_start(float2 *f, int V, const double *key)
+{
+   const int x = blockIdx.x * 256 + threadIdx.x;
+   if (x >= V) return;
+   const uint32_t k0 = (uint32_t)(key[0] * 4294967296.0), k1 = (uint32_t)(key[1] * 4294967296.0);
+   float2 *p = f + (size_t)(x >> 6) * (12 * SB) + (x & (SB - 1));
+   const float s = 1.0f / 16777216.0f;
+#pragma unroll
+   for (int j = 0; j < 6; j++)
+   {
+      const uint4 w = philox4x32_10(make_uint4((uint32_t)x, (uint32_t)j, 0u, 0u), k0, k1);
+      p[(2 * j) * SB] = make_float2((float)(w.x >> 8) * s - 0.5f, (float)(w.y >> 8) * s - 0.5f);
+      p[(2 * j + 1) * SB] = make_float2((float)(w.z >> 8) * s - 0.5f, (float)(w.w >> 8) * s - 0.5f);
+   }
+}
+
+double *d_key = nullptr;
+
 void random_uniform_s(float2 *f)
 {
    Lib &L = lib();
    if (!streams_ready("dfl_modes [dfl_modes.c]") || !f) return;
-   const int nthr = L.g.nthread;
-   k_random_uniform<<<(nthr + RWG - 1) / RWG, RWG, 0, L.stream>>>(d_streams, nthr, f, L.g.VT);
+   if (!d_key) d_key = (double *)dev_alloc(sizeof(double) * 2);
+   if (!d_key) return;
+   k_ranlxd<<<1, 64, 0, L.stream>>>(d_streams, d_key, 2);
+   k_random_start<<<(L.g.V + 255) / 256, 256, 0, L.stream>>>(f, L.g.V, d_key);
 }
 
 void release_rng()
 {
    dev_free(d_streams);
    d_streams = nullptr;
+   dev_free(d_key);
+   d_key = nullptr;
    n_streams = 0;
 }
 
@@ -424,6 +444,23 @@ extern "C" void random_sd(int vol, int icom, spinor_dble *sd, double sigma)
 }
 
 // the random source of main/ms4.c:383-406 on time slice t0 (full field)
+// one start field of the subspace generation (dfl_modes), as test access
+extern "C" void oqh_random_start_field(spinor *s)
+{
+   const char *where = "oqh_random_start_field";
+   Lib &L = lib();
+   if (!streams_ready(where)) return;
+   Res r = resolve(s, F32, L.g.V, 0, false);
+   if (!r.dev) return;
+   if (r.off)
+   {
+      fail(where, "the field must be a full field");
+      return;
+   }
+   random_uniform_s((float2 *)r.dev);
+   finish(r, F32, L.g.V, true);
+}
+
 extern "C" void oqh_random_source(int t0, spinor_dble *sd)
 {
    const char *where = "oqh_random_source";

@@ -81,7 +81,7 @@ _SIG = {
     "oqh_set_sap_graph": (None, [_I]),
     "oqh_sap_autotune": (_D, [_I, _I, _I, _I, P]),
     "start_ranlux": (None, [_I, _I]), "random_ud": (None, []),
-    "random_sd": (None, [_I, _I, P, _D]), "oqh_random_source": (None, [_I, P]), "ranlxd": (None, [P, _I]),
+    "random_sd": (None, [_I, _I, P, _D]), "oqh_random_source": (None, [_I, P]), "oqh_random_start_field": (None, [P]), "ranlxd": (None, [P, _I]),
     "sap_gcr_multi": (None, [_I, _I, _I, _I, _D, _D, P, P, P, P, P]),
     "oqh_nn_create": (_I, [_I, P]), "oqh_nn_nweights": (_I, []),
     "oqh_nn_set_weights": (None, [P]), "oqh_nn_get_weights": (None, [P]),

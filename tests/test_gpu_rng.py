@@ -81,3 +81,28 @@ def test_random_fields(c):
         assert np.abs(m @ np.conj(np.transpose(m, (0, 2, 1))) - np.eye(3)).max() < 1e-14
     finally:
         lat.close()
+
+
+@pytest.mark.gpu
+def test_start_field_philox():
+    """The start fields of dfl_modes: Philox4x32-10 keyed by two ranlxd
+    numbers of stream 0, bit-exact against the numpy restatement
+    (oracle/philox.py, pinned by the Random123 known answers)."""
+    import philox as ph
+    lat = pyoqh.Lattice((16, 8, 8, 8), (8, 8, 8, 8), bc=0)
+    try:
+        P = ctypes.c_void_p
+        lat.lib.start_ranlux(0, 9)
+        s = lat.alloc_s()
+        lat.lib.oqh_random_start_field(s)
+        lat.lib.oqh_random_start_field(s)      # the second field: key from the next two numbers
+        got = lat.get_s(s)
+        lat.lib.start_ranlux(0, 9)
+        key = np.empty(4)
+        lat.lib.ranlxd(key.ctypes.data_as(P), 4)
+        ref = ph.start_field(lat.V, key[2:])
+        assert np.array_equal(got, ref)
+        assert not np.array_equal(ref, ph.start_field(lat.V, key[:2]))
+        lat.free(s)
+    finally:
+        lat.close()
