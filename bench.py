@@ -200,7 +200,7 @@ def deflated_solves(lat, e, p, r, pref=None):
         return q.q[0] + q.q[1]
 
     dfl = {"subspace": "bs 4^4, Ns 28; projection nkv 16 nmx 64 res 1e-2 nmx_gcr 24 res_gcr 1e-2; "
-                       "generation kappa 1/(2(4+m0)) mu 0 ninv 10 nmr 4 ncy 4",
+                       "generation kappa 1/(2(4+m0)) mu 0 ninv 10 nmr 4 ncy 4, FP16 Schwarz cycles",
            "setup": "dfl_modes + set_Awhat(0): once per gauge field, shared by all solves on it"}
     bsd = np.array((4, 4, 4, 4), np.int32)
     lib.set_dfl_parms(bsd.ctypes.data_as(P), 28)
@@ -213,7 +213,11 @@ def deflated_solves(lat, e, p, r, pref=None):
     dst = np.zeros(6, np.int32)
     lat.sync()
     t0 = time.perf_counter()
+    # the generation's Schwarz cycles on FP16 fields (a smoother: the
+    # subspace it spans serves both solves below)
+    lib.oqh_set_sap_prec(1)
     lib.dfl_modes(dif.ctypes.data_as(P), dst.ctypes.data_as(P))
+    lib.oqh_set_sap_prec(0)
     # the little operator of the solve (eoflg 0, mu 0) with its little modes
     if dif[0] == 0 and lib.set_Awhat(0.0) != 0:
         dif[0] = -2

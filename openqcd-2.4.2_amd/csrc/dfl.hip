@@ -447,84 +447,189 @@ __device__ inline void aw_dir(const AwArgs &a, const int *blk, int n, int l, int
    }
 }
 
+// the hopping term of direction MU from outside the block n into x, if the
+// neighbour in that direction lies in another block: its two colour
+// products c0, c1 (times -1/2) and the face index (2 MU + 1 forward, 2 MU
+// backward), else -1
+template <int MU>
+__device__ inline int aw_face(const AwArgs &a, const int *blk, int n, int l, int x, double2 *c0, double2 *c1)
+{
+   const int yf = a.nbr[(size_t)(2 * MU) * a.V + x] & 0x7fffffff;
+   const int yb = a.nbr[(size_t)(2 * MU + 1) * a.V + x] & 0x7fffffff;
+   const bool fwd = blk[yf] != n;
+   if (!fwd && blk[yb] == n) return -1;
+   const int y = fwd ? yf : yb;
+   const double2 *up;
+   if (fwd)
+      up = (x >= a.h) ? link_ptr(a.ud, x - a.h, 2 * MU) : link_ptr(a.ud, y - a.h, 2 * MU + 1);
+   else
+      up = (x >= a.h) ? link_ptr(a.ud, x - a.h, 2 * MU + 1) : link_ptr(a.ud, y - a.h, 2 * MU);
+   double2 sp[12], h0[3], h1[3];
+   ld_basis64(a, l, y, sp);
+   if (fwd)
+   {
+      spin_project<double2, MU, true>(sp, h0, h1);
+      su3_apply<double2, false>(up, h0, c0);
+      su3_apply<double2, false>(up, h1, c1);
+   }
+   else
+   {
+      spin_project<double2, MU, false>(sp, h0, h1);
+      su3_apply<double2, true>(up, h0, c0);
+      su3_apply<double2, true>(up, h1, c1);
+   }
+#pragma unroll
+   for (int c = 0; c < 3; c++)
+   {
+      c0[c] = make_double2(-0.5 * c0[c].x, -0.5 * c0[c].y);
+      c1[c] = make_double2(-0.5 * c1[c].x, -0.5 * c1[c].y);
+   }
+   return fwd ? 2 * MU + 1 : 2 * MU;
+}
+
+// (b, the spinor rebuilt from c0, c1 for direction MU) with b in FP32: the
+// rebuild is the adjoint of the spin projection, so this equals
+// (P b, c) with P b the projected components of b (in double precision)
+template <int MU>
+__device__ inline double2 aw_face_dot(const float2 *b, const double2 *c0, const double2 *c1, bool fwd)
+{
+   double2 bd[12], h0[3], h1[3];
+#pragma unroll
+   for (int c = 0; c < 12; c++) bd[c] = make_double2(b[c].x, b[c].y);
+   if (fwd)
+      spin_project<double2, MU, true>(bd, h0, h1);
+   else
+      spin_project<double2, MU, false>(bd, h0, h1);
+   double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+   for (int c = 0; c < 3; c++)
+   {
+      z.x += h0[c].x * c0[c].x + h0[c].y * c0[c].y + h1[c].x * c1[c].x + h1[c].y * c1[c].y;
+      z.y += h0[c].x * c0[c].y - h0[c].y * c0[c].x + h1[c].x * c1[c].y - h1[c].y * c1[c].x;
+   }
+   return z;
+}
+
+__device__ inline double2 wave_red2(double2 d)
+{
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      d.x += __shfl_xor(d.x, m);
+      d.y += __shfl_xor(d.y, m);
+   }
+   return d;
+}
+
+// column l of Ablk and of the eight Ahop[ifc] of block n in one pass over
+// the block's points: Y (site term and in-block hops of B_l) and the up to
+// four face terms of each point are formed once; every basis vector B_k is
+// then read twice per point (once for Y, once for the faces) instead of
+// once per face
 __global__ __launch_bounds__(DWG) void k_aw_column(AwArgs a, const int *blk, int l)
 {
-   __shared__ double2 part[DWG / 64][NS_MAX];
+   __shared__ double2 part[DWG / 64][9][NS_MAX];
    const int n = blockIdx.x;
    const int *pts = a.bpts + (size_t)n * a.bvol;
    const int Ns = a.Ns;
    const size_t nm = (size_t)Ns * Ns;
    const int w = threadIdx.x >> 6;
-   for (int q = 0; q < 9; q++)
+   const bool lead = (threadIdx.x & 63) == 0;
+   for (int e = threadIdx.x; e < (DWG / 64) * 9 * NS_MAX; e += DWG)
+      (&part[0][0][0])[e] = make_double2(0.0, 0.0);
+   __syncthreads();
+   for (int i0 = 0; i0 < a.bvol; i0 += DWG)
    {
-      for (int k = 0; k < Ns; k++)
-         if ((threadIdx.x & 63) == 0) part[w][k] = make_double2(0.0, 0.0);
-      for (int i0 = 0; i0 < a.bvol; i0 += DWG)
+      const int i = i0 + threadIdx.x;
+      const int x = (i < a.bvol) ? pts[i] : 0;
+      const bool act = (i < a.bvol) && !(a.masked && a.bnd[x]);
+      if (!__any(act)) continue;
+      // Y: site term and the hops from inside the block
       {
-         const int i = i0 + threadIdx.x;
          double2 v[12];
 #pragma unroll
          for (int c = 0; c < 12; c++) v[c] = make_double2(0.0, 0.0);
-         bool any = false;
-         if (i < a.bvol)
+         if (act)
          {
-            const int x = pts[i];
-            if (!(a.masked && a.bnd[x]))
-            {
-               if (q == 0)
-               {
-                  double2 s[12];
-                  ld_basis64(a, l, x, s);
-                  const double m = (x < a.h) ? a.mu : (a.eoflg ? 0.0 : a.mu);
-                  pauli_site(a.swd, x, m, s, v);
-               }
-               const int ifc = q - 1;
-               aw_dir<0>(a, blk, n, l, x, ifc, v);
-               aw_dir<1>(a, blk, n, l, x, ifc, v);
-               aw_dir<2>(a, blk, n, l, x, ifc, v);
-               aw_dir<3>(a, blk, n, l, x, ifc, v);
-               any = true;
-            }
+            double2 s[12];
+            ld_basis64(a, l, x, s);
+            const double m = (x < a.h) ? a.mu : (a.eoflg ? 0.0 : a.mu);
+            pauli_site(a.swd, x, m, s, v);
+            aw_dir<0>(a, blk, n, l, x, -1, v);
+            aw_dir<1>(a, blk, n, l, x, -1, v);
+            aw_dir<2>(a, blk, n, l, x, -1, v);
+            aw_dir<3>(a, blk, n, l, x, -1, v);
          }
-         const bool wave_any = __any(any);
-         if (!wave_any) continue;
          for (int k = 0; k < Ns; k++)
          {
             double2 d = make_double2(0.0, 0.0);
-            if (any)
+            if (act)
             {
                float2 b[12];
-               load_spinor(a.basis + (size_t)k * a.fu, pts[i], b);
+               load_spinor(a.basis + (size_t)k * a.fu, x, b);
                d = sdot(b, v);
             }
-#pragma unroll
-            for (int m = 32; m >= 1; m >>= 1)
+            d = wave_red2(d);
+            if (lead)
             {
-               d.x += __shfl_xor(d.x, m);
-               d.y += __shfl_xor(d.y, m);
-            }
-            if ((threadIdx.x & 63) == 0)
-            {
-               part[w][k].x += d.x;
-               part[w][k].y += d.y;
+               part[w][0][k].x += d.x;
+               part[w][0][k].y += d.y;
             }
          }
       }
-      __syncthreads();
-      for (int k = threadIdx.x; k < Ns; k += DWG)
+      // the face terms
       {
-         double2 s = make_double2(0.0, 0.0);
-         for (int ww = 0; ww < DWG / 64; ww++)
+         double2 f0[4][3], f1[4][3];
+         int fs[4] = {-1, -1, -1, -1};
+         if (act)
          {
-            s.x += part[ww][k].x;
-            s.y += part[ww][k].y;
+            fs[0] = aw_face<0>(a, blk, n, l, x, f0[0], f1[0]);
+            fs[1] = aw_face<1>(a, blk, n, l, x, f0[1], f1[1]);
+            fs[2] = aw_face<2>(a, blk, n, l, x, f0[2], f1[2]);
+            fs[3] = aw_face<3>(a, blk, n, l, x, f0[3], f1[3]);
          }
-         if (q == 0)
-            a.Ablk[(size_t)n * nm + (size_t)k * Ns + l] = s;
-         else
-            a.Ahop[((size_t)n * 8 + (q - 1)) * nm + (size_t)k * Ns + l] = s;
+         bool anyf = false;
+#pragma unroll
+         for (int mu = 0; mu < 4; mu++) anyf = anyf || fs[mu] >= 0;
+         if (!__any(anyf)) continue;
+         for (int k = 0; k < Ns; k++)
+         {
+            float2 b[12];
+            if (anyf) load_spinor(a.basis + (size_t)k * a.fu, x, b);
+            double2 d[4];
+            d[0] = (fs[0] >= 0) ? aw_face_dot<0>(b, f0[0], f1[0], fs[0] == 1) : make_double2(0.0, 0.0);
+            d[1] = (fs[1] >= 0) ? aw_face_dot<1>(b, f0[1], f1[1], fs[1] == 3) : make_double2(0.0, 0.0);
+            d[2] = (fs[2] >= 0) ? aw_face_dot<2>(b, f0[2], f1[2], fs[2] == 5) : make_double2(0.0, 0.0);
+            d[3] = (fs[3] >= 0) ? aw_face_dot<3>(b, f0[3], f1[3], fs[3] == 7) : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int ifc = 0; ifc < 8; ifc++)
+            {
+               const int mu = ifc >> 1;
+               double2 e = (fs[mu] == ifc) ? d[mu] : make_double2(0.0, 0.0);
+               e = wave_red2(e);
+               if (lead)
+               {
+                  part[w][1 + ifc][k].x += e.x;
+                  part[w][1 + ifc][k].y += e.y;
+               }
+            }
+         }
       }
-      __syncthreads();
+   }
+   __syncthreads();
+   for (int e = threadIdx.x; e < 9 * Ns; e += DWG)
+   {
+      const int q = e / Ns, k = e % Ns;
+      double2 s = make_double2(0.0, 0.0);
+      for (int ww = 0; ww < DWG / 64; ww++)
+      {
+         s.x += part[ww][q][k].x;
+         s.y += part[ww][q][k].y;
+      }
+      if (q == 0)
+         a.Ablk[(size_t)n * nm + (size_t)k * Ns + l] = s;
+      else
+         a.Ahop[((size_t)n * 8 + (q - 1)) * nm + (size_t)k * Ns + l] = s;
    }
 }
 
@@ -2056,6 +2161,19 @@ void gen_clean_up(const GenState &g)
    D.aw_ok = D.awhat_ok = false;
 }
 
+// ncy Schwarz cycles from psi = 0 on rho = r (the smoother of the subspace
+// generation): with oqh_set_sap_prec(1) on FP16 fields (HIP graph keyed on
+// the fixed field r), else the FP32 cycles of sap.c
+void gen_cycles(float mus, int ncy, spinor *psi, spinor *r)
+{
+   Lib &L = lib();
+   int off = 0;
+   float2 *p = (float2 *)device_field(psi, F32, &off), *rr = (float2 *)device_field(r, F32, &off);
+   if (L.sap_prec == 1 && sap_mixed_mop(mus, 1, D.nmr, ncy, rr, p, false)) return;
+   set_s2zero(L.g.VT, 2, psi);
+   for (int j = 0; j < ncy; j++) sap_device(mus, 1, D.nmr, p, rr);
+}
+
 // one deflated inverse-iteration step of every mode (dfl_smooth_fields):
 // mds_l <- psi1 + ncy Schwarz cycles on mds_l - D psi1, psi1 the prolongated
 // little solution of the mode's coordinates; adds the little solver's
@@ -2079,9 +2197,7 @@ void dfl_smooth_fields(spinor **mds, spinor **ws, int *st)
       vd2s_dev(w2, p1);
       Dw(mus, ws[1], ws[0]);
       diff_s2s(vt, 2, mds[l], ws[0]);   // ws0 = mds - D psi1
-      set_s2zero(vt, 2, mds[l]);
-      float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
-      for (int j = 0; j < D.ncy; j++) sap_device(mus, 1, D.nmr, p, r);
+      gen_cycles(mus, D.ncy, mds[l], ws[0]);
       add_s2s(vt, 2, ws[1], mds[l]);
    }
 }
@@ -2146,9 +2262,7 @@ extern "C" void dfl_modes(int *ifail, int *status)
          for (int l = 0; l < D.Ns; l++)
          {
             assign_s2s(vt, 2, mds[l], ws[0]);
-            set_s2zero(vt, 2, mds[l]);
-            float2 *p = (float2 *)device_field(mds[l], F32, &off), *r = (float2 *)device_field(ws[0], F32, &off);
-            for (int j = 0; j <= n; j++) sap_device(mus, 1, D.nmr, p, r);
+            gen_cycles(mus, n + 1, mds[l], ws[0]);
          }
       dfl_subspace(mds);
       const int nd = dfl_steps(D.ninv - 3, mds, ws, ifail, status);

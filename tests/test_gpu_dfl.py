@@ -232,8 +232,11 @@ def test_deflated_solver(setup):
     lib.oqh_set_sap_prec(0)
     assert counts[0] < plain_st.value
     assert np.array_equal(lat.get_sd(e), eta)
-    # one more deflated inverse-iteration step on the restored modes
+    # one more deflated inverse-iteration step on the restored modes, its
+    # Schwarz cycles on FP16 fields
+    lib.oqh_set_sap_prec(1)
     lib.dfl_update(1, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
+    lib.oqh_set_sap_prec(0)
     print("dfl_update ifail", ifail[0], "status", st[:2].tolist(), flush=True)
     assert ifail[0] == 0
     lib.dfl_sap_gcr(16, 256, 0, 1e-10, 0.0, e, p, ifail.ctypes.data_as(P), st.ctypes.data_as(P))
