@@ -249,6 +249,40 @@ def deflated_solves(lat, e, p, r, pref=None):
     return dfl
 
 
+def sap_cycles(lat, src, ncy=20):
+    """Schwarz cycles alone on the 48^4 solver workload (4^4 blocks,
+    isolv=1, nmr=4): ncy calls of sap() in each precision on the residual of
+    a Gaussian source, timed with device synchronisation; algorithmic flops
+    per cycle 2 x V/2 x ((nmr+1) x 2076 + 48) (block MinRes with the
+    even-odd block operator, nmr+1 applications of it per colour, and the
+    face update)."""
+    import ctypes as C
+    P = C.c_void_p
+    lib, V = lat.lib, lat.V
+    lib.sap.argtypes = [C.c_float, C.c_int, C.c_int, P, P]
+    lib.set_s2zero.argtypes = [C.c_int, C.c_int, P]
+    lib.assign_sd2s.argtypes = [C.c_int, C.c_int, P, P]
+    psi, eta = lat.alloc_s(), lat.alloc_s()
+    flops = 2 * (V // 2) * (5 * 2076 + 48)
+    rec = {"workload": "48^4 solver field, SAP 4^4 blocks isolv=1 nmr=4, %d cycles per precision" % ncy,
+           "gflop_per_cycle": round(flops / 1e9, 2)}
+    for name, prec in (("fp32", 0), ("fp16_storage", 1), ("fp64", 2)):
+        lib.oqh_set_sap_prec(prec)
+        lib.set_s2zero(V, 0, psi)
+        lib.assign_sd2s(V, 0, src, eta)
+        lib.sap(0.0, 1, 4, psi, eta)     # warm-up (block fields, FP16 copies)
+        lat.sync()
+        t0 = time.perf_counter()
+        for _ in range(ncy):
+            lib.sap(0.0, 1, 4, psi, eta)
+        lat.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / ncy
+        rec[name] = {"ms_per_cycle": round(ms, 3), "algorithmic_TFLOPs": round(flops / ms / 1e9, 2)}
+        log("bench: SAP cycle %s %.3f ms (%.1f TFLOP/s algorithmic)" % (name, ms, flops / ms / 1e9))
+    lib.oqh_set_sap_prec(0)
+    return rec
+
+
 def solver_lattice():
     """The 48^4 solver workload's lattice, field, SAP parameters and a
     Gaussian source (see solve_times)."""
@@ -537,6 +571,8 @@ def main():
     ap.add_argument("--solve-only", action="store_true",
                     help="only the solver section (training, reference solve, variants): "
                          "prints its JSON object")
+    ap.add_argument("--sap-only", action="store_true",
+                    help="only Schwarz cycles on the 48^4 solver workload (profiling): prints their JSON object")
     ap.add_argument("--deflated-only", action="store_true",
                     help="only the deflated solves of the 48^4 solver workload: prints their JSON object")
     args = ap.parse_args()
@@ -547,6 +583,14 @@ def main():
 
     import numpy as np
     import pyoqh
+
+    if args.sap_only:
+        lat, e, p, r = solver_lattice()
+        try:
+            print(json.dumps({"sap_cycles": sap_cycles(lat, e)}))
+        finally:
+            lat.close()
+        return
 
     if args.deflated_only:
         lat, e, p, r = solver_lattice()

@@ -707,18 +707,22 @@ extern "C" void oqh_sync(void)
 }
 
 // plain device copy (16-byte accesses, four per lane with the loads issued
-// before the stores) used to measure the achievable HBM bandwidth and to
-// calibrate the byte counters of profiles
+// before the stores; non-temporal loads and stores, the fastest of the copy
+// forms measured on MI355X) used to measure the achievable HBM bandwidth and
+// to calibrate the byte counters of profiles
+typedef double v2d_t __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n)
 {
+   const v2d_t *av = (const v2d_t *)a;
+   v2d_t *bv = (v2d_t *)b;
    size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
-   double2 v[4];
+   v2d_t v[4];
 #pragma unroll
    for (int k = 0; k < 4; k++)
-      if (base + 256 * k < n) v[k] = a[base + 256 * k];
+      if (base + 256 * k < n) v[k] = __builtin_nontemporal_load(av + base + 256 * k);
 #pragma unroll
    for (int k = 0; k < 4; k++)
-      if (base + 256 * k < n) b[base + 256 * k] = v[k];
+      if (base + 256 * k < n) __builtin_nontemporal_store(v[k], bv + base + 256 * k);
 }
 
 extern "C" double oqh_copy_bandwidth(double gbytes, int nrep)
