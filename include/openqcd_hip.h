@@ -422,6 +422,21 @@ void Dwoo_blk(blk_grid_t grid, int n, float mu, int k, int l);
 void Dwoe_blk(blk_grid_t grid, int n, int k, int l);
 void Dweo_blk(blk_grid_t grid, int n, int k, int l);
 void Dwhat_blk(blk_grid_t grid, int n, float mu, int k, int l);
+/* include/block.h:77-80 and include/dirac.h:48-53 (block/map_s2blk.c,
+   dirac/Dw_dble.c:1891-2290): double-precision block fields sd[0..2] of the
+   library's block grid and the block operators on them, with the FP64
+   links and the FP64 block SW term (inverted on the point set of
+   assign_swd2swbgr / the SAP solver, as the single-precision one) */
+void assign_sd2sdblk(blk_grid_t grid, int n, ptset_t set, spinor_dble *sd, int k);
+void assign_sdblk2sd(blk_grid_t grid, int n, ptset_t set, int k, spinor_dble *sd);
+void Dw_blk_dble(blk_grid_t grid, int n, double mu, int k, int l);
+void Dwee_blk_dble(blk_grid_t grid, int n, double mu, int k, int l);
+void Dwoo_blk_dble(blk_grid_t grid, int n, double mu, int k, int l);
+void Dwoe_blk_dble(blk_grid_t grid, int n, int k, int l);
+void Dweo_blk_dble(blk_grid_t grid, int n, int k, int l);
+void Dwhat_blk_dble(blk_grid_t grid, int n, double mu, int k, int l);
+/* test access: FP64 block field k of block n in block order */
+void oqh_export_sdblk(int n, int k, spinor_dble *host);
 /* include/dirac.h:25 (dirac/Dw_bnd.c:801-1011): the Weyl fields of the
    hopping terms from block n to its exterior, face by face, into the
    block's boundary field l (SAP_BLOCKS has one, l = 0); read them with

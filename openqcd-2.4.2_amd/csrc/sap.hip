@@ -60,6 +60,8 @@ struct SapGrid
    float2 *d_scr = nullptr;              // [nbh][3][Ub]
    float2 *d_s0g = nullptr;              // global field of block corrections
    float2 *d_bs = nullptr;               // block fields of the API, [nb][3][Ub]
+   double2 *d_bsd = nullptr;             // FP64 block fields of the API, [nb][NBF][Ub]
+   bool need64 = false;                  // FP64 block operators in use: keep d_swb64
    __half2 *d_u16 = nullptr;             // FP16 links (FP16 variant of the solver)
    __half2 *d_swb16 = nullptr;           // FP16 block SW field
    float2 *d_s0gm = nullptr;             // block corrections of 16 sources
@@ -1116,12 +1118,14 @@ __global__ __launch_bounds__(256) void k_sap_faces(SArgs a0, const int *face, co
 
 enum BlkOp { BOP_DW = 0, BOP_DWHAT, BOP_MRES, BOP_EO_MRES, BOP_DWEE, BOP_DWOO, BOP_DWOE, BOP_DWEO };
 
-__global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, int l, float mu)
+template <typename C>
+__global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int k, int l, decltype(C::x) mu)
 {
+   typedef decltype(C::x) R;
    const int *imb = a.imb + (size_t)n * a.bvol;
-   float2 *F = a.scr + (size_t)n * NBF * a.Ub;  // the block's fields 0,1,2
-   float2 *Sk = F + (size_t)k * a.Ub, *Sl = F + (size_t)l * a.Ub;
-   float2 *T = F + 3 * a.Ub;  // odd intermediate of Dwhat_blk
+   C *F = a.scr + (size_t)n * NBF * a.Ub;  // the block's fields 0,1,2
+   C *Sk = F + (size_t)k * a.Ub, *Sl = F + (size_t)l * a.Ub;
+   C *T = F + 3 * a.Ub;  // odd intermediate of Dwhat_blk
    // open bc: the operators zero their input on the boundary points (all of
    // them for Dw_blk, the even ones for Dwhat_blk, Dwee_blk and Dwoe_blk,
    // the odd ones for Dwoo_blk and Dweo_blk); in the MinRes solvers the
@@ -1130,7 +1134,7 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
    {
       const bool even = (op == BOP_DWHAT || op == BOP_EO_MRES || op == BOP_DWEE || op == BOP_DWOE);
       const bool odd = (op == BOP_DWOO || op == BOP_DWEO);
-      float2 *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
+      C *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
       for (int i = threadIdx.x; i < a.bvol; i += SWG)
          if (a.bnd[imb[i]] && (!even || i < a.bh) && (!odd || i >= a.bh)) zero_spinor(In, i);
       __syncthreads();
@@ -1146,16 +1150,16 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
    {
       // (A + i mu gamma5) on one parity (Dw.c:1584-1694); the fields may coincide
       const bool ee = op == BOP_DWEE;
-      const float m = ee ? mu : a.muo;
+      const R m = ee ? mu : a.muo;
       for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += SWG)
       {
          const int gx = imb[i];
-         float2 v[12], o[12];
+         C v[12], o[12];
          if (a.masked && a.bnd[gx])
             zero12(o);
          else
          {
-            load_spinor((const float2 *)Sk, i, v);
+            load_spinor((const C *)Sk, i, v);
             pauli_site(a.sw, gx, m, v, o);
          }
          store_spinor(Sl, i, o);
@@ -1170,23 +1174,23 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgs a, int n, int op, int k, i
       for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += SWG)
       {
          const int gx = imb[i];
-         float2 acc[12], o[12];
+         C acc[12], o[12];
          zero12(acc);
          if (a.masked && a.bnd[gx])
             zero12(o);
          else
          {
-            blk_hop(a, imb, (const float2 *)Sk, i, gx, acc);
+            blk_hop(a, imb, (const C *)Sk, i, gx, acc);
             if (oe)
             {
 #pragma unroll
-               for (int c = 0; c < 12; c++) o[c] = cscale(acc[c], -0.5f);
+               for (int c = 0; c < 12; c++) o[c] = cscale(acc[c], (R)-0.5);
             }
             else
             {
-               load_spinor((const float2 *)Sl, i, o);
+               load_spinor((const C *)Sl, i, o);
 #pragma unroll
-               for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], 0.5f));
+               for (int c = 0; c < 12; c++) o[c] = cadd(o[c], cscale(acc[c], (R)0.5));
             }
          }
          store_spinor(Sl, i, o);
@@ -1284,20 +1288,20 @@ __global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int
    }
 }
 
-__global__ void k_blk_copy(float2 *glob, float2 *blk, const int *imb, int i0, int i1, int to_blk)
+template <typename C> __global__ void k_blk_copy(C *glob, C *blk, const int *imb, int i0, int i1, int to_blk)
 {
    int i = i0 + blockIdx.x * 256 + threadIdx.x;
    if (i >= i1) return;
    int gx = imb[i];
-   float2 v[12];
+   C v[12];
    if (to_blk)
    {
-      load_spinor((const float2 *)glob, gx, v);
+      load_spinor((const C *)glob, gx, v);
       store_spinor(blk, i, v);
    }
    else
    {
-      load_spinor((const float2 *)blk, i, v);
+      load_spinor((const C *)blk, i, v);
       store_spinor(glob, gx, v);
    }
 }
@@ -1318,7 +1322,7 @@ static void free_grid()
    dev_free(G.d_swb); dev_free(G.d_scr); dev_free(G.d_s0g); dev_free(G.d_bs);
    dev_free(G.d_u16); dev_free(G.d_swb16);
    for (int k = 0; k < 3; k++) { dev_free(G.d_h16[k]); dev_free(G.d_hsc[k]); dev_free(G.d_h64[k]); }
-   dev_free(G.d_swb64); dev_free(G.d_scr64); dev_free(G.d_s0gm);
+   dev_free(G.d_swb64); dev_free(G.d_scr64); dev_free(G.d_s0gm); dev_free(G.d_bsd);
    dev_free(G.d_ipp); dev_free(G.d_bw);
    G = SapGrid();
 }
@@ -1486,7 +1490,7 @@ static int assign_swb(int set)
    if (set == ALL_PTS || set == ODD_PTS) ifail |= invert_pauli(tmp, tmp, h, h, 0.0);
    convert_f64_to_f32(tmp, G.d_swb, nun, L.stream);
    G.swb_version++;
-   if (L.sap_prec == 2)
+   if (L.sap_prec == 2 || G.need64)
    {
       if (!G.d_swb64) G.d_swb64 = (double2 *)dev_alloc(nun * sizeof(double2));
       if (G.d_swb64)
@@ -1990,7 +1994,7 @@ int sap_grid_nblocks() { return G.ready ? G.nb : 0; }
 void sap_release()
 {
    drop_sap_graph();
-   if (G.ready || G.d_bs) free_grid();
+   if (G.ready || G.d_bs || G.d_bsd) free_grid();
 }
 
 }  // namespace oqh
@@ -2234,6 +2238,134 @@ extern "C" void Dweo_blk(blk_grid_t grid, int n, int k, int l)
 {
    (void)grid;
    blk_op(n, BOP_DWEO, k, l, 0.0f, 0, "Dweo_blk [Dw.c]");
+}
+
+// ---------------------------------------------------------------------------
+// double-precision block fields and operators (block/map_s2blk.c:
+// assign_sd2sdblk, assign_sdblk2sd; dirac/Dw_dble.c:1891-2290): the same
+// block kernels on FP64 block fields with the FP64 links and the FP64 block
+// SW field (inverted on the point set of the single-precision one)
+// ---------------------------------------------------------------------------
+
+static bool blk_check64(int n, int k, const char *where)
+{
+   if (!blk_check(n, k, where)) return false;
+   if (!G.d_bsd) G.d_bsd = (double2 *)dev_alloc((size_t)G.nb * NBF * G.Ub * sizeof(double2));
+   if (!G.d_bsd) return false;
+   return true;
+}
+
+static void blk_copy64(int n, ptset_t set, int k, spinor_dble *sd, int to_blk, const char *where)
+{
+   Lib &L = lib();
+   if (!blk_check64(n, k, where)) return;
+   Res r = resolve(sd, F64, L.g.V, 0, true);
+   if (!r.dev) return;
+   int i0 = (set == ODD_PTS) ? G.bh : 0, i1 = (set == EVEN_PTS) ? G.bh : G.bvol;
+   if (set == NO_PTS) i1 = i0;
+   double2 *blk = G.d_bsd + ((size_t)n * NBF + k) * G.Ub;
+   if (i1 > i0)
+      k_blk_copy<<<(i1 - i0 + 255) / 256, 256, 0, L.stream>>>((double2 *)r.dev, blk, G.d_imb + (size_t)n * G.bvol,
+                                                              i0, i1, to_blk);
+   finish(r, F64, L.g.V, !to_blk);
+}
+
+extern "C" void assign_sd2sdblk(blk_grid_t grid, int n, ptset_t set, spinor_dble *sd, int k)
+{
+   (void)grid;
+   blk_copy64(n, set, k, sd, 1, "assign_sd2sdblk [map_s2blk.c]");
+}
+
+extern "C" void assign_sdblk2sd(blk_grid_t grid, int n, ptset_t set, int k, spinor_dble *sd)
+{
+   (void)grid;
+   blk_copy64(n, set, k, sd, 0, "assign_sdblk2sd [map_s2blk.c]");
+}
+
+static void blk_op64(int n, int op, int k, int l, double mu, const char *where)
+{
+   Lib &L = lib();
+   if (!blk_check64(n, k, where) || !blk_check64(n, l, where)) return;
+   if (!L.ud_set)
+   {
+      fail(where, "gauge field is not set");
+      return;
+   }
+   ensure_u();
+   G.need64 = true;
+   if (G.swb_set < 0 || !L.swd_valid || G.swb_epoch != L.swd_epoch || !G.d_swb64 ||
+       G.swb64_version != G.swb_version)
+   {
+      int set = (G.swb_set >= 0) ? G.swb_set : (L.sap_isolv ? ODD_PTS : NO_PTS);
+      if (assign_swb(set) || !G.d_swb64)
+      {
+         G.swb_set = -1;
+         fail(where, "The inversion of the SW term was not safe");
+         return;
+      }
+   }
+   SArgsD a = sargs64(mu, 0);
+   a.scr = G.d_bsd;
+   k_blk_op<<<1, SWG, 0, L.stream>>>(a, n, op, k, l, mu);
+   HIP_OK(hipStreamSynchronize(L.stream));
+}
+
+extern "C" void Dw_blk_dble(blk_grid_t grid, int n, double mu, int k, int l)
+{
+   (void)grid;
+   if (k == l)
+   {
+      fail("Dw_blk_dble [Dw_dble.c]", "Input and output fields must be different");
+      return;
+   }
+   blk_op64(n, BOP_DW, k, l, mu, "Dw_blk_dble [Dw_dble.c]");
+}
+
+extern "C" void Dwhat_blk_dble(blk_grid_t grid, int n, double mu, int k, int l)
+{
+   (void)grid;
+   if (k == l)
+   {
+      fail("Dwhat_blk_dble [Dw_dble.c]", "Input and output fields must be different");
+      return;
+   }
+   blk_op64(n, BOP_DWHAT, k, l, mu, "Dwhat_blk_dble [Dw_dble.c]");
+}
+
+extern "C" void Dwee_blk_dble(blk_grid_t grid, int n, double mu, int k, int l)
+{
+   (void)grid;
+   blk_op64(n, BOP_DWEE, k, l, mu, "Dwee_blk_dble [Dw_dble.c]");
+}
+
+extern "C" void Dwoo_blk_dble(blk_grid_t grid, int n, double mu, int k, int l)
+{
+   (void)grid;
+   blk_op64(n, BOP_DWOO, k, l, mu, "Dwoo_blk_dble [Dw_dble.c]");
+}
+
+extern "C" void Dwoe_blk_dble(blk_grid_t grid, int n, int k, int l)
+{
+   (void)grid;
+   blk_op64(n, BOP_DWOE, k, l, 0.0, "Dwoe_blk_dble [Dw_dble.c]");
+}
+
+extern "C" void Dweo_blk_dble(blk_grid_t grid, int n, int k, int l)
+{
+   (void)grid;
+   blk_op64(n, BOP_DWEO, k, l, 0.0, "Dweo_blk_dble [Dw_dble.c]");
+}
+
+// FP64 block field k of block n as a host array in block order (test access)
+extern "C" void oqh_export_sdblk(int n, int k, spinor_dble *host)
+{
+   Lib &L = lib();
+   if (!blk_check64(n, k, "oqh_export_sdblk")) return;
+   size_t bytes = (size_t)G.bvol * sizeof(spinor_dble);
+   void *aos = tmp_buffer(bytes);
+   blocks_to_aos_f64(G.d_bsd + ((size_t)n * NBF + k) * G.Ub, (double2 *)aos, G.bvol, 12, L.stream);
+   HIP_OK(hipMemcpyAsync(host, aos, bytes, hipMemcpyDeviceToHost, L.stream));
+   HIP_OK(hipStreamSynchronize(L.stream));
 }
 
 extern "C" void Dw_bnd(blk_grid_t grid, int n, int k, int l)

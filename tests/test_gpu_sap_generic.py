@@ -53,6 +53,13 @@ def _sig(lib):
     lib.blk_eo_mres.argtypes = [C.c_int, C.c_float, C.c_int]
     lib.oqh_export_sblk.argtypes = [C.c_int, C.c_int, P]
     lib.oqh_get_imb.argtypes = [C.c_int, P]
+    lib.assign_sd2sdblk.argtypes = [C.c_int, C.c_int, C.c_int, P, C.c_int]
+    lib.assign_sdblk2sd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P]
+    for nm in ("Dw_blk_dble", "Dwhat_blk_dble", "Dwee_blk_dble", "Dwoo_blk_dble"):
+        getattr(lib, nm).argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
+    for nm in ("Dwoe_blk_dble", "Dweo_blk_dble"):
+        getattr(lib, nm).argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.oqh_export_sdblk.argtypes = [C.c_int, C.c_int, P]
 
 
 @pytest.fixture(scope="module", params=CFGS, ids=lambda c: c["name"])
@@ -143,6 +150,74 @@ def test_generic_block_operators(setup):
             assert rel0 < 1e-4 and rel1 < 1e-3
     lat.free(s)
     lat.free(t)
+
+
+@pytest.mark.gpu
+def test_block_operators_dble(setup):
+    """The FP64 block operators (dirac/Dw_dble.c:1891-2290) on FP64 block
+    fields against the oracle's FP64 restatement, on the first and the last
+    block: Dw_blk_dble, Dwhat_blk_dble (odd half of the output unchanged),
+    Dwee/Dwoo/Dwoe/Dweo_blk_dble with the input's boundary zeros; and the
+    round trip assign_sd2sdblk / assign_sdblk2sd (bit-exact)."""
+    c, o, eta, s32, lat = setup
+    lib = lat.lib
+    lib.oqh_set_sap_prec(0)
+    mu = c["mu"]
+    other = np.random.default_rng(9).standard_normal(eta.shape)
+    s, t, u = lat.alloc_sd(), lat.alloc_sd(), lat.alloc_sd()
+    lat.put_sd(s, eta)
+    lat.put_sd(t, other)
+    nb = o.nblocks()
+
+    def block(n, k, bvol):
+        a = np.empty((bvol, 24))
+        lib.oqh_export_sdblk(n, k, a.ctypes.data_as(P))
+        return a
+
+    for n in (0, nb - 1):
+        imb = o.block_imb(n)
+        bvol, bh = len(imb), len(imb) // 2
+        s1 = np.vstack([eta[imb], np.zeros((1, 24))])
+        # block fields in and out of the global field
+        lib.assign_sd2sdblk(0, n, 0, s, 1)
+        assert np.array_equal(block(n, 1, bvol), eta[imb])
+        lat.put_sd(u, other)
+        lib.assign_sdblk2sd(0, n, 0, 1, u)
+        back = lat.get_sd(u)
+        assert np.array_equal(back[imb], eta[imb])
+        rest = np.ones(len(back), bool)
+        rest[imb] = False
+        assert np.array_equal(back[rest], other[rest])
+        # Dw_blk_dble
+        lib.Dw_blk_dble(0, n, mu, 1, 2)
+        ref = o.Dw_blk_dble(n, mu, s1.copy())[:-1]
+        rel, absz = site_rel(block(n, 2, bvol), ref)
+        print("block", n, "Dw_blk_dble", rel, flush=True)
+        assert rel < 1e-13 and absz == 0.0
+        if c["isolv"]:
+            lib.assign_sd2sdblk(0, n, 0, t, 2)
+            before = block(n, 2, bvol)
+            lib.Dwhat_blk_dble(0, n, mu, 1, 2)
+            out = block(n, 2, bvol)
+            ref = o.Dwhat_blk_dble(n, mu, s1.copy())[:bh]
+            rel, absz = site_rel(out[:bh], ref)
+            print("block", n, "Dwhat_blk_dble", rel, flush=True)
+            assert rel < 1e-13 and absz == 0.0
+            assert np.array_equal(out[bh:], before[bh:])
+        calls = [lambda: lib.Dwee_blk_dble(0, n, mu, 1, 2), lambda: lib.Dwoo_blk_dble(0, n, mu, 1, 2),
+                 lambda: lib.Dwoe_blk_dble(0, n, 1, 2), lambda: lib.Dweo_blk_dble(0, n, 1, 2)]
+        for op, call in enumerate(calls):
+            lib.assign_sd2sdblk(0, n, 0, t, 1)
+            lib.assign_sd2sdblk(0, n, 0, s, 2)
+            call()
+            s_ref, r_ref = o.blk_eo_op_dble(n, op, mu, np.vstack([other[imb], np.zeros((1, 24))]), s1.copy())
+            rel, absz = site_rel(block(n, 2, bvol), r_ref[:-1])
+            rels, abss = site_rel(block(n, 1, bvol), s_ref[:-1])
+            print("block", n, "eo op", op, rel, rels, flush=True)
+            assert rel < 1e-13 and absz == 0.0 and rels == 0.0 and abss == 0.0
+    lat.free(s)
+    lat.free(t)
+    lat.free(u)
 
 
 @pytest.mark.gpu
