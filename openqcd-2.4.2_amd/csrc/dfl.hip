@@ -66,6 +66,7 @@ struct Dfl
    int nb = 0, nbh = 0, bvol = 0;
    int *d_bpts = nullptr;    // [nb][bvol] points of each block
    int *d_inn = nullptr;     // [nb][8] neighbouring blocks
+   int *d_blk = nullptr;     // [V] block of every point
    std::vector<int> inn;     // host copy
    std::vector<int> bpts;    // host copy
    // subspace
@@ -100,6 +101,9 @@ struct Dfl
    double2 *d_lB = nullptr;     // [Ns][Ns] (w_l, Ahat w_j) at column j
    double2 *d_lpart = nullptr;  // [Ns][LNBK] work-group shares
    double *d_lbc = nullptr;     // [Ns][4] Gram-Schmidt norms
+   // Gram-Schmidt of the global modes (dfl_renormalize_modes)
+   float2 *d_gcoef = nullptr, *d_ggram = nullptr, *d_gscr = nullptr;
+   int gcap = 0;
 };
 
 Dfl D;
@@ -171,6 +175,7 @@ bool dfl_grid(const char *where)
    }
    D.d_bpts = (int *)dev_alloc(sizeof(int) * D.bpts.size());
    D.d_inn = (int *)dev_alloc(sizeof(int) * D.inn.size());
+   D.d_blk = (int *)dev_alloc(sizeof(int) * L.g.V);
    D.fu = units_for(L.g.V, 12);
    D.d_basis = (float2 *)dev_alloc(sizeof(float2) * D.fu * D.Ns);
    D.d_vmds = (float2 *)dev_alloc(sizeof(float2) * (size_t)D.Ns * D.nb * D.Ns);
@@ -184,7 +189,7 @@ bool dfl_grid(const char *where)
    D.d_unsafe = (int *)dev_alloc(sizeof(int));
    D.d_red = (double *)dev_alloc(sizeof(double) * 4 * 1024);
    HIP_OK(hipHostMalloc((void **)&D.h_red, sizeof(double) * 4 * 1024, 0));
-   if (!D.d_bpts || !D.d_inn || !D.d_basis || !D.d_vmds || !D.d_s64 || !D.d_Ablk || !D.d_Ahop || !D.d_AinvT ||
+   if (!D.d_bpts || !D.d_inn || !D.d_blk || !D.d_basis || !D.d_vmds || !D.d_s64 || !D.d_Ablk || !D.d_Ahop || !D.d_AinvT ||
        !D.d_MT || !D.d_MT32 || !D.d_unsafe || !D.d_red)
    {
       dfl_free_grid();
@@ -192,6 +197,12 @@ bool dfl_grid(const char *where)
    }
    HIP_OK(hipMemcpy(D.d_bpts, D.bpts.data(), sizeof(int) * D.bpts.size(), hipMemcpyHostToDevice));
    HIP_OK(hipMemcpy(D.d_inn, D.inn.data(), sizeof(int) * D.inn.size(), hipMemcpyHostToDevice));
+   {
+      std::vector<int> blk(L.g.V);
+      for (int n = 0; n < D.nb; n++)
+         for (int i = 0; i < D.bvol; i++) blk[D.bpts[(size_t)n * D.bvol + i]] = n;
+      HIP_OK(hipMemcpy(D.d_blk, blk.data(), sizeof(int) * L.g.V, hipMemcpyHostToDevice));
+   }
    HIP_OK(hipMemset(D.d_basis, 0, sizeof(float2) * D.fu * D.Ns));
    D.grid = true;
    return true;
@@ -296,6 +307,66 @@ __global__ __launch_bounds__(DWG) void k_blk_prolong(const float2 *basis, size_t
 // block Gram-Schmidt of mode l (dfl_subspace): S = mds_l in double, S -=
 // (B_j, S) B_j for j < l (modified Gram-Schmidt), B_l = S / |S|; the
 // coordinates of mds_l in the basis go to vm[n][0..Ns) (complex float)
+// the same for blocks of at most DWG points: one point per lane, the FP64
+// work spinor in registers, so that only the basis vectors are read (twice
+// per previous vector, the second time from the cache)
+__global__ __launch_bounds__(DWG) void k_blk_gs_reg(float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
+                                                     const float2 *mds, int l, float2 *vmds, int *degen)
+{
+   __shared__ double2 sh[DWG / 64];
+   const int n = blockIdx.x;
+   const int *pts = bpts + (size_t)n * bvol;
+   float2 *vm = vmds + ((size_t)l * gridDim.x + n) * Ns;
+   const int i = threadIdx.x;
+   const bool act = i < bvol;
+   const int x = act ? pts[i] : 0;
+   double2 d[12];
+   {
+      float2 v[12];
+      if (act) load_spinor(mds, x, v);
+#pragma unroll
+      for (int c = 0; c < 12; c++) d[c] = act ? make_double2(v[c].x, v[c].y) : make_double2(0.0, 0.0);
+   }
+   for (int j = 0; j < l; j++)
+   {
+      const float2 *Bj = basis + (size_t)j * fu;
+      float2 v[12];
+      double2 z = make_double2(0.0, 0.0);
+      if (act)
+      {
+         load_spinor(Bj, x, v);
+         z = sdot(v, d);
+      }
+      z = wg_sum(z, sh);
+      if (threadIdx.x == 0) vm[j] = make_float2((float)z.x, (float)z.y);
+      if (act)
+      {
+#pragma unroll
+         for (int c = 0; c < 12; c++)
+            d[c] = make_double2(d[c].x - (z.x * v[c].x - z.y * v[c].y), d[c].y - (z.x * v[c].y + z.y * v[c].x));
+      }
+   }
+   double2 r2 = make_double2(0.0, 0.0);
+#pragma unroll
+   for (int c = 0; c < 12; c++) r2.x += d[c].x * d[c].x + d[c].y * d[c].y;
+   r2 = wg_sum(r2, sh);
+   const double r = sqrt(r2.x);
+   if (threadIdx.x == 0)
+   {
+      vm[l] = make_float2((float)r, 0.0f);
+      for (int j = l + 1; j < Ns; j++) vm[j] = make_float2(0.0f, 0.0f);
+      if ((float)r == 0.0f) atomicOr(degen, 1);
+   }
+   const double f = r > 0.0 ? 1.0 / r : 0.0;
+   if (act)
+   {
+      float2 v[12];
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = make_float2((float)(d[c].x * f), (float)(d[c].y * f));
+      store_spinor(basis + (size_t)l * fu, x, v);
+   }
+}
+
 __global__ __launch_bounds__(DWG) void k_blk_gs(float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
                                                  const float2 *mds, double2 *S, int l, float2 *vmds, int *degen)
 {
@@ -1148,13 +1219,13 @@ template <typename C> bool lkry_alloc(LKry<C> &K, int nmx, size_t nv)
 
 void dfl_free_grid()
 {
-   dev_free(D.d_bpts); dev_free(D.d_inn); dev_free(D.d_basis); dev_free(D.d_vmds); dev_free(D.d_s64);
+   dev_free(D.d_bpts); dev_free(D.d_inn); dev_free(D.d_blk); dev_free(D.d_basis); dev_free(D.d_vmds); dev_free(D.d_s64);
    dev_free(D.d_Ablk); dev_free(D.d_Ahop); dev_free(D.d_AinvT); dev_free(D.d_MT); dev_free(D.d_MT32);
    dev_free(D.d_unsafe); dev_free(D.d_red);
    for (auto p : D.wvd) dev_free(p);
    for (auto p : D.wv) dev_free(p);
    if (D.h_red) (void)hipHostFree(D.h_red);
-   D.d_bpts = D.d_inn = nullptr;
+   D.d_bpts = D.d_inn = D.d_blk = nullptr;
    D.d_basis = nullptr; D.d_vmds = nullptr; D.d_s64 = nullptr;
    D.d_Ablk = D.d_Ahop = D.d_AinvT = D.d_MT = nullptr;
    D.d_MT32 = nullptr; D.d_unsafe = nullptr; D.d_red = nullptr; D.h_red = nullptr;
@@ -1167,6 +1238,9 @@ void dfl_free_grid()
    D.d_lw = D.d_law = nullptr; D.d_lw32 = D.d_law32 = nullptr; D.d_lM = D.d_lc = D.d_ld = D.d_lB = nullptr;
    D.d_lpart = nullptr; D.d_lbc = nullptr;
    D.lm_ok = false;
+   dev_free(D.d_gcoef); dev_free(D.d_ggram); dev_free(D.d_gscr);
+   D.d_gcoef = D.d_ggram = D.d_gscr = nullptr;
+   D.gcap = 0;
 }
 
 // step n of the Gram-Schmidt process on chi_n (see above); returns false if
@@ -1627,8 +1701,12 @@ extern "C" void dfl_subspace(spinor **mds)
          fail(where, "the modes must be full fields");
          return;
       }
-      k_blk_gs<<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, (const float2 *)r.dev,
-                                           D.d_s64, l, D.d_vmds, D.d_unsafe);
+      if (D.bvol <= DWG)
+         k_blk_gs_reg<<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, (const float2 *)r.dev, l,
+                                                  D.d_vmds, D.d_unsafe);
+      else
+         k_blk_gs<<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, (const float2 *)r.dev,
+                                              D.d_s64, l, D.d_vmds, D.d_unsafe);
       finish(r, F32, L.g.V, false);
    }
    int degen = 0;
@@ -1670,15 +1748,50 @@ extern "C" void dfl_vd2s(complex_dble *vd, spinor *s)
    finish(r, F32, L.g.V, true);
 }
 
+// Gram-Schmidt of the global modes (dfl/dfl_subspace.c:261-276).  Device
+// fields: per mode one pass forms its products with the previous modes and
+// the Gram row of the newest one, a triangular solve turns them into the
+// coefficients of the modified process, and one pass applies them and
+// normalises (the compact form of the solver's Gram-Schmidt); host arrays
+// take the reference's mode-by-mode project/normalize
 extern "C" void dfl_renormalize_modes(spinor **mds)
 {
    Lib &L = lib();
-   const int vt = L.g.VT;
-   for (int l = 0; l < D.Ns; l++)
+   const int vt = L.g.VT, Ns = D.Ns;
+   std::vector<float2 *> q(Ns, nullptr);
+   bool dev = Ns > 0;
+   for (int l = 0; l < Ns && dev; l++)
    {
-      for (int j = 0; j < l; j++) project(vt, 3, mds[l], mds[j]);
-      (void)normalize(vt, 3, mds[l]);
+      int off = 0;
+      q[l] = (float2 *)device_field(mds[l], F32, &off);
+      dev = q[l] != nullptr && off == 0;
    }
+   float2 *&coef = D.d_gcoef, *&gram = D.d_ggram, *&scr = D.d_gscr;
+   int &cap = D.gcap;
+   if (dev && cap < Ns)
+   {
+      dev_free(coef);
+      dev_free(gram);
+      dev_free(scr);
+      coef = (float2 *)dev_alloc(sizeof(float2) * (Ns + 3));
+      gram = (float2 *)dev_alloc(sizeof(float2) * Ns * Ns);
+      scr = (float2 *)dev_alloc(sizeof(float2) * units_for(L.g.V, 12));
+      cap = Ns;
+      if (!coef || !gram || !scr)
+         cap = 0;
+      else  // the solver's residual slot: zero, so it stays zero
+         HIP_OK(hipMemsetAsync(scr, 0, sizeof(float2) * units_for(L.g.V, 12), L.stream));
+   }
+   if (!dev || cap == 0)
+   {
+      for (int l = 0; l < Ns; l++)
+      {
+         for (int j = 0; j < l; j++) project(vt, 3, mds[l], mds[j]);
+         (void)normalize(vt, 3, mds[l]);
+      }
+      return;
+   }
+   for (int l = 0; l < Ns; l++) gs_classical(q[l], q.data(), l, scr, coef, cap, cap + 1, cap + 2, L.g.V, gram, cap);
 }
 
 // basis vector k as a host array in the reference's layout (test access)
@@ -1752,19 +1865,6 @@ extern "C" void set_Aw(double mu)
        L.swd_valid && !L.swd_inv[0] && !L.swd_inv[1])
       return;
    ensure_swd(0, 0);
-   static int *d_blk = nullptr;
-   static int blk_n = 0;
-   if (!d_blk || blk_n != L.g.V)
-   {
-      dev_free(d_blk);
-      std::vector<int> blk(L.g.V);
-      for (int n = 0; n < D.nb; n++)
-         for (int i = 0; i < D.bvol; i++) blk[D.bpts[(size_t)n * D.bvol + i]] = n;
-      d_blk = (int *)dev_alloc(sizeof(int) * L.g.V);
-      if (!d_blk) return;
-      HIP_OK(hipMemcpy(d_blk, blk.data(), sizeof(int) * L.g.V, hipMemcpyHostToDevice));
-      blk_n = L.g.V;
-   }
    AwArgs a;
    memset(&a, 0, sizeof(a));
    a.basis = D.d_basis;
@@ -1783,7 +1883,7 @@ extern "C" void set_Aw(double mu)
    a.mu = mu;
    a.Ablk = D.d_Ablk;
    a.Ahop = D.d_Ahop;
-   for (int l = 0; l < D.Ns; l++) k_aw_column<<<D.nb, DWG, 0, L.stream>>>(a, d_blk, l);
+   for (int l = 0; l < D.Ns; l++) k_aw_column<<<D.nb, DWG, 0, L.stream>>>(a, D.d_blk, l);
    HIP_OK(hipStreamSynchronize(L.stream));
    D.aw_ok = true;
    D.awhat_ok = false;

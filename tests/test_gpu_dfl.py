@@ -95,6 +95,7 @@ def setup():
     lat = pyoqh.Lattice(L, T, bc=3, m0=M0, csw=1.77)
     _sig(lat.lib)
     lat.lib.dfl_restore_modes.argtypes = [P]
+    lat.lib.dfl_renormalize_modes.argtypes = [P]
     lat.lib.dfl_update.argtypes = [C.c_int, P, P]
     lat.import_ud(o.ud())
     assert lat.sw_term(3) == 0
@@ -250,3 +251,81 @@ def test_deflated_solver(setup):
     assert ifail[0] == 0 and ifail[1] == 0 and res <= 1.01e-10
     lat.free(e)
     lat.free(p)
+
+
+@pytest.mark.gpu
+def test_subspace_large_blocks(setup):
+    """Blocks of more points than a work-group has lanes (8x4x4x4 = 512):
+    the block Gram-Schmidt keeps its work spinor in global memory there; the
+    basis against the oracle's Gram-Schmidt and the little operator against
+    its definition, as for 4^4 blocks."""
+    o, eta, s32, lat = setup
+    lib, V = lat.lib, lat.V
+    lib.oqh_dfl_release()
+    bs = np.array((8, 4, 4, 4), np.int32)
+    assert lib.set_dfl_parms(bs.ctypes.data_as(P), NS).Ns == NS
+    lib.set_dfl_pro_parms(16, 256, 1e-10, 24, 1e-2)
+    rng = np.random.default_rng(21)
+    mds = [rng.standard_normal((V, 24)).astype(np.float32) for _ in range(NS)]
+    fm = [lat.alloc_s() for _ in range(NS)]
+    for f, m in zip(fm, mds):
+        lat.put_s(f, m)
+    lib.dfl_subspace((P * NS)(*fm))
+    nb = lib.oqh_dfl_blocks(None, None)
+    assert nb == 16
+    inn = np.empty((nb, 8), np.int32)
+    bpts = np.empty((nb, 512), np.int32)
+    lib.oqh_dfl_blocks(inn.ctypes.data_as(P), bpts.ctypes.data_as(P))
+    basis_o, _ = od.block_gram_schmidt([m.astype(np.float64) for m in mds], list(bpts))
+    basis_g = np.zeros((NS, V, 12), complex)
+    for k in range(NS):
+        b = np.empty((V, 24), np.float32)
+        lib.oqh_dfl_export_basis(k, b.ctypes.data_as(P))
+        basis_g[k] = od.cplx(b)
+    dev = np.abs(basis_g - basis_o).max()
+    print("large blocks: basis max deviation", dev, flush=True)
+    assert dev < 2e-6 * np.abs(basis_o).max()
+    mu = 0.02
+    lib.set_Aw(mu)
+    A = od.little_operator(o, basis_g, list(bpts), mu)
+    ablk = np.empty((nb, NS, NS), np.complex128)
+    lib.oqh_dfl_export_Aw(ablk.ctypes.data_as(P), None)
+    scale = np.abs(A).max()
+    worst = max(np.abs(ablk[n] - A[n * NS:(n + 1) * NS, n * NS:(n + 1) * NS]).max() for n in range(nb))
+    print("large blocks: Ablk max deviation / scale", worst / scale, flush=True)
+    assert worst < 1e-11 * scale
+    for f in fm:
+        lat.free(f)
+    lib.oqh_dfl_release()
+
+
+@pytest.mark.gpu
+def test_renormalize_modes(setup):
+    """dfl_renormalize_modes (dfl_subspace.c:261-276): the modes become
+    orthonormal (FP32 storage: 1e-6), each mode l lies in the span of the
+    original modes 0..l (the upper-triangular factor of a QR decomposition,
+    against numpy's QR with the phases of its diagonal removed)."""
+    o, eta, s32, lat = setup
+    lib, V = lat.lib, lat.V
+    lib.oqh_dfl_release()
+    bs = np.array(BS, np.int32)
+    assert lib.set_dfl_parms(bs.ctypes.data_as(P), 8).Ns == 8
+    rng = np.random.default_rng(33)
+    mds = [rng.standard_normal((V, 24)).astype(np.float32) for _ in range(8)]
+    fm = [lat.alloc_s() for _ in range(8)]
+    for f, m in zip(fm, mds):
+        lat.put_s(f, m)
+    lib.dfl_renormalize_modes((P * 8)(*fm))
+    got = np.stack([od.cplx(lat.get_s(f)).ravel() for f in fm], axis=1)
+    gram = got.conj().T @ got
+    print("renormalize: |G - 1| =", np.abs(gram - np.eye(8)).max(), flush=True)
+    assert np.abs(gram - np.eye(8)).max() < 1e-5
+    M = np.stack([od.cplx(m.astype(np.float64)).ravel() for m in mds], axis=1)
+    Q, R = np.linalg.qr(M)
+    ph = np.diag(R) / np.abs(np.diag(R))
+    Q = Q * ph[None, :]
+    print("renormalize: |Q - QR| =", np.abs(got - Q).max(), flush=True)
+    assert np.abs(got - Q).max() < 1e-4 * np.abs(Q).max()
+    for f in fm:
+        lat.free(f)
+    lib.oqh_dfl_release()
