@@ -37,7 +37,11 @@ sflds/unorm su3fcts/chexp su3fcts/su3prod su3fcts/su3ren su3fcts/cm3x3 su3fcts/r
 sw_term/pauli sw_term/pauli_dble sw_term/swalg sw_term/swexp sw_term/swflds sw_term/sw_term
 tcharge/ftcom tcharge/ftensor uflds/plaq_sum uflds/shift uflds/udcom uflds/uflds
 uflds/uinit utils/endian utils/error utils/mutils utils/qsum utils/utils utils/wspace
-forces/force0 mdflds/mdflds mdflds/fcom wflow/wflow uflds/bstap archive/archive archive/sarchive""".split()
+forces/force0 mdflds/mdflds mdflds/fcom wflow/wflow uflds/bstap archive/archive archive/sarchive
+dfl/dfl_subspace dfl/dfl_modes dfl/dfl_sap_gcr dfl/ltl_gcr little/Aw little/Aw_blk little/Aw_com1
+little/Aw_com2 little/Aw_dble little/Aw_gen little/Aw_hop little/Aw_ops little/ltl_modes vflds/vcom
+vflds/vdcom vflds/vflds vflds/vinit linalg/cmatrix linalg/valg linalg/valg_dble linsolv/fgcr4vd
+linsolv/gcr4v""".split()
 
 # tag: (L[4], Ltrd[4], simd)   simd=False -> portable C branches (parity)
 #                              simd=True  -> -DAVX -DFMA3 -DPM (timing)

@@ -12,7 +12,12 @@
  *                                      sums, export_cnfg(OUT)
  *   ref_<tag> gcr [options]            time sap_gcr on the reference's own
  *                                      field and source, print JSON
+ *   ref_<tag> dfl [options]            dfl_modes and dfl_sap_gcr on the same
+ *                                      field and source, print JSON
  * Options: -bc T -cF a b -csw c -m0 m -seed s -mu mu -eoflg e
+ *          -dbs b0 b1 b2 b3 -Ns n (deflation subspace), -dpro nkv nmx res
+ *          nmx_gcr res_gcr (little solver), -dgen kappa mu ninv nmr ncy
+ *          (subspace generation)
  *          -bs b0 b1 b2 b3 -isolv i -nmr n -ncy n -nkv n -nmx n -res r
  *          -secs t (bench: target seconds per operator)
  *          -flow n eps (dump: plaquette force of the random field to frc0,
@@ -46,6 +51,9 @@
 #include "forces.h"
 #include "wflow.h"
 #include "archive.h"
+#include "vflds.h"
+#include "little.h"
+#include "dfl.h"
 #include "global.h"
 
 static char outdir[512];
@@ -73,8 +81,16 @@ int main(int argc, char *argv[])
    if (argc < 2) { fprintf(stderr, "usage: %s dump OUTDIR|bench [options]\n", argv[0]); return 1; }
    i = 2;
    const char *cnfg_in = NULL, *cnfg_out = NULL;
-   int gcr_mode = 0;
-   if (strcmp(argv[1], "gcr") == 0)
+   int gcr_mode = 0, dfl_mode = 0;
+   int dbs[4] = {4, 4, 4, 4}, dNs = 8, dnkv = 16, dnmx = 256, dnmx_gcr = 24, dninv = 6, dnmr = 4, dncy = 5;
+   double dres = 1.0e-2, dres_gcr = 1.0e-2, dkappa = 0.125, dmu = 0.0;
+   if (strcmp(argv[1], "dfl") == 0)
+   {
+      gcr_mode = 1;
+      dfl_mode = 1;
+      dump = 0;
+   }
+   else if (strcmp(argv[1], "gcr") == 0)
    {
       /* SAP grid parameters are set below as in dump mode */
       gcr_mode = 1;
@@ -114,6 +130,18 @@ int main(int argc, char *argv[])
       else if (!strcmp(argv[i], "-res")) res = atof(argv[++i]);
       else if (!strcmp(argv[i], "-secs")) secs = atof(argv[++i]);
       else if (!strcmp(argv[i], "-flow")) { nflow = atoi(argv[++i]); eflow = atof(argv[++i]); }
+      else if (!strcmp(argv[i], "-dbs")) { for (int k = 0; k < 4; k++) dbs[k] = atoi(argv[++i]); }
+      else if (!strcmp(argv[i], "-Ns")) dNs = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-dpro"))
+      {
+         dnkv = atoi(argv[++i]); dnmx = atoi(argv[++i]); dres = atof(argv[++i]);
+         dnmx_gcr = atoi(argv[++i]); dres_gcr = atof(argv[++i]);
+      }
+      else if (!strcmp(argv[i], "-dgen"))
+      {
+         dkappa = atof(argv[++i]); dmu = atof(argv[++i]); dninv = atoi(argv[++i]);
+         dnmr = atoi(argv[++i]); dncy = atoi(argv[++i]);
+      }
    }
 
    mpi_init(argc, argv);
@@ -127,7 +155,15 @@ int main(int argc, char *argv[])
       set_sap_parms(bs, isolv, nmr, ncy);
 
    alloc_wsd(10);
-   alloc_ws(2 * nkv + 9);
+   alloc_ws(2 * nkv + 9 + (dfl_mode ? dNs + 4 : 0));
+   if (dfl_mode)
+   {
+      set_dfl_parms(dbs, dNs);
+      set_dfl_pro_parms(dnkv, dnmx, dres, dnmx_gcr, dres_gcr);
+      set_dfl_gen_parms(dkappa, dmu, dninv, dnmr, dncy);
+      alloc_wv(2 * dnmx_gcr + 3);
+      alloc_wvd(2 * dnkv + 6);
+   }
    spinor_dble **wsd = reserve_wsd(8);
    spinor **ws = reserve_ws(8);
 
@@ -158,6 +194,26 @@ int main(int argc, char *argv[])
       }
       random_sd(VOLUME_TRD, 2, wsd[0], 1.0);
       bnd_sd2zero(ALL_PTS, wsd[0]);
+      if (dfl_mode)
+      {
+         int dif[2] = {0, 0}, dst[6] = {0, 0, 0, 0, 0, 0};
+         double t0 = wtime();
+         dfl_modes(dif, dst);
+         t1 = wtime();
+         rho = dfl_sap_gcr(nkv, nmx, 0, res, mu, wsd[0], wsd[1], dif + 1, dst + 2);
+         t2 = wtime();
+         Dw_dble(mu, wsd[1], wsd[2]);
+         mulr_spinor_add_dble(VOLUME_TRD, 2, wsd[2], wsd[0], -1.0);
+         q1 = norm_square_dble(VOLUME_TRD, 3, wsd[2]);
+         q0 = norm_square_dble(VOLUME_TRD, 3, wsd[0]);
+         rq = sqrt(q1.q[0] / q0.q[0]);
+         printf("{\"nthread\": %d, \"volume\": %d, \"modes_ifail\": %d, \"modes_status\": [%d, %d], "
+                "\"dfl_ifail\": %d, \"dfl_status\": [%d, %d, %d], \"dfl_rho\": %.6e, "
+                "\"true_residual\": %.6e, \"modes_sec\": %.6e, \"dfl_sec\": %.6e}\n",
+                NTHREAD, VOLUME, dif[0], dst[0], dst[1], dif[1], dst[2], dst[3], dst[4], rho, rq, t1 - t0, t2 - t1);
+         MPI_Finalize();
+         return 0;
+      }
       t1 = wtime();
       rho = sap_gcr(nkv, nmx, 0, res, mu, wsd[0], wsd[1], &ifail, &status);
       t2 = wtime();

@@ -11,6 +11,10 @@ ref_8x8x8x8_*.npz: per-point samples and norms of the reference build's
 outputs (oracle/_ref, see oracle/build_ref.py) for the 8^4 configurations, so
 that the oracle can be checked against the reference where the reference
 build is absent.
+
+ref_16x8x8x8_dfl.json: Krylov counts and true residuals of the reference
+build's SAP+GCR and deflated SAP+GCR on one field and source (the GPU test
+of the deflated solver compares its counts with them).
 """
 import json
 import os
@@ -87,8 +91,40 @@ def flow_samples():
                             ud_flow=udf[links], ud_flow_norm=np.linalg.norm(udf))
 
 
+DFL_CASE = dict(L=(16, 8, 8, 8), bc=3, m0=-0.5, csw=1.77, seed=4, flow=(50, 0.01), bs=(4, 4, 4, 4), isolv=1,
+                nmr=4, ncy=5, nkv=16, nmx=256, res=1e-10, dbs=(4, 4, 4, 4), Ns=10,
+                dpro=(16, 256, 1e-2, 24, 1e-2), dgen=(1.0 / (2.0 * (-0.5 + 4.0)), 0.0, 6, 4, 5))
+
+
+def dfl_counts():
+    """Krylov counts of the reference build's SAP+GCR and deflated SAP+GCR
+    (dfl_modes, then dfl_sap_gcr) on its random field of seed 4 (16x8^3,
+    50 flow steps, periodic bc with the quark phase) and random source."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_16x8x8x8_t8")
+    if not os.path.exists(exe):
+        print("reference build not present")
+        return
+    c = DFL_CASE
+    common = ["-bc", str(c["bc"]), "-m0", repr(c["m0"]), "-csw", repr(c["csw"]), "-seed", str(c["seed"]),
+              "-flow", str(c["flow"][0]), repr(c["flow"][1]), "-bs"] + [str(b) for b in c["bs"]] + \
+             ["-isolv", str(c["isolv"]), "-nmr", str(c["nmr"]), "-ncy", str(c["ncy"]), "-nkv", str(c["nkv"]),
+              "-nmx", str(c["nmx"]), "-res", repr(c["res"])]
+    dfl = ["-dbs"] + [str(b) for b in c["dbs"]] + ["-Ns", str(c["Ns"]), "-dpro"] + \
+          [repr(v) for v in c["dpro"]] + ["-dgen"] + [repr(v) for v in c["dgen"]]
+    env = dict(os.environ, OMP_NUM_THREADS="8")
+    out = {"case": c}
+    for mode, extra in (("gcr", []), ("dfl", dfl)):
+        r = subprocess.run([exe, mode] + common + extra, capture_output=True, text=True, env=env, timeout=900,
+                           check=True)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
+        out[mode] = json.loads(line)
+    json.dump(out, open(os.path.join(HERE, "ref_16x8x8x8_dfl.json"), "w"), indent=1)
+
+
 if __name__ == "__main__":
     kat()
     ref_samples()
     flow_samples()
+    dfl_counts()
     print("fixtures written")

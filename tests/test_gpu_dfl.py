@@ -20,6 +20,8 @@ the 64 lanes, the case that a wrong ordering of the elimination breaks):
   oracle reaches 1e-10 (devel/sap/check3.c judges the reference so), with
   fewer Krylov vectors than SAP+GCR without deflation."""
 import ctypes as C
+import json
+import os
 
 import numpy as np
 import pytest
@@ -217,6 +219,7 @@ def test_deflated_solver(setup):
     lib.dfl_modes(ifail.ctypes.data_as(P), st.ctypes.data_as(P))
     print("dfl_modes ifail", ifail[0], "status", st[:2].tolist(), flush=True)
     assert ifail[0] == 0
+    gen_status = st[:2].copy()
     e, p = lat.alloc_sd(), lat.alloc_sd()
     lat.put_sd(e, eta)
     plain_if, plain_st = C.c_int(), C.c_int()
@@ -233,6 +236,18 @@ def test_deflated_solver(setup):
     lib.oqh_set_sap_prec(0)
     assert counts[0] < plain_st.value
     assert np.array_equal(lat.get_sd(e), eta)
+    # the reference build on the same field and source
+    # (tests/golden/ref_16x8x8x8_dfl.json): SAP+GCR within one Krylov vector,
+    # the little solver's counts during the generation equal, and the
+    # deflated count within 20 % (the subspaces start from different random
+    # fields: ranlxs there, Philox here)
+    ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                      "ref_16x8x8x8_dfl.json")))
+    print("reference: plain", ref["gcr"]["gcr_status"], "modes", ref["dfl"]["modes_status"], "deflated",
+          ref["dfl"]["dfl_status"], flush=True)
+    assert abs(plain_st.value - ref["gcr"]["gcr_status"]) <= 1
+    assert list(gen_status) == ref["dfl"]["modes_status"]
+    assert abs(counts[0] - ref["dfl"]["dfl_status"][0]) <= 0.2 * ref["dfl"]["dfl_status"][0]
     # one more deflated inverse-iteration step on the restored modes, its
     # Schwarz cycles on FP16 fields
     lib.oqh_set_sap_prec(1)

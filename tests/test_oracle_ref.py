@@ -172,3 +172,30 @@ def test_flow_matches_reference(tag, bc, tmp_path):
     assert np.array_equal(o.plaq_frc(), l1("frc0.f64", np.float64, (4 * V, 8)))
     o.fwd_euler(4, 0.02)
     assert np.array_equal(o.ud(), l1("ud_flow.f64", np.float64, (4 * V, 18)))
+
+
+def test_reference_deflation_fixture(tmp_path):
+    """The reference build's SAP+GCR and deflated SAP+GCR counts recorded in
+    tests/golden/ref_16x8x8x8_dfl.json (used by the GPU deflation test) are
+    those the current reference build gives, and both solves reach the
+    requested residual."""
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(os.path.dirname(here), "oracle", "_ref", "ref_16x8x8x8_t8")
+    if not os.path.exists(exe):
+        pytest.skip("reference build not present")
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_golden
+    ref = json.load(open(os.path.join(here, "golden", "ref_16x8x8x8_dfl.json")))
+    assert ref["case"] == json.loads(json.dumps(make_golden.DFL_CASE))
+    keep = make_golden.HERE
+    try:
+        make_golden.HERE = str(tmp_path)
+        make_golden.dfl_counts()
+        new = json.load(open(os.path.join(str(tmp_path), "ref_16x8x8x8_dfl.json")))
+    finally:
+        make_golden.HERE = keep
+    for mode, keys in (("gcr", ("gcr_status", "gcr_ifail")), ("dfl", ("modes_status", "dfl_status", "dfl_ifail"))):
+        for k in keys:
+            assert new[mode][k] == ref[mode][k], (mode, k)
+    assert ref["gcr"]["true_residual"] <= 1.01e-10 and ref["dfl"]["true_residual"] <= 1.01e-10
