@@ -236,6 +236,18 @@ __device__ inline double2 wg_sum(double2 v, double2 *sh)
 }
 
 // (a, b) of two spinors, a FP32 promoted, b of type C
+// sum over the 64 lanes of a wave
+__device__ inline double2 wave_red2(double2 d)
+{
+#pragma unroll
+   for (int m = 32; m >= 1; m >>= 1)
+   {
+      d.x += __shfl_xor(d.x, m);
+      d.y += __shfl_xor(d.y, m);
+   }
+   return d;
+}
+
 template <typename C> __device__ inline double2 sdot(const float2 *a, const C *b)
 {
    double2 z = make_double2(0.0, 0.0);
@@ -277,6 +289,43 @@ __global__ __launch_bounds__(DWG) void k_blk_proj(const float2 *basis, size_t fu
    }
 }
 
+// the same for blocks of at most DWG points: the field is read once into
+// registers and the Ns products are reduced per wave, then across the waves
+template <typename C>
+__global__ __launch_bounds__(DWG) void k_blk_proj_reg(const float2 *basis, size_t fu, int Ns, const int *bpts,
+                                                       int bvol, const C *F, double2 *w)
+{
+   __shared__ double2 sh[DWG / 64][NS_MAX];
+   const int n = blockIdx.x, i = threadIdx.x, wv = threadIdx.x >> 6;
+   const bool act = i < bvol;
+   const int x = act ? bpts[(size_t)n * bvol + i] : 0;
+   C f[12];
+   if (act) load_spinor(F, x, f);
+   for (int k = 0; k < Ns; k++)
+   {
+      double2 d = make_double2(0.0, 0.0);
+      if (act)
+      {
+         float2 v[12];
+         load_spinor(basis + (size_t)k * fu, x, v);
+         d = sdot(v, f);
+      }
+      d = wave_red2(d);
+      if ((threadIdx.x & 63) == 0) sh[wv][k] = d;
+   }
+   __syncthreads();
+   for (int k = threadIdx.x; k < Ns; k += DWG)
+   {
+      double2 z = make_double2(0.0, 0.0);
+      for (int q = 0; q < DWG / 64; q++)
+      {
+         z.x += sh[q][k].x;
+         z.y += sh[q][k].y;
+      }
+      w[(size_t)n * Ns + k] = z;
+   }
+}
+
 // s(x) = sum_k (float) w[n(x)][k] B_k(x) (dfl_vd2s: single-precision
 // arithmetic as mulc_spinor_add)
 __global__ __launch_bounds__(DWG) void k_blk_prolong(const float2 *basis, size_t fu, int Ns, const int *bpts,
@@ -304,12 +353,9 @@ __global__ __launch_bounds__(DWG) void k_blk_prolong(const float2 *basis, size_t
    }
 }
 
-// block Gram-Schmidt of mode l (dfl_subspace): S = mds_l in double, S -=
-// (B_j, S) B_j for j < l (modified Gram-Schmidt), B_l = S / |S|; the
-// coordinates of mds_l in the basis go to vm[n][0..Ns) (complex float)
-// the same for blocks of at most DWG points: one point per lane, the FP64
-// work spinor in registers, so that only the basis vectors are read (twice
-// per previous vector, the second time from the cache)
+// block Gram-Schmidt of mode l for blocks of at most DWG points: one point
+// per lane, the FP64 work spinor in registers, so that only the basis vectors
+// are read (twice per previous vector, the second time from the cache)
 __global__ __launch_bounds__(DWG) void k_blk_gs_reg(float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
                                                      const float2 *mds, int l, float2 *vmds, int *degen)
 {
@@ -367,6 +413,9 @@ __global__ __launch_bounds__(DWG) void k_blk_gs_reg(float2 *basis, size_t fu, in
    }
 }
 
+// block Gram-Schmidt of mode l (dfl_subspace): S = mds_l in double, S -=
+// (B_j, S) B_j for j < l (modified Gram-Schmidt), B_l = S / |S|; the
+// coordinates of mds_l in the basis go to vm[n][0..Ns) (complex float)
 __global__ __launch_bounds__(DWG) void k_blk_gs(float2 *basis, size_t fu, int Ns, const int *bpts, int bvol,
                                                  const float2 *mds, double2 *S, int l, float2 *vmds, int *degen)
 {
@@ -579,17 +628,6 @@ __device__ inline double2 aw_face_dot(const float2 *b, const double2 *c0, const 
       z.y += h0[c].x * c0[c].y - h0[c].y * c0[c].x + h1[c].x * c1[c].y - h1[c].y * c1[c].x;
    }
    return z;
-}
-
-__device__ inline double2 wave_red2(double2 d)
-{
-#pragma unroll
-   for (int m = 32; m >= 1; m >>= 1)
-   {
-      d.x += __shfl_xor(d.x, m);
-      d.y += __shfl_xor(d.y, m);
-   }
-   return d;
 }
 
 // column l of Ablk and of the eight Ahop[ifc] of block n in one pass over
@@ -1663,7 +1701,10 @@ namespace {
 bool s2vd_dev(const float2 *s, double2 *vd)
 {
    Lib &L = lib();
-   k_blk_proj<float2><<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, s, vd);
+   if (D.bvol <= DWG)
+      k_blk_proj_reg<float2><<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, s, vd);
+   else
+      k_blk_proj<float2><<<D.nb, DWG, 0, L.stream>>>(D.d_basis, D.fu, D.Ns, D.d_bpts, D.bvol, s, vd);
    return true;
 }
 
