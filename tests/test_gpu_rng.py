@@ -106,3 +106,55 @@ def test_start_field_philox():
         lat.free(s)
     finally:
         lat.close()
+
+
+PAR = [dict(name="12^4-one-stream-level2", L=(12, 12, 12, 12), T=None, bc=3, level=1, seed=3),
+       dict(name="16^4-open-nthread16", L=(16, 16, 16, 16), T=(8, 8, 8, 8), bc=0, level=0, seed=21),
+       dict(name="12^4-open-nthread81", L=(12, 12, 12, 12), T=(4, 4, 4, 4), bc=0, level=0, seed=8)]
+
+
+def _draws(lat, c):
+    """random_ud, random_sd of every thread, the ms4 random source on time
+    slice 1, random_sd of the master thread, random_sd again (FP32) and 50
+    ranlxd numbers, all from one start_ranlux."""
+    P = ctypes.c_void_p
+    lat.lib.start_ranlux(c["level"], c["seed"])
+    lat.lib.random_ud()
+    out = [lat.export_ud()]
+    s = lat.alloc_sd()
+    lat.lib.random_sd(lat.VT, 2, s, 1.0)
+    out.append(lat.get_sd(s))
+    lat.lib.oqh_random_source(1, s)
+    out.append(lat.get_sd(s))
+    lat.lib.random_sd(77, 0, s, 0.5)
+    out.append(lat.get_sd(s)[:77])
+    lat.lib.random_ud()
+    out.append(lat.export_ud())
+    r = np.empty(50)
+    lat.lib.ranlxd(r.ctypes.data_as(P), 50)
+    out.append(r)
+    lat.free(s)
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", PAR, ids=lambda c: c["name"])
+def test_parallel_streams_match_serial(c, monkeypatch):
+    """The parallel generation (every refill of every stream reached by a
+    jump, the links of random_ud assigned along their chain of draws) hands
+    out the same numbers and leaves the streams in the same state as the
+    walk of each stream by one lane (OQH_RNG_SERIAL=1), bit for bit."""
+    print("config", c, flush=True)
+    lat = pyoqh.Lattice(c["L"], c["T"], bc=c["bc"])
+    try:
+        monkeypatch.setenv("OQH_RNG_SERIAL", "1")
+        ser = _draws(lat, c)
+        monkeypatch.delenv("OQH_RNG_SERIAL")
+        par = _draws(lat, c)
+        names = ["random_ud", "random_sd", "source", "random_sd master", "random_ud again", "ranlxd"]
+        for k, (a, b) in enumerate(zip(ser, par)):
+            print(names[k], "max deviation", np.abs(a - b).max(), "equal", np.array_equal(a, b), flush=True)
+        for k, (a, b) in enumerate(zip(ser, par)):
+            assert np.array_equal(a, b), names[k]
+    finally:
+        lat.close()
