@@ -558,17 +558,20 @@ template <typename S> struct RegBlock
    bool bp;    // open-bc boundary point
 };
 
-// out = (A + i mu) v on the two Weyl halves, A from registers (pauli_half
-// with unit stride)
+// out = (A + i mu) v on the two Weyl halves (pauli_half); the half hh = 0
+// of A is read at m0 with stride st0, the half hh = 1 at m1 with stride st1
+// (registers or LDS)
 template <typename S>
-__device__ inline void pauli_reg(const S *m, float mu, const float2 *v, float2 *out, int stride = 1)
+__device__ inline void pauli_reg2(const S *m0, int st0, const S *m1, int st1, float mu, const float2 *v, float2 *out)
 {
 #pragma unroll
    for (int hh = 0; hh < 2; hh++)
    {
       float2 q[18];
+      const S *m = hh ? m1 : m0;
+      const int stride = hh ? st1 : st0;
 #pragma unroll
-      for (int k = 0; k < 18; k++) q[k] = to_f2(m[(18 * hh + k) * stride]);
+      for (int k = 0; k < 18; k++) q[k] = to_f2(m[k * stride]);
       const float2 *w = v + 6 * hh;
       float2 *o = out + 6 * hh;
       float mm = hh ? -mu : mu;
@@ -587,6 +590,12 @@ __device__ inline void pauli_reg(const S *m, float mu, const float2 *v, float2 *
             p++;
          }
    }
+}
+
+template <typename S>
+__device__ inline void pauli_reg(const S *m, float mu, const float2 *v, float2 *out, int stride = 1)
+{
+   pauli_reg2(m, stride, m + 18 * stride, stride, mu, v, out);
 }
 
 // OUT: parity of the points whose terms are needed (0 even, 1 odd, 2 all);
@@ -837,9 +846,18 @@ __device__ inline void pull_faces(const SArgs &a, const RegBlock<S> &b, const S 
 #define OQH_SAP_WAVES16 2
 #endif
 #ifndef OQH_SAP_WAVES32
-#define OQH_SAP_WAVES32 1
+#define OQH_SAP_WAVES32 2
 #endif
-// FP16 storage frees enough registers for two wavefronts per SIMD
+#ifndef OQH_SAP_MHALF
+#define OQH_SAP_MHALF 1
+#endif
+
+// Two wavefronts per SIMD (256 registers per lane and at most 80 KiB of LDS
+// per work-group): the FP16 variant keeps its links in FP16 registers and
+// its Pauli matrices in LDS; the FP32 variant keeps the first Pauli matrix
+// of each point in LDS and the second in registers (OQH_SAP_MHALF; with
+// both in registers it needs 310 registers and runs one wavefront per SIMD,
+// 1.50 against 0.82 ms per colour at 48^4).
 // MASKED: open-bc boundary points exist (bc != 3); without them the masking
 // of boundary values is compiled out
 template <int ISOLV, typename S, bool MASKED>
@@ -849,9 +867,13 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
 {
    __shared__ float2 LB[18 * RBV];  // hopping exchange (rb_hop)
    // FP16 variant: the Pauli matrices of the block live in LDS, and psi,
-   // rho, s0g are FP16 spinors with per-site scales
+   // rho, s0g are FP16 spinors with per-site scales; FP32 variant with
+   // OQH_SAP_MHALF: the first Pauli matrix of each point in LDS, the second
+   // in registers
    constexpr bool MLDS = sizeof(S) == 4;
-   __shared__ S MS[MLDS ? 36 : 1][RBV];
+   constexpr bool MHALF = !MLDS && OQH_SAP_MHALF;
+   constexpr bool MALL = MLDS;
+   __shared__ S MS[MALL ? 36 : (MHALF ? 18 : 1)][RBV];
    // several sources: consecutive work-groups take the same block for
    // different sources, so its links and SW matrices come from the cache
    const int src = a0.nsrc ? (int)(blockIdx.x % a0.nsrc) : 0;
@@ -890,14 +912,16 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
 #pragma unroll
       for (int k = 0; k < 36; k++)
       {
-         if (MLDS)
+         if (MALL || (MHALF && k < 18))
             MS[k][x] = m[k * SB];
          else
             b.m[k] = m[k * SB];
       }
    }
-   const S *mp = MLDS ? &MS[0][x] : b.m;
-   const int mst = MLDS ? RBV : 1;
+   const S *mp0 = (MALL || MHALF) ? &MS[0][x] : b.m;
+   const int mst0 = (MALL || MHALF) ? RBV : 1;
+   const S *mp1 = MALL ? &MS[18][x] : b.m + 18;
+   const int mst1 = MALL ? RBV : 1;
    float2 s0[12], s1[12], s2[12], acc[12];
    if (MLDS)
       load_h16(a.rho16, a.rsc, gx, s1);
@@ -919,7 +943,7 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
       if (!even)
       {
          float2 o[12];
-         pauli_reg(mp, 0.0f, s1, o, mst);
+         pauli_reg2(mp0, mst0, mp1, mst1, 0.0f, s1, o);
 #pragma unroll
          for (int c = 0; c < 12; c++) s1[c] = b.bp ? make_float2(0.0f, 0.0f) : o[c];
       }
@@ -943,7 +967,7 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
          {
 #pragma unroll
             for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
-            pauli_reg(mp, 0.0f, acc, s2, mst);
+            pauli_reg2(mp0, mst0, mp1, mst1, 0.0f, acc, s2);
             if (b.bp)
             {
 #pragma unroll
@@ -953,7 +977,7 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
          rb_hop<0>(b, !even, s2, acc, LB);
          if (even)
          {
-            pauli_reg(mp, a.mu, s1, s2, mst);
+            pauli_reg2(mp0, mst0, mp1, mst1, a.mu, s1, s2);
 #pragma unroll
             for (int c = 0; c < 12; c++)
                s2[c] = b.bp ? make_float2(0.0f, 0.0f) : cadd(s2[c], cscale(acc[c], 0.5f));
@@ -962,7 +986,7 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
       else
       {
          rb_hop<2>(b, !even, s1, acc, LB);
-         pauli_reg(mp, even ? a.mu : a.muo, s1, s2, mst);
+         pauli_reg2(mp0, mst0, mp1, mst1, even ? a.mu : a.muo, s1, s2);
 #pragma unroll
          for (int c = 0; c < 12; c++)
             s2[c] = b.bp ? make_float2(0.0f, 0.0f) : cadd(s2[c], cscale(acc[c], -0.5f));
@@ -993,7 +1017,7 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
          float2 o[12];
 #pragma unroll
          for (int c = 0; c < 12; c++) acc[c] = cscale(acc[c], -0.5f);
-         pauli_reg(mp, 0.0f, acc, o, mst);
+         pauli_reg2(mp0, mst0, mp1, mst1, 0.0f, acc, o);
 #pragma unroll
          for (int c = 0; c < 12; c++)
          {
@@ -1623,8 +1647,9 @@ static void sap_push_faces(const SArgs &a, int c)
 // colour 0); finish: complete rho at the end (faces of colour 0).
 static void sap_cycle(const SArgs &a, int isolv, bool first, bool finish)
 {
-   // the FP32 register kernel runs one wavefront per SIMD, too few to hide
-   // the latency of the scattered face loads: it keeps the face kernel
+   // the FP32 register kernel keeps the face kernel: pulling the face terms
+   // (scattered loads at the start of every block solve) measured no faster
+   // at two wavefronts per SIMD (2.54 vs 2.49 ms per cycle at 48^4)
    if (half_variant() && !a.nsrc)
    {
       sap_blocks(a, 0, isolv, !first);
