@@ -16,8 +16,9 @@
 // leave.
 //
 // k_ptc: a work-group takes 128 points.  For two directions p at a time its
-// threads transport the input channels of the neighbours with the FP32 links
-// and store them as BF16 in LDS (a K x N matrix, K = real and imaginary
+// threads transport the input channels of the neighbours with the links (FP32
+// in training, an FP16 copy in the preconditioner, where the links are most
+// of the bytes a layer reads) and store them as BF16 in LDS (a K x N matrix, K = real and imaginary
 // parts of the C_in channels, N = 3 colours x 128 points); the channel
 // mixing is then a real GEMM [2 C_out x K] x [K x N] on the matrix cores
 // (v_mfma_f32_32x32x16_bf16, FP32 accumulation over the 9 directions), the
@@ -26,6 +27,7 @@
 // W'[p] = W[pbar]^+ (T_p^+ = T_pbar); the weight gradient
 // dW[p][o][i] = sum_{x,a} g(x)[o][a] conj((T_p in)(x)[i][a]) is formed in
 // FP32 per work-group and summed in double.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -53,7 +55,8 @@ __device__ inline int pbar_of(int p) { return p == 0 ? 0 : ((p & 1) ? p + 1 : p 
 
 // address of the link of the transport T_p at point x (neighbour y), and
 // whether it enters daggered
-__device__ inline const float2 *ptc_link(const float2 *u, int x, int y, int p, int h, bool &dag)
+template <typename UL>
+__device__ inline const UL *ptc_link(const UL *u, int x, int y, int p, int h, bool &dag)
 {
    const int mu = (p - 1) >> 1;
    const bool fwd = p & 1;
@@ -106,14 +109,18 @@ __device__ inline void st_act(bf16x2 *p, size_t i, float2 v)
    p[i] = b;
 }
 
+// links: FP32, or the FP16 copy that the preconditioner reads
+__device__ inline float2 lnk(float2 v) { return v; }
+__device__ inline float2 lnk(__half2 v) { return __half22float2(v); }
+
 // out (+)= layer(in); wp: packed BF16 weights [9][32][kpad(CI)].  Two
 // directions per phase (five phases): the two halves of the work-group
 // transport all input channels of their point for direction 2q and 2q+1,
 // so every link is loaded once, and one barrier pair serves two directions.
 // accum (out += ...) only with FP32 output.
-template <int CI, int CO, typename TI = float2, typename TO = float2>
+template <int CI, int CO, typename TI = float2, typename TO = float2, typename UL = float2>
 __global__ __launch_bounds__(PWG) void k_ptc(const TI *__restrict__ in, TO *__restrict__ out,
-                                             const __bf16 *__restrict__ wp, const float2 *__restrict__ u,
+                                             const __bf16 *__restrict__ wp, const UL *__restrict__ u,
                                              const int *__restrict__ nbr, int V, int accum)
 {
    constexpr int KP = kpad(CI);
@@ -142,9 +149,9 @@ __global__ __launch_bounds__(PWG) void k_ptc(const TI *__restrict__ in, TO *__re
          bool dag = false;
          if (p > 0)
          {
-            const float2 *up = ptc_link(u, x, y, p, h, dag);
+            const UL *up = ptc_link(u, x, y, p, h, dag);
 #pragma unroll
-            for (int k = 0; k < 9; k++) um[k] = up[k * SB];
+            for (int k = 0; k < 9; k++) um[k] = lnk(up[k * SB]);
          }
          // all loads first (independent), then the products
          float2 v[CI][3];
@@ -398,6 +405,12 @@ __global__ void k_adam(float *w, const float *g, float *m, float *v, int n, floa
 // host side
 // ---------------------------------------------------------------------------
 
+__global__ void k_ptc_link16(const float2 *a, __half2 *b, size_t n)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i < n) b[i] = __float22half2_rn(a[i]);
+}
+
 namespace {
 
 struct Net
@@ -417,6 +430,8 @@ struct Net
    double *d_lpart = nullptr, *h_lpart = nullptr;
    float2 *d_tmp = nullptr;         // spinor work field of the preconditioner
    bf16x2 *d_inf[2] = {};           // BF16 activations of the preconditioner (ping-pong)
+   __half2 *d_u16 = nullptr;        // FP16 links of the preconditioner
+   int u16_epoch = -1;              // link epoch of d_u16
 };
 
 Net N;
@@ -429,7 +444,7 @@ void free_net()
    for (int l = 0; l < NLMAX; l++) { dev_free(N.d_pf[l]); dev_free(N.d_pa[l]); }
    for (int l = 0; l <= NLMAX; l++) dev_free(N.d_act[l]);
    dev_free(N.d_gb[0]); dev_free(N.d_gb[1]); dev_free(N.d_part); dev_free(N.d_lpart); dev_free(N.d_tmp);
-   dev_free(N.d_inf[0]); dev_free(N.d_inf[1]);
+   dev_free(N.d_inf[0]); dev_free(N.d_inf[1]); dev_free(N.d_u16);
    if (N.h_lpart) HIP_OK(hipHostFree(N.h_lpart));
    N = Net();
 }
@@ -456,7 +471,9 @@ bool ensure_fields()
    for (int l = 0; l <= NLMAX; l++) { dev_free(N.d_act[l]); N.d_act[l] = nullptr; }
    for (int k = 0; k < 2; k++) { dev_free(N.d_gb[k]); N.d_gb[k] = nullptr; }
    for (int k = 0; k < 2; k++) { dev_free(N.d_inf[k]); N.d_inf[k] = nullptr; }
-   dev_free(N.d_part); dev_free(N.d_tmp);
+   dev_free(N.d_part); dev_free(N.d_tmp); dev_free(N.d_u16);
+   N.d_u16 = nullptr;
+   N.u16_epoch = -1;
    N.V = L.g.V;
    for (int l = 1; l < N.nl; l++) N.d_act[l] = (float2 *)dev_alloc(units_for(N.V, 3 * CMAX) * sizeof(float2));
    for (int k = 0; k < 2; k++) N.d_gb[k] = (float2 *)dev_alloc(units_for(N.V, 3 * CMAX) * sizeof(float2));
@@ -464,7 +481,8 @@ bool ensure_fields()
    N.d_part = (float2 *)dev_alloc((size_t)nb * NDIR * CMAX * CMAX * sizeof(float2));
    N.d_tmp = (float2 *)dev_alloc(units_for(N.V, 12) * sizeof(float2));
    for (int k = 0; k < 2; k++) N.d_inf[k] = (bf16x2 *)dev_alloc(units_for(N.V, 3 * CMAX) * sizeof(bf16x2));
-   return N.d_gb[0] && N.d_gb[1] && N.d_part && N.d_tmp && N.d_inf[0] && N.d_inf[1];
+   N.d_u16 = (__half2 *)dev_alloc(units_for(N.V / 2, 72) * sizeof(__half2));
+   return N.d_gb[0] && N.d_gb[1] && N.d_part && N.d_tmp && N.d_inf[0] && N.d_inf[1] && N.d_u16;
 }
 
 template <int CI, int CO>
@@ -483,14 +501,19 @@ void launch_inf(const void *in, void *out, const __bf16 *wp, int mode)
    Lib &L = lib();
    const int V = L.g.V;
    const unsigned nb = (V + PNS - 1) / PNS;
+   const __half2 *u = N.d_u16;
    if (mode == 0)
-      k_ptc<CI, CO, float2, bf16x2><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (bf16x2 *)out, wp, L.d_u, L.d_nbr, V, 0);
+      k_ptc<CI, CO, float2, bf16x2, __half2><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (bf16x2 *)out, wp, u,
+                                                                       L.d_nbr, V, 0);
    else if (mode == 1)
-      k_ptc<CI, CO, bf16x2, bf16x2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (bf16x2 *)out, wp, L.d_u, L.d_nbr, V, 0);
+      k_ptc<CI, CO, bf16x2, bf16x2, __half2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (bf16x2 *)out, wp, u,
+                                                                       L.d_nbr, V, 0);
    else if (mode == 2)
-      k_ptc<CI, CO, bf16x2, float2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (float2 *)out, wp, L.d_u, L.d_nbr, V, 1);
+      k_ptc<CI, CO, bf16x2, float2, __half2><<<nb, PWG, 0, L.stream>>>((const bf16x2 *)in, (float2 *)out, wp, u,
+                                                                       L.d_nbr, V, 1);
    else
-      k_ptc<CI, CO><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (float2 *)out, wp, L.d_u, L.d_nbr, V, 1);
+      k_ptc<CI, CO, float2, float2, __half2><<<nb, PWG, 0, L.stream>>>((const float2 *)in, (float2 *)out, wp, u,
+                                                                       L.d_nbr, V, 1);
 }
 
 template <int CI, int CO> void launch_wgrad(const float2 *in, const float2 *g)
@@ -633,6 +656,16 @@ bool nn_correct(const float2 *r, float2 *phi)
 {
    if (!N.nl || !ensure_fields()) return false;
    pack();
+   Lib &L = lib();
+   if (N.u16_epoch != L.u_epoch)
+   {
+      // the transports of the preconditioner read the links in FP16 (half
+      // the bytes of the layer's dominant stream; the BF16 channel mixing
+      // rounds more coarsely)
+      const size_t nu = units_for(L.g.V / 2, 72);
+      k_ptc_link16<<<(unsigned)((nu + 255) / 256), 256, 0, L.stream>>>(L.d_u, N.d_u16, nu);
+      N.u16_epoch = L.u_epoch;
+   }
    const void *x = r;
    for (int l = 0; l < N.nl; l++)
    {
