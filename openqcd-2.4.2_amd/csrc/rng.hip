@@ -801,6 +801,19 @@ bool ensure_jtab()
    return true;
 }
 
+// device memory for the parallel path, or nullptr (the per-stream walk is
+// then used) if the device cannot provide it
+char *try_alloc(size_t bytes)
+{
+   void *p = nullptr;
+   if (hipMalloc(&p, bytes) != hipSuccess)
+   {
+      (void)hipGetLastError();
+      return nullptr;
+   }
+   return (char *)p;
+}
+
 // one device allocation cut into aligned pieces
 struct Carve
 {
@@ -837,8 +850,8 @@ void produce(int nstr, double *U, long long nu, int *bad)
 }
 
 // random_ud with the streams jumped: 0 done, 1 the per-stream walk is needed
-// (a link that spans more than the chunk window, or the groups ran out
-// before the last link), -1 error
+// (a link that spans more than the chunk window, the groups ran out before
+// the last link, or no device memory for the buffers), -1 error
 int random_ud_parallel()
 {
    Lib &L = lib();
@@ -861,8 +874,8 @@ int random_ud_parallel()
                 oCN = cv.take(sizeof(int) * nstr * (size_t)nch * W), oCB = cv.take(sizeof(int) * nstr * (size_t)nch),
                 oCE = cv.take(sizeof(int) * nstr * (size_t)nch), oS = cv.take(sizeof(int) * nstr * (size_t)sst),
                 ocons = cv.take(sizeof(long long) * nstr), obad = cv.take(sizeof(int));
-   cv.base = (char *)dev_alloc(cv.total);
-   if (!cv.base) return -1;
+   cv.base = try_alloc(cv.total);
+   if (!cv.base) return 1;
    double *U = cv.at<double>(oU);
    unsigned char *acc = cv.at<unsigned char>(oacc);
    int *A0 = cv.at<int>(oA0), *A1 = cv.at<int>(oA1), *rank = cv.at<int>(ork), *NL = cv.at<int>(oNL);
@@ -909,8 +922,8 @@ template <typename C> int random_field_parallel(int nstr, int vol, C *dev, int o
    Carve cv;
    const size_t oU = cv.take(sizeof(double) * nstr * nu), ocons = cv.take(sizeof(long long) * nstr),
                 obad = cv.take(sizeof(int));
-   cv.base = (char *)dev_alloc(cv.total);
-   if (!cv.base) return -1;
+   cv.base = try_alloc(cv.total);
+   if (!cv.base) return 1;
    double *U = cv.at<double>(oU);
    long long *cons = cv.at<long long>(ocons);
    int *bad = cv.at<int>(obad);
@@ -946,8 +959,8 @@ int random_source_parallel(double2 *dev, int t0)
    const size_t oU = cv.take(sizeof(double) * nstr * nu), ork = cv.take(sizeof(int) * (size_t)L.g.V),
                 oNL = cv.take(sizeof(int) * nstr), ocons = cv.take(sizeof(long long) * nstr),
                 obad = cv.take(sizeof(int));
-   cv.base = (char *)dev_alloc(cv.total);
-   if (!cv.base) return -1;
+   cv.base = try_alloc(cv.total);
+   if (!cv.base) return 1;
    double *U = cv.at<double>(oU);
    int *rank = cv.at<int>(ork), *NL = cv.at<int>(oNL), *bad = cv.at<int>(obad);
    long long *cons = cv.at<long long>(ocons);
