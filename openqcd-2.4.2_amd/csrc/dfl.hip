@@ -631,13 +631,13 @@ __device__ inline double2 aw_face_dot(const float2 *b, const double2 *c0, const 
 }
 
 // column l of Ablk and of the eight Ahop[ifc] of block n in one pass over
-// the block's points: Y (site term and in-block hops of B_l) and the up to
-// four face terms of each point are formed once; every basis vector B_k is
-// then read twice per point (once for Y, once for the faces) instead of
-// once per face
+// the block's points: Y (site term and in-block hops of B_l, kept in LDS)
+// and the up to four face terms of each point (in registers) are formed
+// once; every basis vector B_k is then read once per point for both
 __global__ __launch_bounds__(DWG) void k_aw_column(AwArgs a, const int *blk, int l)
 {
    __shared__ double2 part[DWG / 64][9][NS_MAX];
+   __shared__ double2 Ysh[12][DWG];  // Y of the lane's point (read back by the same lane)
    const int n = blockIdx.x;
    const int *pts = a.bpts + (size_t)n * a.bvol;
    const int Ns = a.Ns;
@@ -669,58 +669,57 @@ __global__ __launch_bounds__(DWG) void k_aw_column(AwArgs a, const int *blk, int
             aw_dir<2>(a, blk, n, l, x, -1, v);
             aw_dir<3>(a, blk, n, l, x, -1, v);
          }
-         for (int k = 0; k < Ns; k++)
-         {
-            double2 d = make_double2(0.0, 0.0);
-            if (act)
-            {
-               float2 b[12];
-               load_spinor(a.basis + (size_t)k * a.fu, x, b);
-               d = sdot(b, v);
-            }
-            d = wave_red2(d);
-            if (lead)
-            {
-               part[w][0][k].x += d.x;
-               part[w][0][k].y += d.y;
-            }
-         }
+#pragma unroll
+         for (int c = 0; c < 12; c++) Ysh[c][threadIdx.x] = v[c];
       }
-      // the face terms
+      // the face terms, then one pass over the basis vectors for both
+      double2 f0[4][3], f1[4][3];
+      int fs[4] = {-1, -1, -1, -1};
+      if (act)
       {
-         double2 f0[4][3], f1[4][3];
-         int fs[4] = {-1, -1, -1, -1};
+         fs[0] = aw_face<0>(a, blk, n, l, x, f0[0], f1[0]);
+         fs[1] = aw_face<1>(a, blk, n, l, x, f0[1], f1[1]);
+         fs[2] = aw_face<2>(a, blk, n, l, x, f0[2], f1[2]);
+         fs[3] = aw_face<3>(a, blk, n, l, x, f0[3], f1[3]);
+      }
+      bool anyf = false;
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) anyf = anyf || fs[mu] >= 0;
+      const bool wf = __any(anyf);
+      for (int k = 0; k < Ns; k++)
+      {
+         float2 b[12];
+         if (act) load_spinor(a.basis + (size_t)k * a.fu, x, b);
+         double2 d0 = make_double2(0.0, 0.0);
          if (act)
          {
-            fs[0] = aw_face<0>(a, blk, n, l, x, f0[0], f1[0]);
-            fs[1] = aw_face<1>(a, blk, n, l, x, f0[1], f1[1]);
-            fs[2] = aw_face<2>(a, blk, n, l, x, f0[2], f1[2]);
-            fs[3] = aw_face<3>(a, blk, n, l, x, f0[3], f1[3]);
+            double2 v[12];
+#pragma unroll
+            for (int c = 0; c < 12; c++) v[c] = Ysh[c][threadIdx.x];
+            d0 = sdot(b, v);
          }
-         bool anyf = false;
-#pragma unroll
-         for (int mu = 0; mu < 4; mu++) anyf = anyf || fs[mu] >= 0;
-         if (!__any(anyf)) continue;
-         for (int k = 0; k < Ns; k++)
+         d0 = wave_red2(d0);
+         if (lead)
          {
-            float2 b[12];
-            if (anyf) load_spinor(a.basis + (size_t)k * a.fu, x, b);
-            double2 d[4];
-            d[0] = (fs[0] >= 0) ? aw_face_dot<0>(b, f0[0], f1[0], fs[0] == 1) : make_double2(0.0, 0.0);
-            d[1] = (fs[1] >= 0) ? aw_face_dot<1>(b, f0[1], f1[1], fs[1] == 3) : make_double2(0.0, 0.0);
-            d[2] = (fs[2] >= 0) ? aw_face_dot<2>(b, f0[2], f1[2], fs[2] == 5) : make_double2(0.0, 0.0);
-            d[3] = (fs[3] >= 0) ? aw_face_dot<3>(b, f0[3], f1[3], fs[3] == 7) : make_double2(0.0, 0.0);
+            part[w][0][k].x += d0.x;
+            part[w][0][k].y += d0.y;
+         }
+         if (!wf) continue;
+         double2 d[4];
+         d[0] = (fs[0] >= 0) ? aw_face_dot<0>(b, f0[0], f1[0], fs[0] == 1) : make_double2(0.0, 0.0);
+         d[1] = (fs[1] >= 0) ? aw_face_dot<1>(b, f0[1], f1[1], fs[1] == 3) : make_double2(0.0, 0.0);
+         d[2] = (fs[2] >= 0) ? aw_face_dot<2>(b, f0[2], f1[2], fs[2] == 5) : make_double2(0.0, 0.0);
+         d[3] = (fs[3] >= 0) ? aw_face_dot<3>(b, f0[3], f1[3], fs[3] == 7) : make_double2(0.0, 0.0);
 #pragma unroll
-            for (int ifc = 0; ifc < 8; ifc++)
+         for (int ifc = 0; ifc < 8; ifc++)
+         {
+            const int mu = ifc >> 1;
+            double2 e = (fs[mu] == ifc) ? d[mu] : make_double2(0.0, 0.0);
+            e = wave_red2(e);
+            if (lead)
             {
-               const int mu = ifc >> 1;
-               double2 e = (fs[mu] == ifc) ? d[mu] : make_double2(0.0, 0.0);
-               e = wave_red2(e);
-               if (lead)
-               {
-                  part[w][1 + ifc][k].x += e.x;
-                  part[w][1 + ifc][k].y += e.y;
-               }
+               part[w][1 + ifc][k].x += e.x;
+               part[w][1 + ifc][k].y += e.y;
             }
          }
       }
