@@ -1876,20 +1876,38 @@ void sap_mixed_residual(float2 *r)
 
 // the Schwarz cycles of the FP16 preconditioner as one HIP graph: its kernel
 // arguments depend only on rho (fixed within fgcr), the SAP parameters and
-// the grid, so the graph is made once and replayed for every Krylov vector
-static struct
+// the grid, so the graph is made when the same arguments come a second time
+// in a row and replayed for every further Krylov vector; calls with
+// changing arguments (the fields of dfl_modes) run eagerly
+struct SapCall
 {
-   hipGraphExec_t exec = nullptr;
    SArgs a;
    const float2 *rho = nullptr;
    int isolv = -1, ncy = -1, want_res = -1;
+   bool same(const SapCall &o) const
+   {
+      return rho && rho == o.rho && isolv == o.isolv && ncy == o.ncy && want_res == o.want_res &&
+             memcmp(&a, &o.a, sizeof(a)) == 0;
+   }
+};
+static struct
+{
+   hipGraphExec_t exec = nullptr;
+   SapCall made;  // the arguments of exec
+   SapCall last;  // the arguments of the last eager call
 } SG;
 
 static void drop_sap_graph()
 {
-   if (SG.exec) HIP_OK(hipGraphExecDestroy(SG.exec));
+   if (SG.exec)
+   {
+      // a launch of the graph may still be running on the library stream
+      HIP_OK(hipStreamSynchronize(lib().stream));
+      HIP_OK(hipGraphExecDestroy(SG.exec));
+   }
    SG.exec = nullptr;
-   SG.rho = nullptr;
+   SG.made = SapCall();
+   SG.last = SapCall();
 }
 
 // rho -> FP16, then ncy cycles from psi = 0 (enqueued on the library stream);
@@ -1919,29 +1937,32 @@ bool sap_mixed_mop(float mu, int isolv, int nmr, int ncy, const float2 *rho, flo
    Lib &L = lib();
    const int V = L.g.V;
    SArgs a = sargs(mu, nmr);
-   if (OQH_SAP_GRAPH && L.sap_graph)
+   SapCall c;
+   memset(&c.a, 0, sizeof(c.a));  // padding included, for the byte comparison
+   c.a = a;
+   c.rho = rho;
+   c.isolv = isolv;
+   c.ncy = ncy;
+   c.want_res = (int)want_res;
+   if (OQH_SAP_GRAPH && L.sap_graph && SG.exec && SG.made.same(c))
+      HIP_OK(hipGraphLaunch(SG.exec, L.stream));
+   else if (OQH_SAP_GRAPH && L.sap_graph && SG.last.same(c))
    {
-      bool hit = SG.exec && SG.rho == rho && SG.isolv == isolv && SG.ncy == ncy && SG.want_res == (int)want_res &&
-                 memcmp(&SG.a, &a, sizeof(a)) == 0;
-      if (!hit)
-      {
-         drop_sap_graph();
-         hipGraph_t g;
-         HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
-         enqueue_mixed_cycles(a, isolv, ncy, rho, want_res);
-         HIP_OK(hipStreamEndCapture(L.stream, &g));
-         HIP_OK(hipGraphInstantiate(&SG.exec, g, nullptr, nullptr, 0));
-         HIP_OK(hipGraphDestroy(g));
-         SG.a = a;
-         SG.rho = rho;
-         SG.isolv = isolv;
-         SG.ncy = ncy;
-         SG.want_res = want_res;
-      }
+      drop_sap_graph();
+      hipGraph_t g;
+      HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+      enqueue_mixed_cycles(a, isolv, ncy, rho, want_res);
+      HIP_OK(hipStreamEndCapture(L.stream, &g));
+      HIP_OK(hipGraphInstantiate(&SG.exec, g, nullptr, nullptr, 0));
+      HIP_OK(hipGraphDestroy(g));
+      SG.made = c;
       HIP_OK(hipGraphLaunch(SG.exec, L.stream));
    }
    else
+   {
       enqueue_mixed_cycles(a, isolv, ncy, rho, want_res);
+      SG.last = c;
+   }
    k_h16_to_s32<<<(V + 255) / 256, 256, 0, L.stream>>>(G.d_h16[0], G.d_hsc[0], phi, V);
    return true;
 }
