@@ -1063,6 +1063,10 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
    }
 }
 
+#ifndef OQH_FACE_GROUP
+#define OQH_FACE_GROUP 4
+#endif
+
 // rho(y) -= hopping terms into y from the block corrections of the
 // neighbouring blocks, for the face points y of the blocks of one colour
 // (S = __half2: FP16 links, FP16 s0g and rho with per-site scales)
@@ -1081,48 +1085,70 @@ __global__ __launch_bounds__(256) void k_sap_faces(SArgs a0, const int *face, co
    if (k >= nf) return;
    int y = face[k];
    int m = fmask[k];
-   float2 acc[12], sp[12], u[9], h0[3], h1[3], c0[3], c1[3];
+   float2 acc[12], h0[3], h1[3], c0[3], c1[3];
    zero12(acc);
    int odd = y >= a.h;
    const S *ub = H ? u16 : (const S *)a.u;
-#define OQH_LDU(J, L)                                                                  \
+   // a point of a block of width >= 2 lies on at most one of the faces -mu,
+   // +mu: the loads of each direction are issued for all lanes at once (one
+   // neighbour and one link per direction and lane), OQH_FACE_GROUP
+   // directions ahead of the arithmetic, which adds the terms in the order
+   // mu = 0..3 (forward before backward)
+   constexpr int FG = OQH_FACE_GROUP;
+   for (int m0 = 0; m0 < 4; m0 += FG)
+   {
+      float2 sp[FG][12], u[FG][9];
+#pragma unroll
+      for (int j = 0; j < FG; j++)
+      {
+         const int MU = m0 + j;
+         const bool fw = m & (1 << (2 * MU)), bw = m & (2 << (2 * MU));
+         if (fw || bw)
+         {
+            const int dir = fw ? 2 * MU : 2 * MU + 1;
+            const int x = a.nbr[(size_t)dir * a.V + y] & 0x7fffffff;
+            if (H)
+               load_h16(a.s0g16, a.s0sc, x, sp[j]);
+            else
+               load_spinor((const float2 *)a.s0g, x, sp[j]);
+            // forward: U(y, mu) (odd y) or U(x, mu) stored at the odd x
+            const int J = odd ? y - a.h : x - a.h;
+            const int l = (odd == fw) ? 2 * MU : 2 * MU + 1;
+            const S *up = ub + (J >> 6) * (size_t)(72 * SB) + (J & (SB - 1)) + (size_t)(9 * l) * SB;
+#pragma unroll
+            for (int q = 0; q < 9; q++) u[j][q] = to_f2(up[q * SB]);
+         }
+      }
+#pragma unroll
+      for (int j = 0; j < FG; j++)
+      {
+         const int MU = m0 + j;
+         const bool fw = m & (1 << (2 * MU)), bw = m & (2 << (2 * MU));
+#define OQH_FACE_TERM(M)                                                               \
+   if (MU == M)                                                                       \
    {                                                                                  \
-      const S *up = ub + ((J) >> 6) * (size_t)(72 * SB) + ((J) & (SB - 1)) + (size_t)(9 * (L)) * SB; \
-      _Pragma("unroll") for (int q = 0; q < 9; q++) u[q] = to_f2(up[q * SB]);          \
+      if (fw)                                                                         \
+      {                                                                               \
+         spin_project<float2, M, true>(sp[j], h0, h1);                                \
+         su3_mul_f<false>(u[j], h0, c0);                                              \
+         su3_mul_f<false>(u[j], h1, c1);                                              \
+         spin_expand_add<float2, M, true>(c0, c1, acc);                               \
+      }                                                                               \
+      else if (bw)                                                                    \
+      {                                                                               \
+         spin_project<float2, M, false>(sp[j], h0, h1);                               \
+         su3_mul_f<true>(u[j], h0, c0);                                               \
+         su3_mul_f<true>(u[j], h1, c1);                                               \
+         spin_expand_add<float2, M, false>(c0, c1, acc);                              \
+      }                                                                               \
    }
-#define OQH_FACE(MU)                                                                   \
-   if (m & (1 << (2 * MU)))                                                           \
-   {                                                                                  \
-      int x = a.nbr[(size_t)(2 * MU) * a.V + y] & 0x7fffffff;                         \
-      if (H)                                                                          \
-         load_h16(a.s0g16, a.s0sc, x, sp);                                            \
-      else                                                                            \
-         load_spinor((const float2 *)a.s0g, x, sp);                                   \
-      if (odd) OQH_LDU(y - a.h, 2 * MU) else OQH_LDU(x - a.h, 2 * MU + 1)               \
-      spin_project<float2, MU, true>(sp, h0, h1);                                     \
-      su3_mul_f<false>(u, h0, c0);                                              \
-      su3_mul_f<false>(u, h1, c1);                                              \
-      spin_expand_add<float2, MU, true>(c0, c1, acc);                                 \
-   }                                                                                  \
-   if (m & (2 << (2 * MU)))                                                           \
-   {                                                                                  \
-      int x = a.nbr[(size_t)(2 * MU + 1) * a.V + y] & 0x7fffffff;                     \
-      if (H)                                                                          \
-         load_h16(a.s0g16, a.s0sc, x, sp);                                            \
-      else                                                                            \
-         load_spinor((const float2 *)a.s0g, x, sp);                                   \
-      if (odd) OQH_LDU(y - a.h, 2 * MU + 1) else OQH_LDU(x - a.h, 2 * MU)               \
-      spin_project<float2, MU, false>(sp, h0, h1);                                    \
-      su3_mul_f<true>(u, h0, c0);                                               \
-      su3_mul_f<true>(u, h1, c1);                                               \
-      spin_expand_add<float2, MU, false>(c0, c1, acc);                                \
+         OQH_FACE_TERM(0)
+         OQH_FACE_TERM(1)
+         OQH_FACE_TERM(2)
+         OQH_FACE_TERM(3)
+#undef OQH_FACE_TERM
+      }
    }
-   OQH_FACE(0)
-   OQH_FACE(1)
-   OQH_FACE(2)
-   OQH_FACE(3)
-#undef OQH_FACE
-#undef OQH_LDU
    float2 r[12];
    if (H)
       load_h16(a.rho16, a.rsc, y, r);
