@@ -60,9 +60,10 @@ def setup(request):
 
 @pytest.mark.gpu
 def test_block_operator_and_mres(setup):
+    """Dw_blk and blk_mres (Dw.c:1475-1582, blk_solv.c:1062-1186) against the
+    oracle on the first and the last block; with isolv=1 on the grid that
+    SAP prepares, whose odd block SW term is inverted, in both."""
     c, o, eta, s32, lat = setup
-    if c["isolv"]:
-        pytest.skip("block Dw/MinRes compared on the isolv=0 grid")
     assert o.sap_prepare() == 0
     s = lat.alloc_s()
     lat.put_s(s, s32)
