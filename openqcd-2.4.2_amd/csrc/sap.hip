@@ -793,51 +793,71 @@ __device__ inline void rb_sums(bool act, const float2 *s2, const float2 *s1, dou
 // rho(x) += 1/2 hopping terms into block point x from the corrections s0g
 // of the neighbouring blocks (the other colour), as k_sap_faces applies
 // them, taken here at the start of the block solve
+#ifndef OQH_PULL_GROUP
+#define OQH_PULL_GROUP 2
+#endif
 template <typename S>
 __device__ inline void pull_faces(const SArgs &a, const RegBlock<S> &b, const S *ub, int gx, float2 *s1)
 {
    constexpr bool H = sizeof(S) == 4;
    const bool odd = gx >= a.h;
-   float2 acc[12], sp[12], u[9], h0[3], h1[3], c0[3], c1[3];
+   float2 acc[12], h0[3], h1[3], c0[3], c1[3];
    zero12(acc);
-#define OQH_PULL_LDU(J, L)                                                             \
+   // as in k_sap_faces: one neighbour and one link load per direction,
+   // OQH_PULL_GROUP directions ahead of the arithmetic
+   constexpr int FG = OQH_PULL_GROUP;
+   for (int m0 = 0; m0 < 4; m0 += FG)
+   {
+      float2 sp[FG][12], u[FG][9];
+#pragma unroll
+      for (int j = 0; j < FG; j++)
+      {
+         const int MU = m0 + j;
+         const bool fw = b.fwd[MU] == RBV, bw = b.bwd[MU] == RBV;
+         if (fw || bw)
+         {
+            const int y = a.nbr[(size_t)(fw ? 2 * MU : 2 * MU + 1) * a.V + gx] & 0x7fffffff;
+            if (H)
+               load_h16(a.s0g16, a.s0sc, y, sp[j]);
+            else
+               load_spinor((const float2 *)a.s0g, y, sp[j]);
+            const int J = odd ? gx - a.h : y - a.h;
+            const int l = (odd == fw) ? 2 * MU : 2 * MU + 1;
+            const S *up = ub + (J >> 6) * (size_t)(72 * SB) + (J & (SB - 1)) + (size_t)(9 * l) * SB;
+#pragma unroll
+            for (int q = 0; q < 9; q++) u[j][q] = to_f2(up[q * SB]);
+         }
+      }
+#pragma unroll
+      for (int j = 0; j < FG; j++)
+      {
+         const int MU = m0 + j;
+         const bool fw = b.fwd[MU] == RBV, bw = b.bwd[MU] == RBV;
+#define OQH_PULL_TERM(M)                                                               \
+   if (MU == M)                                                                       \
    {                                                                                  \
-      const S *up = ub + ((J) >> 6) * (size_t)(72 * SB) + ((J) & (SB - 1)) + (size_t)(9 * (L)) * SB; \
-      _Pragma("unroll") for (int q = 0; q < 9; q++) u[q] = to_f2(up[q * SB]);          \
+      if (fw)                                                                         \
+      {                                                                               \
+         spin_project<float2, M, true>(sp[j], h0, h1);                                \
+         su3_mul_f<false>(u[j], h0, c0);                                              \
+         su3_mul_f<false>(u[j], h1, c1);                                              \
+         spin_expand_add<float2, M, true>(c0, c1, acc);                               \
+      }                                                                               \
+      else if (bw)                                                                    \
+      {                                                                               \
+         spin_project<float2, M, false>(sp[j], h0, h1);                               \
+         su3_mul_f<true>(u[j], h0, c0);                                               \
+         su3_mul_f<true>(u[j], h1, c1);                                               \
+         spin_expand_add<float2, M, false>(c0, c1, acc);                              \
+      }                                                                               \
    }
-#define OQH_PULL(MU)                                                                   \
-   if (b.fwd[MU] == RBV)                                                              \
-   {                                                                                  \
-      int y = a.nbr[(size_t)(2 * MU) * a.V + gx] & 0x7fffffff;                        \
-      if (H)                                                                          \
-         load_h16(a.s0g16, a.s0sc, y, sp);                                            \
-      else                                                                            \
-         load_spinor((const float2 *)a.s0g, y, sp);                                   \
-      if (odd) OQH_PULL_LDU(gx - a.h, 2 * MU) else OQH_PULL_LDU(y - a.h, 2 * MU + 1)    \
-      spin_project<float2, MU, true>(sp, h0, h1);                                     \
-      su3_mul_f<false>(u, h0, c0);                                                    \
-      su3_mul_f<false>(u, h1, c1);                                                    \
-      spin_expand_add<float2, MU, true>(c0, c1, acc);                                 \
-   }                                                                                  \
-   if (b.bwd[MU] == RBV)                                                              \
-   {                                                                                  \
-      int y = a.nbr[(size_t)(2 * MU + 1) * a.V + gx] & 0x7fffffff;                    \
-      if (H)                                                                          \
-         load_h16(a.s0g16, a.s0sc, y, sp);                                            \
-      else                                                                            \
-         load_spinor((const float2 *)a.s0g, y, sp);                                   \
-      if (odd) OQH_PULL_LDU(gx - a.h, 2 * MU + 1) else OQH_PULL_LDU(y - a.h, 2 * MU)    \
-      spin_project<float2, MU, false>(sp, h0, h1);                                    \
-      su3_mul_f<true>(u, h0, c0);                                                     \
-      su3_mul_f<true>(u, h1, c1);                                                     \
-      spin_expand_add<float2, MU, false>(c0, c1, acc);                                \
+         OQH_PULL_TERM(0)
+         OQH_PULL_TERM(1)
+         OQH_PULL_TERM(2)
+         OQH_PULL_TERM(3)
+#undef OQH_PULL_TERM
+      }
    }
-   OQH_PULL(0)
-   OQH_PULL(1)
-   OQH_PULL(2)
-   OQH_PULL(3)
-#undef OQH_PULL
-#undef OQH_PULL_LDU
 #pragma unroll
    for (int c = 0; c < 12; c++) s1[c] = cadd(s1[c], cscale(acc[c], 0.5f));
 }
@@ -927,7 +947,11 @@ void k_sap_reg(SArgs a0, const S *u16, const S *sw16)
       load_h16(a.rho16, a.rsc, gx, s1);
    else
       load_spinor((const float2 *)a.rho, gx, s1);
-   if (a.pull) pull_faces<S>(a, b, ub, gx, s1);
+   // only the FP16 variant pulls the face terms (the FP32 cycles use the
+   // face kernel); the pull is not compiled into the FP32 kernel, whose
+   // registers it would crowd
+   if constexpr (MLDS)
+      if (a.pull) pull_faces<S>(a, b, ub, gx, s1);
 #pragma unroll
    for (int c = 0; c < 12; c++)
    {
