@@ -37,7 +37,8 @@
 
 namespace oqh {
 
-constexpr int SWG = 256;  // work-group size of the block kernels
+constexpr int SWG = 256;      // work-group size of the block kernels
+constexpr int SWG_MAX = 1024; // work-group size of the generic FP32 block solver for blocks of >= 1024 points
 constexpr int NBF = 4;    // block fields of the API per block: s[0..2] + the Dwhat_blk intermediate
 #ifndef OQH_SAP_REG
 #define OQH_SAP_REG 1
@@ -188,7 +189,7 @@ template <typename C> __device__ inline void zero12(C *v)
 // work-group sum of two doubles, broadcast to all threads
 __device__ inline void wg_sum2(double &x, double &y)
 {
-   __shared__ double sh[2][SWG / 64];
+   __shared__ double sh[2][SWG_MAX / 64];
 #pragma unroll
    for (int m = 32; m >= 1; m >>= 1)
    {
@@ -204,7 +205,7 @@ __device__ inline void wg_sum2(double &x, double &y)
    __syncthreads();
    x = 0.0;
    y = 0.0;
-   for (int k = 0; k < SWG / 64; k++)
+   for (int k = 0; k < (int)(blockDim.x >> 6); k++)
    {
       x += sh[0][k];
       y += sh[1][k];
@@ -214,7 +215,7 @@ __device__ inline void wg_sum2(double &x, double &y)
 
 __device__ inline void wg_sum3(double &x, double &y, double &z)
 {
-   __shared__ double sh[3][SWG / 64];
+   __shared__ double sh[3][SWG_MAX / 64];
 #pragma unroll
    for (int m = 32; m >= 1; m >>= 1)
    {
@@ -231,7 +232,7 @@ __device__ inline void wg_sum3(double &x, double &y, double &z)
    }
    __syncthreads();
    x = y = z = 0.0;
-   for (int k = 0; k < SWG / 64; k++)
+   for (int k = 0; k < (int)(blockDim.x >> 6); k++)
    {
       x += sh[0][k];
       y += sh[1][k];
@@ -245,7 +246,7 @@ template <typename C>
 __device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0, int i1, decltype(C::x) mu_e)
 {
    typedef decltype(C::x) R;
-   for (int i = i0 + threadIdx.x; i < i1; i += SWG)
+   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
    {
       int gx = imb[i];
       C o[12];
@@ -273,7 +274,7 @@ template <typename C>
 __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T, decltype(C::x) mu)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
    {
       int gx = imb[i];
       C o[12];
@@ -292,7 +293,7 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T
       store_spinor(T, i, o);
    }
    __syncthreads();
-   for (int i = threadIdx.x; i < a.bh; i += SWG)
+   for (int i = threadIdx.x; i < a.bh; i += blockDim.x)
    {
       int gx = imb[i];
       C o[12];
@@ -321,7 +322,7 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
    typedef decltype(C::x) R;
    const R rmin = sizeof(R) == 4 ? (R)FLT_MIN : (R)DBL_MIN;
    int nv = EO ? a.bh : a.bvol;
-   for (int i = threadIdx.x; i < nv; i += SWG) zero_spinor(S0, i);
+   for (int i = threadIdx.x; i < nv; i += blockDim.x) zero_spinor(S0, i);
    __syncthreads();
    for (int imr = 0; imr < a.nmr; imr++)
    {
@@ -331,7 +332,7 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
          blk_dw(a, imb, S1, S2, 0, a.bvol, mu);
       __syncthreads();
       double rr = 0.0, zr = 0.0, zi = 0.0;
-      for (int i = threadIdx.x; i < nv; i += SWG)
+      for (int i = threadIdx.x; i < nv; i += blockDim.x)
       {
          const C *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
          const C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
@@ -357,7 +358,7 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
       if (r < (R)2 * rmin) break;
       r = (R)1 / r;
       C z = cmk<C>((R)zr * r, (R)zi * r);
-      for (int i = threadIdx.x; i < nv; i += SWG)
+      for (int i = threadIdx.x; i < nv; i += blockDim.x)
       {
          C *p0 = S0 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
          C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
@@ -379,7 +380,7 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
 template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const int *imb, C *S1)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
    {
       int gx = imb[i];
       C v[12], o[12];
@@ -393,7 +394,7 @@ template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const i
       store_spinor(S1, i, o);
    }
    __syncthreads();
-   for (int i = threadIdx.x; i < a.bh; i += SWG)
+   for (int i = threadIdx.x; i < a.bh; i += blockDim.x)
    {
       int gx = imb[i];
       C o[12];
@@ -417,7 +418,7 @@ template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const i
 template <typename C> __device__ void blk_eo_finish(const SArgsT<C> &a, const int *imb, C *S0, C *S1)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += SWG)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
    {
       int gx = imb[i];
       C o[12];
@@ -445,8 +446,8 @@ template <typename C> __device__ void blk_eo_finish(const SArgsT<C> &a, const in
 // the Schwarz sweep over the blocks of one colour
 // ---------------------------------------------------------------------------
 
-template <int ISOLV, typename C>
-__global__ __launch_bounds__(SWG) void k_sap_blocks(SArgsT<C> a)
+template <int ISOLV, typename C, int TPB = SWG>
+__global__ __launch_bounds__(TPB) void k_sap_blocks(SArgsT<C> a)
 {
    typedef decltype(C::x) R;
    int n = a.n0 + blockIdx.x;
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgsT<C> a)
    C *S0 = a.scr + (size_t)blockIdx.x * 3 * a.Ub;
    C *S1 = S0 + a.Ub, *S2 = S1 + a.Ub;
 
-   for (int i = threadIdx.x; i < a.bvol; i += SWG)
+   for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
    {
       int gx = imb[i];
       C v[12];
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(SWG) void k_sap_blocks(SArgsT<C> a)
    else
       blk_mres_dev<false>(a, imb, S0, S1, S2, S2, a.mu);
 
-   for (int i = threadIdx.x; i < a.bvol; i += SWG)
+   for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
    {
       int gx = imb[i];
       C s0[12], s1[12], p[12];
@@ -1209,7 +1210,7 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       const bool even = (op == BOP_DWHAT || op == BOP_EO_MRES || op == BOP_DWEE || op == BOP_DWOE);
       const bool odd = (op == BOP_DWOO || op == BOP_DWEO);
       C *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
-      for (int i = threadIdx.x; i < a.bvol; i += SWG)
+      for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
          if (a.bnd[imb[i]] && (!even || i < a.bh) && (!odd || i >= a.bh)) zero_spinor(In, i);
       __syncthreads();
    }
@@ -1225,7 +1226,7 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       // (A + i mu gamma5) on one parity (Dw.c:1584-1694); the fields may coincide
       const bool ee = op == BOP_DWEE;
       const R m = ee ? mu : a.muo;
-      for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += SWG)
+      for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += blockDim.x)
       {
          const int gx = imb[i];
          C v[12], o[12];
@@ -1245,7 +1246,7 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       // Dwoe_blk assigns D_oe s to the odd points, Dweo_blk subtracts D_eo s
       // from the even points; the output parity is never read
       const bool oe = op == BOP_DWOE;
-      for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += SWG)
+      for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += blockDim.x)
       {
          const int gx = imb[i];
          C acc[12], o[12];
@@ -1295,7 +1296,7 @@ __global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int
    {
       const int mu = ifc >> 1;
       const bool up = ifc & 1;
-      for (int j = fo.o[ifc] + threadIdx.x; j < fo.o[ifc + 1]; j += SWG)
+      for (int j = fo.o[ifc] + threadIdx.x; j < fo.o[ifc + 1]; j += blockDim.x)
       {
          const int x = ipp[j], gx = imb[x];
          float2 w[6];
@@ -1665,6 +1666,15 @@ static void sap_blocks(const SArgs &a0, int ic, int isolv, int pull)
          k_sap_reg<0, float2, true><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
       else
          k_sap_reg<0, float2, false><<<G.nbh, RBV, 0, L.stream>>>(a, nullptr, nullptr);
+   }
+   else if (G.bvol >= SWG_MAX)
+   {
+      // large blocks (6^4, 8^4): 1024 threads per block, the few blocks of a
+      // small lattice would otherwise leave most of the chip idle
+      if (isolv)
+         k_sap_blocks<1, float2, SWG_MAX><<<G.nbh, SWG_MAX, 0, L.stream>>>(a);
+      else
+         k_sap_blocks<0, float2, SWG_MAX><<<G.nbh, SWG_MAX, 0, L.stream>>>(a);
    }
    else if (isolv)
       k_sap_blocks<1><<<G.nbh, SWG, 0, L.stream>>>(a);
