@@ -906,6 +906,58 @@ __global__ __launch_bounds__(AWH) void k_aw_hop(const C *MT, const int *inn, int
    out[(size_t)b * Ns + k] = cmk<C>(sx, sy);
 }
 
+// the same in FP32 for an even Ns <= 32: lane kk of a group reads the
+// outputs 2 kk, 2 kk + 1 of a row with one 16-byte load; group g takes the
+// face g & 7 and the half g >> 3 of the sum over l, and the 16 partial sums
+// of an output are added in LDS in a fixed order
+__global__ __launch_bounds__(AWH) void k_aw_hop2(const float2 *MT, const int *inn, int Ns, int b0, int i0,
+                                                 const float2 *in, const float2 *add, float2 *out, int neg)
+{
+   __shared__ float4 part[16][16];
+   const int b = blockIdx.x, g = threadIdx.x >> 4, kk = threadIdx.x & 15;
+   const int f = g & 7, hf = g >> 3;
+   const size_t nm = (size_t)Ns * Ns;
+   const int n = b0 + b;
+   float s0x = 0.0f, s0y = 0.0f, s1x = 0.0f, s1y = 0.0f;
+   if (2 * kk < Ns)
+   {
+      const float2 *M = MT + ((size_t)n * 8 + f) * nm;
+      const float2 *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
+      const int lh = Ns >> 1, l0 = hf ? lh : 0, l1 = hf ? Ns : lh;
+      for (int l = l0; l < l1; l++)
+      {
+         const float4 m = *(const float4 *)(M + (size_t)l * Ns + 2 * kk);
+         const float2 x = v[l];
+         s0x += m.x * x.x - m.y * x.y;
+         s0y += m.x * x.y + m.y * x.x;
+         s1x += m.z * x.x - m.w * x.y;
+         s1y += m.z * x.y + m.w * x.x;
+      }
+   }
+   part[g][kk] = make_float4(s0x, s0y, s1x, s1y);
+   __syncthreads();
+   const int k = threadIdx.x;
+   if (k >= Ns) return;
+   float sx = 0.0f, sy = 0.0f;
+   for (int q = 0; q < 16; q++)
+   {
+      const float4 p = part[q][k >> 1];
+      sx += (k & 1) ? p.z : p.x;
+      sy += (k & 1) ? p.w : p.y;
+   }
+   if (neg)
+   {
+      sx = -sx;
+      sy = -sy;
+   }
+   if (add)
+   {
+      sx += add[(size_t)b * Ns + k].x;
+      sy += add[(size_t)b * Ns + k].y;
+   }
+   out[(size_t)b * Ns + k] = make_float2(sx, sy);
+}
+
 // out[b] = AinvT^T of block b0+b times in[b] (+ add)
 __global__ __launch_bounds__(64) void k_aw_diag(const double2 *AinvT, int Ns, int b0, const double2 *in, double2 *out)
 {
@@ -922,6 +974,15 @@ __global__ __launch_bounds__(64) void k_aw_diag(const double2 *AinvT, int Ns, in
 template <typename C> void ahat(const C *MT, const C *v, C *t, C *w)
 {
    Lib &L = lib();
+   if constexpr (sizeof(C) == sizeof(float2))
+   {
+      if (D.Ns % 2 == 0 && D.Ns <= 32)
+      {
+         k_aw_hop2<<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
+         k_aw_hop2<<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
+         return;
+      }
+   }
    k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
    k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
 }
