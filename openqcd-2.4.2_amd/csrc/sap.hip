@@ -187,7 +187,7 @@ template <typename C> __device__ inline void zero12(C *v)
 }
 
 // work-group sum of two doubles, broadcast to all threads
-__device__ inline void wg_sum2(double &x, double &y)
+__device__ inline void wg_sum2(double &x, double &y, int nt)
 {
    __shared__ double sh[2][SWG_MAX / 64];
 #pragma unroll
@@ -205,7 +205,7 @@ __device__ inline void wg_sum2(double &x, double &y)
    __syncthreads();
    x = 0.0;
    y = 0.0;
-   for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+   for (int k = 0; k < (nt >> 6); k++)
    {
       x += sh[0][k];
       y += sh[1][k];
@@ -213,7 +213,7 @@ __device__ inline void wg_sum2(double &x, double &y)
    __syncthreads();
 }
 
-__device__ inline void wg_sum3(double &x, double &y, double &z)
+__device__ inline void wg_sum3(double &x, double &y, double &z, int nt)
 {
    __shared__ double sh[3][SWG_MAX / 64];
 #pragma unroll
@@ -232,7 +232,7 @@ __device__ inline void wg_sum3(double &x, double &y, double &z)
    }
    __syncthreads();
    x = y = z = 0.0;
-   for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+   for (int k = 0; k < (nt >> 6); k++)
    {
       x += sh[0][k];
       y += sh[1][k];
@@ -243,10 +243,10 @@ __device__ inline void wg_sum3(double &x, double &y, double &z)
 
 // S2 = Dw_blk S1 on the points [i0, i1) of the block (Dw.c:1475)
 template <typename C>
-__device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0, int i1, decltype(C::x) mu_e)
+__device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0, int i1, decltype(C::x) mu_e, int nt)
 {
    typedef decltype(C::x) R;
-   for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+   for (int i = i0 + threadIdx.x; i < i1; i += nt)
    {
       int gx = imb[i];
       C o[12];
@@ -271,10 +271,10 @@ __device__ void blk_dw(const SArgsT<C> &a, const int *imb, C *S1, C *S2, int i0,
 // intermediate A_oo^-1 D_oe S1 (T = S2 inside the Schwarz cycles, a separate
 // field for the block API, which leaves the odd half of S2 unchanged)
 template <typename C>
-__device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T, decltype(C::x) mu)
+__device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T, decltype(C::x) mu, int nt)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += nt)
    {
       int gx = imb[i];
       C o[12];
@@ -293,7 +293,7 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T
       store_spinor(T, i, o);
    }
    __syncthreads();
-   for (int i = threadIdx.x; i < a.bh; i += blockDim.x)
+   for (int i = threadIdx.x; i < a.bh; i += nt)
    {
       int gx = imb[i];
       C o[12];
@@ -317,22 +317,22 @@ __device__ void blk_dwhat(const SArgsT<C> &a, const int *imb, C *S1, C *S2, C *T
 // minimal residual steps on points [0, nv) (blk_solv.c:1117-1186); T: odd
 // intermediate of Dwhat_blk (EO)
 template <bool EO, typename C>
-__device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C *S2, C *T, decltype(C::x) mu)
+__device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C *S2, C *T, decltype(C::x) mu, int nt)
 {
    typedef decltype(C::x) R;
    const R rmin = sizeof(R) == 4 ? (R)FLT_MIN : (R)DBL_MIN;
    int nv = EO ? a.bh : a.bvol;
-   for (int i = threadIdx.x; i < nv; i += blockDim.x) zero_spinor(S0, i);
+   for (int i = threadIdx.x; i < nv; i += nt) zero_spinor(S0, i);
    __syncthreads();
    for (int imr = 0; imr < a.nmr; imr++)
    {
       if (EO)
-         blk_dwhat(a, imb, S1, S2, T, mu);
+         blk_dwhat(a, imb, S1, S2, T, mu, nt);
       else
-         blk_dw(a, imb, S1, S2, 0, a.bvol, mu);
+         blk_dw(a, imb, S1, S2, 0, a.bvol, mu, nt);
       __syncthreads();
       double rr = 0.0, zr = 0.0, zi = 0.0;
-      for (int i = threadIdx.x; i < nv; i += blockDim.x)
+      for (int i = threadIdx.x; i < nv; i += nt)
       {
          const C *p2 = S2 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
          const C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
@@ -353,12 +353,12 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
             zi += (double)pi;
          }
       }
-      wg_sum3(rr, zr, zi);
+      wg_sum3(rr, zr, zi, nt);
       R r = (R)rr;
       if (r < (R)2 * rmin) break;
       r = (R)1 / r;
       C z = cmk<C>((R)zr * r, (R)zi * r);
-      for (int i = threadIdx.x; i < nv; i += blockDim.x)
+      for (int i = threadIdx.x; i < nv; i += nt)
       {
          C *p0 = S0 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
          C *p1 = S1 + (size_t)(i >> 6) * (12 * SB) + (i & (SB - 1));
@@ -377,10 +377,10 @@ __device__ void blk_mres_dev(const SArgsT<C> &a, const int *imb, C *S0, C *S1, C
 
 // isolv=1 preparation: S1_o = A_oo^-1 S1_o, then S1_e -= D_eo S1_o
 // (Dwoo_blk, Dweo_blk of sap.c:759-761)
-template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const int *imb, C *S1)
+template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const int *imb, C *S1, int nt)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += nt)
    {
       int gx = imb[i];
       C v[12], o[12];
@@ -394,7 +394,7 @@ template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const i
       store_spinor(S1, i, o);
    }
    __syncthreads();
-   for (int i = threadIdx.x; i < a.bh; i += blockDim.x)
+   for (int i = threadIdx.x; i < a.bh; i += nt)
    {
       int gx = imb[i];
       C o[12];
@@ -415,10 +415,10 @@ template <typename C> __device__ void blk_eo_prepare(const SArgsT<C> &a, const i
 }
 
 // isolv=1 completion: odd solution (Dwoe_blk, Dwoo_blk; update_flds1/2)
-template <typename C> __device__ void blk_eo_finish(const SArgsT<C> &a, const int *imb, C *S0, C *S1)
+template <typename C> __device__ void blk_eo_finish(const SArgsT<C> &a, const int *imb, C *S0, C *S1, int nt)
 {
    typedef decltype(C::x) R;
-   for (int i = a.bh + threadIdx.x; i < a.bvol; i += blockDim.x)
+   for (int i = a.bh + threadIdx.x; i < a.bvol; i += nt)
    {
       int gx = imb[i];
       C o[12];
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(TPB) void k_sap_blocks(SArgsT<C> a)
    C *S0 = a.scr + (size_t)blockIdx.x * 3 * a.Ub;
    C *S1 = S0 + a.Ub, *S2 = S1 + a.Ub;
 
-   for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
+   for (int i = threadIdx.x; i < a.bvol; i += TPB)
    {
       int gx = imb[i];
       C v[12];
@@ -469,15 +469,15 @@ __global__ __launch_bounds__(TPB) void k_sap_blocks(SArgsT<C> a)
 
    if (ISOLV)
    {
-      blk_eo_prepare(a, imb, S1);
-      blk_mres_dev<true>(a, imb, S0, S1, S2, S2, a.mu);
-      blk_eo_finish(a, imb, S0, S1);
+      blk_eo_prepare(a, imb, S1, TPB);
+      blk_mres_dev<true>(a, imb, S0, S1, S2, S2, a.mu, TPB);
+      blk_eo_finish(a, imb, S0, S1, TPB);
       __syncthreads();
    }
    else
-      blk_mres_dev<false>(a, imb, S0, S1, S2, S2, a.mu);
+      blk_mres_dev<false>(a, imb, S0, S1, S2, S2, a.mu, TPB);
 
-   for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
+   for (int i = threadIdx.x; i < a.bvol; i += TPB)
    {
       int gx = imb[i];
       C s0[12], s1[12], p[12];
@@ -788,7 +788,7 @@ __device__ inline void rb_sums(bool act, const float2 *s2, const float2 *s1, dou
          zi += (double)pi;
       }
    }
-   wg_sum3(rr, zr, zi);
+   wg_sum3(rr, zr, zi, RBV);
 }
 
 // rho(x) += 1/2 hopping terms into block point x from the corrections s0g
@@ -1210,23 +1210,23 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       const bool even = (op == BOP_DWHAT || op == BOP_EO_MRES || op == BOP_DWEE || op == BOP_DWOE);
       const bool odd = (op == BOP_DWOO || op == BOP_DWEO);
       C *In = (op == BOP_MRES || op == BOP_EO_MRES) ? F + a.Ub : Sk;
-      for (int i = threadIdx.x; i < a.bvol; i += blockDim.x)
+      for (int i = threadIdx.x; i < a.bvol; i += SWG)
          if (a.bnd[imb[i]] && (!even || i < a.bh) && (!odd || i >= a.bh)) zero_spinor(In, i);
       __syncthreads();
    }
    if (op == BOP_DW)
-      blk_dw(a, imb, Sk, Sl, 0, a.bvol, mu);
+      blk_dw(a, imb, Sk, Sl, 0, a.bvol, mu, SWG);
    else if (op == BOP_DWHAT)
    {
       // Dwhat_blk leaves the odd half of the output unchanged
-      blk_dwhat(a, imb, Sk, Sl, T, mu);
+      blk_dwhat(a, imb, Sk, Sl, T, mu, SWG);
    }
    else if (op == BOP_DWEE || op == BOP_DWOO)
    {
       // (A + i mu gamma5) on one parity (Dw.c:1584-1694); the fields may coincide
       const bool ee = op == BOP_DWEE;
       const R m = ee ? mu : a.muo;
-      for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += blockDim.x)
+      for (int i = (ee ? 0 : a.bh) + threadIdx.x; i < (ee ? a.bh : a.bvol); i += SWG)
       {
          const int gx = imb[i];
          C v[12], o[12];
@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       // Dwoe_blk assigns D_oe s to the odd points, Dweo_blk subtracts D_eo s
       // from the even points; the output parity is never read
       const bool oe = op == BOP_DWOE;
-      for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += blockDim.x)
+      for (int i = (oe ? a.bh : 0) + threadIdx.x; i < (oe ? a.bvol : a.bh); i += SWG)
       {
          const int gx = imb[i];
          C acc[12], o[12];
@@ -1272,9 +1272,9 @@ __global__ __launch_bounds__(SWG) void k_blk_op(SArgsT<C> a, int n, int op, int 
       }
    }
    else if (op == BOP_MRES)
-      blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu);
+      blk_mres_dev<false>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu, SWG);
    else
-      blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu);
+      blk_mres_dev<true>(a, imb, F, F + a.Ub, F + 2 * a.Ub, T, mu, SWG);
 }
 
 // Dw_bnd (dirac/Dw_bnd.c:801-1011): for the face points x of block n the
@@ -1296,7 +1296,7 @@ __global__ __launch_bounds__(SWG) void k_dw_bnd(SArgs a, int n, int k, const int
    {
       const int mu = ifc >> 1;
       const bool up = ifc & 1;
-      for (int j = fo.o[ifc] + threadIdx.x; j < fo.o[ifc + 1]; j += blockDim.x)
+      for (int j = fo.o[ifc] + threadIdx.x; j < fo.o[ifc + 1]; j += SWG)
       {
          const int x = ipp[j], gx = imb[x];
          float2 w[6];
