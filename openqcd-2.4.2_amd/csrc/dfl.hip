@@ -906,44 +906,61 @@ __global__ __launch_bounds__(AWH) void k_aw_hop(const C *MT, const int *inn, int
    out[(size_t)b * Ns + k] = cmk<C>(sx, sy);
 }
 
-// the same in FP32 for an even Ns <= 32: lane kk of a group reads the
-// outputs 2 kk, 2 kk + 1 of a row with one 16-byte load; group g takes the
-// face g & 7 and the half g >> 3 of the sum over l, and the 16 partial sums
-// of an output are added in LDS in a fixed order
-__global__ __launch_bounds__(AWH) void k_aw_hop2(const float2 *MT, const int *inn, int Ns, int b0, int i0,
-                                                 const float2 *in, const float2 *add, float2 *out, int neg)
+// the same for an even Ns <= 32: lane kk of a group forms the outputs
+// 2 kk, 2 kk + 1 from adjacent entries of a row (one 16-byte load in FP32);
+// group g takes the face g & 7 and the half g >> 3 of the sum over l, and
+// the 16 partial sums of an output are added in LDS in a fixed order
+template <typename C> struct Pair2
 {
-   __shared__ float4 part[16][16];
+   decltype(C::x) ax, ay, bx, by;
+};
+template <typename C>
+__global__ __launch_bounds__(AWH) void k_aw_hop2(const C *MT, const int *inn, int Ns, int b0, int i0, const C *in,
+                                                 const C *add, C *out, int neg)
+{
+   typedef decltype(C::x) R;
+   __shared__ Pair2<C> part[16][16];
    const int b = blockIdx.x, g = threadIdx.x >> 4, kk = threadIdx.x & 15;
    const int f = g & 7, hf = g >> 3;
    const size_t nm = (size_t)Ns * Ns;
    const int n = b0 + b;
-   float s0x = 0.0f, s0y = 0.0f, s1x = 0.0f, s1y = 0.0f;
+   R s0x = 0, s0y = 0, s1x = 0, s1y = 0;
    if (2 * kk < Ns)
    {
-      const float2 *M = MT + ((size_t)n * 8 + f) * nm;
-      const float2 *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
+      const C *M = MT + ((size_t)n * 8 + f) * nm;
+      const C *v = in + (size_t)(inn[8 * n + f] - i0) * Ns;
       const int lh = Ns >> 1, l0 = hf ? lh : 0, l1 = hf ? Ns : lh;
       for (int l = l0; l < l1; l++)
       {
-         const float4 m = *(const float4 *)(M + (size_t)l * Ns + 2 * kk);
-         const float2 x = v[l];
-         s0x += m.x * x.x - m.y * x.y;
-         s0y += m.x * x.y + m.y * x.x;
-         s1x += m.z * x.x - m.w * x.y;
-         s1y += m.z * x.y + m.w * x.x;
+         C m0, m1;
+         if constexpr (sizeof(C) == sizeof(float2))
+         {
+            const float4 m = *(const float4 *)(M + (size_t)l * Ns + 2 * kk);
+            m0 = cmk<C>(m.x, m.y);
+            m1 = cmk<C>(m.z, m.w);
+         }
+         else
+         {
+            m0 = M[(size_t)l * Ns + 2 * kk];
+            m1 = M[(size_t)l * Ns + 2 * kk + 1];
+         }
+         const C x = v[l];
+         s0x += m0.x * x.x - m0.y * x.y;
+         s0y += m0.x * x.y + m0.y * x.x;
+         s1x += m1.x * x.x - m1.y * x.y;
+         s1y += m1.x * x.y + m1.y * x.x;
       }
    }
-   part[g][kk] = make_float4(s0x, s0y, s1x, s1y);
+   part[g][kk] = {s0x, s0y, s1x, s1y};
    __syncthreads();
    const int k = threadIdx.x;
    if (k >= Ns) return;
-   float sx = 0.0f, sy = 0.0f;
+   R sx = 0, sy = 0;
    for (int q = 0; q < 16; q++)
    {
-      const float4 p = part[q][k >> 1];
-      sx += (k & 1) ? p.z : p.x;
-      sy += (k & 1) ? p.w : p.y;
+      const Pair2<C> &p = part[q][k >> 1];
+      sx += (k & 1) ? p.bx : p.ax;
+      sy += (k & 1) ? p.by : p.ay;
    }
    if (neg)
    {
@@ -955,7 +972,7 @@ __global__ __launch_bounds__(AWH) void k_aw_hop2(const float2 *MT, const int *in
       sx += add[(size_t)b * Ns + k].x;
       sy += add[(size_t)b * Ns + k].y;
    }
-   out[(size_t)b * Ns + k] = make_float2(sx, sy);
+   out[(size_t)b * Ns + k] = cmk<C>(sx, sy);
 }
 
 // out[b] = AinvT^T of block b0+b times in[b] (+ add)
@@ -971,20 +988,20 @@ __global__ __launch_bounds__(64) void k_aw_diag(const double2 *AinvT, int Ns, in
 }
 
 // Ahat v on the even blocks; t: work vector on the odd blocks
-template <typename C> void ahat(const C *MT, const C *v, C *t, C *w)
+// one hop pass over the blocks [b0, b0 + nbh) (k_aw_hop2 for an even Ns <= 32)
+template <typename C> void aw_hop(const C *MT, int b0, int i0, const C *in, const C *add, C *out, int neg)
 {
    Lib &L = lib();
-   if constexpr (sizeof(C) == sizeof(float2))
-   {
-      if (D.Ns % 2 == 0 && D.Ns <= 32)
-      {
-         k_aw_hop2<<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
-         k_aw_hop2<<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
-         return;
-      }
-   }
-   k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, D.nbh, 0, v, nullptr, t, 0);
-   k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, 0, D.nbh, t, v, w, 1);
+   if (D.Ns % 2 == 0 && D.Ns <= 32)
+      k_aw_hop2<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, b0, i0, in, add, out, neg);
+   else
+      k_aw_hop<C><<<D.nbh, AWH, 0, L.stream>>>(MT, D.d_inn, D.Ns, b0, i0, in, add, out, neg);
+}
+
+template <typename C> void ahat(const C *MT, const C *v, C *t, C *w)
+{
+   aw_hop<C>(MT, D.nbh, 0, v, nullptr, t, 0);
+   aw_hop<C>(MT, 0, D.nbh, t, v, w, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2064,7 +2081,7 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
    // u_o = Aoo^-1 eta_o, b = Aee^-1 eta_e - sum M_e u_o
    k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, D.nbh, eta + nv, u);
    k_aw_diag<<<D.nbh, 64, 0, L.stream>>>(D.d_AinvT, D.Ns, 0, eta, t);
-   k_aw_hop<double2><<<D.nbh, AWH, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, 0, D.nbh, u, t, b, 1);
+   aw_hop<double2>(D.d_MT, 0, D.nbh, u, t, b, 1);
    double bn2;
    vdot<double2>(b, b, nv, nullptr, nullptr, &bn2);
    const double rn0 = sqrt(bn2);
@@ -2132,7 +2149,7 @@ double ltl_solve(const double2 *eta, double2 *psi, int *ifail, int *status)
    if (D.lm_ok && rn0 > 0.0) vaxpy<double2>(x, x0, nv, 1.0, 0.0);
    // psi_e = x, psi_o = u_o - sum M_o x_e
    HIP_OK(hipMemcpyAsync(psi, x, sizeof(double2) * nv, hipMemcpyDeviceToDevice, L.stream));
-   k_aw_hop<double2><<<D.nbh, AWH, 0, L.stream>>>(D.d_MT, D.d_inn, D.Ns, D.nbh, 0, x, u, psi + nv, 1);
+   aw_hop<double2>(D.d_MT, D.nbh, 0, x, u, psi + nv, 1);
    return rn;
 }
 
