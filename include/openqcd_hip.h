@@ -97,9 +97,10 @@ void oqh_get_sap_parms(int *bs, int *isolv, int *nmr, int *ncy);
    precision. */
 void oqh_set_sap_prec(int prec);
 /* the ncy Schwarz cycles of the FP16 preconditioner of sap_gcr are captured
-   once per solve as a HIP graph and replayed for every Krylov vector (1,
-   default), or launched kernel by kernel (0); no counterpart in the
-   reference, the results are the same */
+   as a HIP graph when the same arguments come twice in a row (within a
+   solve) and replayed for every further Krylov vector (1, default), or
+   launched kernel by kernel (0); no counterpart in the reference, the
+   results are the same */
 void oqh_set_sap_graph(int on);
 /* SAP block-size autotuner (no counterpart in the reference): among the
    legal block sizes of sap_parms.c (even, >= 4, an even number of blocks per
